@@ -1,0 +1,174 @@
+/*
+ * mgw_hip.h -- C ABI of the MI355X (gfx950) screening + Sigma_c backend.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (MGW_OK = 0); on failure mgw_hip_last_error(ctx) gives the message, which
+ * the C++ side of an integration turns into
+ * std::runtime_error("mgw_hip_<fn> failed: <message>") like the reference's
+ * check_cuda / check_cublas helpers
+ * (src/workspace/device_screening_workspace.cu:25-43).
+ *
+ * Contexts are not thread-safe: one context per host thread, as the
+ * reference's device workspaces are one per MPI rank
+ * (BackendCapabilities.thread_safe = false, src/linalg/linalg_cublas.cu:208-214).
+ * All device work of a context runs on one HIP stream of default priority;
+ * every call returns after its results are on the host.
+ *
+ * Arrays are row-major (C order) like the reference's Matrix / Tensor4
+ * (include/types.hpp:56-59,182-185).  Complex values are passed as separate
+ * real and imaginary arrays.
+ */
+#ifndef MGW_HIP_H
+#define MGW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mgw_hip_ctx mgw_hip_ctx;
+
+enum mgw_status {
+    MGW_OK = 0,
+    MGW_ERR_ARG = 1,         /* bad argument or inconsistent dimensions          */
+    MGW_ERR_HIP = 2,         /* HIP runtime error                                */
+    MGW_ERR_NUMERIC = 3,     /* factorization breakdown (non-SPD dielectric)      */
+    MGW_ERR_UNSUPPORTED = 4, /* valid request this backend does not implement     */
+    MGW_ERR_MEMORY = 5,      /* device allocation failed / over the HBM budget    */
+    MGW_ERR_STATE = 6        /* call order (e.g. quadratic forms before compute_wc) */
+};
+
+enum mgw_precision {
+    MGW_PREC_FP64 = 0,       /* everything in FP64                                */
+    MGW_PREC_MIXED = 1       /* BF16 hi/lo split contraction on MFMA (FP32 acc),  */
+                             /* FP64 factorization, FP64 Sigma_c/Pade/QP          */
+};
+
+/* ---- library / context ------------------------------------------------ */
+
+/* Version string of the library. */
+const char* mgw_hip_version(void);
+
+/* Message of the last failed call on ctx; with ctx == NULL, of the last
+ * failed call on this host thread (e.g. a failed mgw_hip_create). */
+const char* mgw_hip_last_error(const mgw_hip_ctx* ctx);
+
+/* Number of visible HIP devices. */
+int mgw_hip_device_count(int* count);
+
+/* Device for an MPI-style local rank: (local_rank / tasks_per_gpu) % count.
+ * Replaces set_cuda_device_for_local_rank (include/linalg/linalg_cublas.hpp:46-48,
+ * src/linalg/linalg_cublas.cu:393-405). */
+int mgw_hip_device_for_local_rank(size_t local_rank, size_t tasks_per_gpu, int* device);
+
+/* Create / destroy a context bound to one device.  Replaces the construction
+ * of DeviceScreeningWorkspace + DevicePqPhPanelView
+ * (src/gw/sigma_c_driver.cpp:489-491). */
+int mgw_hip_create(int device, mgw_hip_ctx** out);
+int mgw_hip_destroy(mgw_hip_ctx* ctx);
+
+/* Device bytes currently held by the context
+ * (DeviceScreeningWorkspace::estimated_device_bytes,
+ * include/workspace/device_screening_workspace.hpp:39). */
+int mgw_hip_device_bytes(const mgw_hip_ctx* ctx, size_t* bytes);
+
+/* Cap on the device bytes a context may hold (0 = no cap).  Allocations past
+ * the cap fail with MGW_ERR_MEMORY; the hoisted path sizes its frequency
+ * batches to fit. */
+int mgw_hip_set_memory_budget(mgw_hip_ctx* ctx, size_t bytes);
+
+/* ---- DeviceScreeningWorkspace (include/workspace/device_screening_workspace.hpp:27-59) */
+
+/* Upload V_ph (n x n, row-major, V[jb, ia] = (jb|ia)); replaces the
+ * DeviceScreeningWorkspace(const MatrixReal& v_ph) constructor
+ * (src/workspace/device_screening_workspace.cu:289-310). */
+int mgw_hip_upload_vph(mgw_hip_ctx* ctx, const double* v_rowmajor, int32_t n);
+
+/* W_c = (I - diag(pi0) V)^-1 diag(pi0), kept resident; replaces
+ * DeviceScreeningWorkspace::compute_w_c (src/workspace/device_screening_workspace.cu:337-374).
+ * pi0_im may be NULL (all zero).  This backend requires Im(pi0) == 0 and
+ * pi0 <= 0 (eta == 0 with positive ph gaps); otherwise MGW_ERR_UNSUPPORTED. */
+int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0_im, int32_t n);
+
+/* Copy the resident W_c (n x n, row-major) to the host.  wc_im may be NULL. */
+int mgw_hip_download_wc(mgw_hip_ctx* ctx, double* wc_re, double* wc_im, int32_t n);
+
+/* q_j = x_j^T W_c x_j for a host panel X (n x nvec, row-major); replaces
+ * DeviceScreeningWorkspace::quadratic_forms_panel(const MatrixReal&)
+ * (src/workspace/device_screening_workspace.cu:419-436).  q_im may be NULL. */
+int mgw_hip_quadratic_forms_host(mgw_hip_ctx* ctx, const double* x_rowmajor, int32_t n,
+                                 int32_t nvec, double* q_re, double* q_im);
+
+/* ---- DevicePqPhPanelView (include/workspace/device_pq_ph_panel.hpp:67-99) */
+
+/* Upload the MO ERI (nmo^4, C order) and keep it resident in HBM; replaces
+ * DevicePqPhPanelView(..., FullEriResident) (src/workspace/device_pq_ph_panel.cu:204-252). */
+int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t nocc);
+
+/* Build the resident V_ph from the resident ERI on the device; replaces the
+ * host gather calculate_v_ph_matrix (src/gw/sigma.cpp:56-71) + upload. */
+int mgw_hip_build_vph_from_eri(mgw_hip_ctx* ctx);
+
+/* q[kk] = x_{p,k0+kk}^T W_c x_{p,k0+kk}, kk < width, with x_{pk}[ia] = (pk|ia)
+ * streamed from the resident ERI; replaces fill_panel + quadratic_forms_panel
+ * (src/workspace/device_pq_ph_panel.cu:284-314, device_screening_workspace.cu:376-417). */
+int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width,
+                            double* q_re, double* q_im);
+
+/* ---- hoisted G0W0 on the resident ERI (compute_sigma_c_with_backend's
+ *      device branch, src/gw/sigma_c_driver.cpp:471-514, and run_g0w0,
+ *      src/gw/gw.cpp:125-191) */
+
+typedef struct mgw_g0w0_job {
+    int32_t nmo, nocc;
+    int32_t nfreq;              /* transformed Gauss-Legendre points          */
+    int32_t npar;               /* Pade parameters (<= 64)                    */
+    int32_t nstates;            /* 0 = all states                             */
+    const int32_t* states;      /* nstates 0-based indices (ascending), or NULL */
+    const double* mo_energy;    /* nmo                                        */
+    const double* vxc_diag;     /* nmo: vxc(p, p)                             */
+    double fermi_energy;
+    double eta;                 /* must be 0 (MGW_ERR_UNSUPPORTED otherwise)  */
+    int32_t precision;          /* enum mgw_precision                         */
+} mgw_g0w0_job;
+
+typedef struct mgw_g0w0_out {
+    double* omega;              /* nfreq, may be NULL                         */
+    double* weight;             /* nfreq, may be NULL                         */
+    double* sigma_x_diag;       /* nmo, may be NULL                           */
+    double* sigma_c_re;         /* nstates x nfreq (row-major), may be NULL   */
+    double* sigma_c_im;         /* nstates x nfreq, may be NULL               */
+    double* qp;                 /* nmo, NaN for states not computed           */
+    int32_t* qp_iterations;     /* nmo, may be NULL                           */
+} mgw_g0w0_out;
+
+/* One molecule, all frequencies, on the ERI made resident by mgw_hip_upload_eri. */
+int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
+
+/* Phase times of the last mgw_hip_g0w0 call in milliseconds (HIP events on the
+ * context's stream): [0] V build, [1] dielectric build, [2] factorization
+ * (potrf + trtri), [3] contraction, [4] Sigma_c quadrature, [5] Pade + QP,
+ * [6] total.  n is the number of entries the caller provides (<= 8). */
+int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
+
+/* Diagnostic with no counterpart in the reference: C = A B^T for 64 x 64
+ * row-major A, B through the factorization kernels' FP64 MFMA tile (checks
+ * the operand / result lane layout on a new device). */
+int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, double* c);
+
+/* Transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:62-93). */
+int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight);
+
+/* HBM bytes mgw_hip_g0w0 needs for one molecule with nbatch frequencies
+ * resident at once (ERI + V + dielectric batch + contraction scratch);
+ * the admission estimate of the request loop. */
+size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t nstates,
+                              int32_t nbatch, int32_t precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGW_HIP_H */

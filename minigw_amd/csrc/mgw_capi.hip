@@ -1,0 +1,746 @@
+// C ABI of the MI355X screening + Sigma_c backend (include/mgw_hip.h) and the
+// host runtime behind it: device buffers, the frequency grid, batching of
+// imaginary frequencies, and the hoisted G0W0 driver.
+#include "mgw_hip.h"
+
+#include "mgw_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_thread_error;
+
+struct MgwError : std::runtime_error {
+    int code;
+    MgwError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw MgwError(code, buf);
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(MGW_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+struct mgw_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    size_t budget = 0;     // 0 = no cap
+    size_t held = 0;
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // grow-only device buffer with budget accounting
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<Buf*> all;
+
+    void* get(Buf& b, size_t bytes, const char* name) {
+        if (bytes <= b.bytes && b.p) return b.p;
+        release(b);
+        if (bytes == 0) return nullptr;
+        if (budget && held + bytes > budget)
+            fail(MGW_ERR_MEMORY, "%s: %zu bytes would exceed the device budget (%zu held of %zu)",
+                 name, bytes, held, budget);
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(MGW_ERR_MEMORY, "hipMalloc(%s, %zu bytes) failed: %s", name, bytes,
+                 hipGetErrorString(e));
+        }
+        b.p = p;
+        b.bytes = bytes;
+        held += bytes;
+        return p;
+    }
+    void release(Buf& b) {
+        if (b.p) {
+            (void)hipFree(b.p);
+            held -= b.bytes;
+        }
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+
+    // resident inputs
+    Buf eri, v, colidx, rowbase, delta;
+    int nmo = 0, nocc = 0;        // of the resident ERI (0 = none)
+    int n = 0, np = 0;            // ph dimension of the resident V
+    bool v_ready = false;
+    // single-frequency workspace (compute_wc / quadratic forms)
+    Buf wA, wS, wdinv, wtbuf, winfo, wpkrow, wqpart, wq, wsrc, wcol;
+    bool wc_ready = false;
+    // hoisted path
+    Buf hA, hS, hdinv, htbuf, hinfo, hpkrow, hqpart, hq, homega, hweight, hmo, hsig_re, hsig_im,
+        hsx, hvxc, hstates, hqp, hiters;
+    Buf xhi, xlo, xscale, mhi, mlo, mscale;
+
+    mgw_hip_ctx() {
+        all = {&eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
+               &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
+               &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
+               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale};
+    }
+    ~mgw_hip_ctx() {
+        (void)hipSetDevice(device);
+        for (Buf* b : all) release(*b);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+    template <class T>
+    void h2d(void* dst, const T* src, size_t count) {
+        hip_check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync H2D");
+    }
+    template <class T>
+    void d2h(T* dst, const void* src, size_t count) {
+        hip_check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, stream),
+                  "hipMemcpyAsync D2H");
+    }
+};
+
+namespace {
+
+template <class F>
+int guarded(mgw_hip_ctx* ctx, F&& body) {
+    try {
+        if (ctx) hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        body();
+        if (ctx) ctx->error.clear();
+        return MGW_OK;
+    } catch (const MgwError& e) {
+        (ctx ? ctx->error : g_thread_error) = e.what();
+        if (ctx) g_thread_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        (ctx ? ctx->error : g_thread_error) = "host allocation failed";
+        return MGW_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        (ctx ? ctx->error : g_thread_error) = e.what();
+        return MGW_ERR_HIP;
+    }
+}
+
+void need(bool cond, const char* fmt, ...) {
+    if (cond) return;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw MgwError(MGW_ERR_ARG, buf);
+}
+
+template <class T>
+T* as(void* p) {
+    return static_cast<T*>(p);
+}
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// Numerical-Recipes Gauss-Legendre (src/gw/frequency_grids.cpp:12-44) and the
+// (1+y)/(2(1-y)) transform, sorted ascending, first node clamped >= 1e-12
+// (src/gw/frequency_grids.cpp:62-93).
+void transformed_legendre(int nfreq, double* omega, double* weight) {
+    const int n = nfreq;
+    std::vector<double> x(n), w(n);
+    const double eps = 3e-14;
+    const int m = (n + 1) / 2;
+    const double xm = 0.0, xl = 1.0;
+    for (int i = 0; i < m; ++i) {
+        double z = std::cos(kPi * ((double)(i + 1) - 0.25) / ((double)n + 0.5));
+        double pp = 0.0;
+        for (int iter = 0; iter < 100; ++iter) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * (double)j - 1.0) * z * p2 - ((double)j - 1.0) * p3) / (double)j;
+            }
+            pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+            const double z_old = z;
+            z = z_old - p1 / pp;
+            if (std::fabs(z - z_old) <= eps) break;
+        }
+        x[i] = xm - xl * z;
+        x[n - 1 - i] = xm + xl * z;
+        w[i] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+        w[n - 1 - i] = w[i];
+    }
+    std::vector<double> tv(n), tw(n);
+    for (int i = 0; i < n; ++i) {
+        tv[i] = 0.5 * (1.0 + x[i]) / (1.0 - x[i]);
+        tw[i] = w[i] / ((1.0 - x[i]) * (1.0 - x[i]));
+    }
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tv[a] < tv[b]; });
+    for (int i = 0; i < n; ++i) {
+        omega[i] = tv[order[i]];
+        weight[i] = tw[order[i]];
+    }
+    if (n > 0 && omega[0] < 1e-12) omega[0] = 1e-12;
+}
+
+struct PhLayout {
+    int nmo, nocc, nvir, n, np;
+    std::vector<int> colidx;        // np: i*nmo + nocc + a (0 in the padding)
+    std::vector<int64_t> rowbase;   // np: (j*nmo + nocc + b) * nmo^2 (0 in the padding)
+};
+
+PhLayout ph_layout(int nmo, int nocc) {
+    PhLayout L;
+    L.nmo = nmo;
+    L.nocc = nocc;
+    L.nvir = nmo - nocc;
+    L.n = nocc * L.nvir;
+    L.np = mgw::round_up(std::max(L.n, 1), mgw::kCT);
+    L.colidx.assign(L.np, 0);
+    L.rowbase.assign(L.np, 0);
+    const int64_t n2 = (int64_t)nmo * nmo;
+    for (int i = 0; i < nocc; ++i)
+        for (int a = 0; a < L.nvir; ++a) {
+            const int ph = i * L.nvir + a;
+            L.colidx[ph] = i * nmo + nocc + a;
+            L.rowbase[ph] = ((int64_t)i * nmo + nocc + a) * n2;
+        }
+    return L;
+}
+
+size_t batch_bytes(int np, int mpad, int precision) {
+    const size_t nblk = np / mgw::kNB;
+    size_t b = (size_t)np * np * 8           // A
+               + (size_t)np * 8              // S
+               + nblk * 64 * 64 * 8          // dinv
+               + (size_t)np * 64 * 8         // tbuf
+               + (size_t)(np / mgw::kCT) * mpad * 8  // qpart
+               + 4;                          // info
+    if (precision == MGW_PREC_MIXED) b += (size_t)np * np * 4 + (size_t)np * 8;  // M planes
+    return b;
+}
+
+// Factor + invert nb symmetrised dielectric matrices already in A (lower).
+void factor_batch(mgw_hip_ctx* ctx, double* A, double* dinv, int* info, double* tbuf, int np,
+                  int nb) {
+    hip_check(mgw::launch_potrf(A, dinv, info, np, nb, ctx->stream), "potrf launch");
+    hip_check(mgw::launch_trtri(A, dinv, tbuf, np, nb, ctx->stream), "trtri launch");
+}
+
+void check_info(mgw_hip_ctx* ctx, const int* dinfo, int count, const char* what) {
+    std::vector<int> info(count);
+    ctx->d2h(info.data(), dinfo, count);
+    ctx->sync();
+    for (int i = 0; i < count; ++i)
+        if (info[i] != 0)
+            fail(MGW_ERR_NUMERIC,
+                 "%s: symmetrised dielectric matrix %d is not positive definite (pivot %d)", what, i,
+                 info[i]);
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* mgw_hip_version(void) { return "minigw-amd 0.1 (gfx950)"; }
+
+const char* mgw_hip_last_error(const mgw_hip_ctx* ctx) {
+    return ctx ? ctx->error.c_str() : g_thread_error.c_str();
+}
+
+int mgw_hip_device_count(int* count) {
+    return guarded(nullptr, [&] {
+        need(count != nullptr, "mgw_hip_device_count: count is NULL");
+        hip_check(hipGetDeviceCount(count), "hipGetDeviceCount");
+    });
+}
+
+int mgw_hip_device_for_local_rank(size_t local_rank, size_t tasks_per_gpu, int* device) {
+    return guarded(nullptr, [&] {
+        need(device != nullptr, "mgw_hip_device_for_local_rank: device is NULL");
+        need(tasks_per_gpu > 0, "mgw_hip_device_for_local_rank: tasks_per_gpu must be positive");
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (count <= 0) fail(MGW_ERR_HIP, "no HIP devices are visible");
+        *device = (int)((local_rank / tasks_per_gpu) % (size_t)count);
+    });
+}
+
+int mgw_hip_create(int device, mgw_hip_ctx** out) {
+    return guarded(nullptr, [&] {
+        need(out != nullptr, "mgw_hip_create: out is NULL");
+        *out = nullptr;
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        need(device >= 0 && device < count, "mgw_hip_create: device %d out of range (%d visible)",
+             device, count);
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        auto* ctx = new mgw_hip_ctx();
+        ctx->device = device;
+        const hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ctx;
+            hip_check(e, "hipStreamCreateWithFlags");
+        }
+        *out = ctx;
+    });
+}
+
+int mgw_hip_destroy(mgw_hip_ctx* ctx) {
+    if (!ctx) return MGW_OK;
+    delete ctx;
+    return MGW_OK;
+}
+
+int mgw_hip_device_bytes(const mgw_hip_ctx* ctx, size_t* bytes) {
+    if (!ctx || !bytes) {
+        g_thread_error = "mgw_hip_device_bytes: NULL argument";
+        return MGW_ERR_ARG;
+    }
+    *bytes = ctx->held;
+    return MGW_OK;
+}
+
+int mgw_hip_set_memory_budget(mgw_hip_ctx* ctx, size_t bytes) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_memory_budget: ctx is NULL");
+        ctx->budget = bytes;
+    });
+}
+
+int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight) {
+    return guarded(nullptr, [&] {
+        need(nfreq > 0, "generate_transformed_legendre_grid: num_freq_points must be positive");
+        need(omega && weight, "mgw_grid_transformed_legendre: NULL output");
+        transformed_legendre(nfreq, omega, weight);
+    });
+}
+
+// ---------------------------------------------------- DeviceScreeningWorkspace
+int mgw_hip_upload_vph(mgw_hip_ctx* ctx, const double* v, int32_t n) {
+    return guarded(ctx, [&] {
+        need(ctx && v, "mgw_hip_upload_vph: NULL argument");
+        need(n > 0, "DeviceScreeningWorkspace: V_ph must be non-empty and square");
+        const int np = mgw::round_up(n, mgw::kCT);
+        double* tmp = as<double>(ctx->get(ctx->wsrc, (size_t)n * n * 8, "V_ph staging"));
+        double* dv = as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
+        ctx->h2d(tmp, v, (size_t)n * n);
+        hip_check(mgw::launch_pad_square(tmp, dv, n, np, ctx->stream), "pad V_ph");
+        ctx->sync();
+        ctx->release(ctx->wsrc);
+        ctx->n = n;
+        ctx->np = np;
+        ctx->v_ready = true;
+        ctx->wc_ready = false;
+    });
+}
+
+int mgw_hip_build_vph_from_eri(mgw_hip_ctx* ctx) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_build_vph_from_eri: ctx is NULL");
+        if (!ctx->nmo) fail(MGW_ERR_STATE, "mgw_hip_build_vph_from_eri: no resident ERI");
+        const PhLayout L = ph_layout(ctx->nmo, ctx->nocc);
+        double* dv = as<double>(ctx->get(ctx->v, (size_t)L.np * L.np * 8, "V_ph"));
+        hip_check(mgw::launch_build_v(as<double>(ctx->eri.p), as<int64_t>(ctx->rowbase.p),
+                                      as<int>(ctx->colidx.p), dv, L.n, L.np, ctx->stream),
+                  "build V_ph");
+        ctx->sync();
+        ctx->n = L.n;
+        ctx->np = L.np;
+        ctx->v_ready = true;
+        ctx->wc_ready = false;
+    });
+}
+
+int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0_im, int32_t n) {
+    return guarded(ctx, [&] {
+        need(ctx && pi0_re, "mgw_hip_compute_wc: NULL argument");
+        if (!ctx->v_ready) fail(MGW_ERR_STATE, "compute_w_c before V_ph was uploaded");
+        need(n == ctx->n, "DeviceScreeningWorkspace::compute_w_c: inconsistent pi0 dimension");
+        for (int i = 0; i < n; ++i) {
+            if (pi0_im && pi0_im[i] != 0.0)
+                fail(MGW_ERR_UNSUPPORTED,
+                     "compute_w_c: complex Pi0 (eta != 0) is not supported by the hip backend");
+            if (!(pi0_re[i] <= 0.0))
+                fail(MGW_ERR_UNSUPPORTED,
+                     "compute_w_c: Pi0[%d] = %g > 0 (non-positive particle-hole gap)", i, pi0_re[i]);
+        }
+        const int np = ctx->np;
+        double* A = as<double>(ctx->get(ctx->wA, (size_t)np * np * 8, "dielectric"));
+        double* S = as<double>(ctx->get(ctx->wS, (size_t)np * 8 * 2, "scale"));
+        double* dinv = as<double>(ctx->get(ctx->wdinv, (size_t)np * 64 * 8, "dinv"));
+        double* tbuf = as<double>(ctx->get(ctx->wtbuf, (size_t)np * 64 * 8, "trtri scratch"));
+        int* info = as<int>(ctx->get(ctx->winfo, sizeof(int), "info"));
+        double* pi0 = S + np;
+        ctx->h2d(pi0, pi0_re, n);
+        hip_check(hipMemsetAsync(info, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
+        hip_check(mgw::launch_scale_from_pi0(pi0, S, n, np, ctx->stream), "scale");
+        hip_check(mgw::launch_dielectric(as<double>(ctx->v.p), S, A, np, 1, ctx->stream),
+                  "dielectric");
+        factor_batch(ctx, A, dinv, info, tbuf, np, 1);
+        check_info(ctx, info, 1, "compute_w_c");
+        ctx->wc_ready = true;
+    });
+}
+
+int mgw_hip_download_wc(mgw_hip_ctx* ctx, double* wc_re, double* wc_im, int32_t n) {
+    return guarded(ctx, [&] {
+        need(ctx && wc_re, "mgw_hip_download_wc: NULL argument");
+        if (!ctx->wc_ready) fail(MGW_ERR_STATE, "W_c requested before compute_w_c");
+        need(n == ctx->n, "mgw_hip_download_wc: n = %d, workspace has %d", n, ctx->n);
+        double* wc = as<double>(ctx->get(ctx->wsrc, (size_t)n * n * 8, "W_c staging"));
+        hip_check(mgw::launch_wc_explicit(as<double>(ctx->wA.p), as<double>(ctx->wS.p), wc, n,
+                                          ctx->np, ctx->stream),
+                  "W_c");
+        ctx->d2h(wc_re, wc, (size_t)n * n);
+        ctx->sync();
+        if (wc_im) std::fill(wc_im, wc_im + (size_t)n * n, 0.0);
+    });
+}
+
+namespace {
+// q for columns described by (src, pkrow[mpad], colidx[np]) with the resident M.
+void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow, int m,
+                     const int* hcolidx, double* q_re, double* q_im) {
+    const int np = ctx->np;
+    const int mpad = mgw::round_up(std::max(m, 1), mgw::kCT);
+    std::vector<int64_t> pk(mpad, 0);
+    std::copy(hpkrow, hpkrow + m, pk.begin());
+    int64_t* dpk = as<int64_t>(ctx->get(ctx->wpkrow, (size_t)mpad * 8, "pkrow"));
+    int* dcol = as<int>(ctx->get(ctx->wcol, (size_t)np * 4, "colidx"));
+    double* qpart =
+        as<double>(ctx->get(ctx->wqpart, (size_t)(np / mgw::kCT) * mpad * 8, "qpart"));
+    double* q = as<double>(ctx->get(ctx->wq, (size_t)mpad * 8, "q"));
+    ctx->h2d(dpk, pk.data(), mpad);
+    ctx->h2d(dcol, hcolidx, np);
+    hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), src, dpk,
+                                       dcol, qpart, np, mpad, 1, ctx->stream),
+              "contract");
+    hip_check(mgw::launch_reduce_q(qpart, q, np, m, mpad, 0, 1, ctx->stream), "reduce q");
+    ctx->d2h(q_re, q, m);
+    ctx->sync();
+    if (q_im) std::fill(q_im, q_im + m, 0.0);
+}
+}  // namespace
+
+int mgw_hip_quadratic_forms_host(mgw_hip_ctx* ctx, const double* x, int32_t n, int32_t nvec,
+                                 double* q_re, double* q_im) {
+    return guarded(ctx, [&] {
+        need(ctx && x && q_re, "mgw_hip_quadratic_forms_host: NULL argument");
+        if (!ctx->wc_ready) fail(MGW_ERR_STATE, "quadratic_forms_panel before compute_w_c");
+        need(n == ctx->n && nvec >= 0,
+             "DeviceScreeningWorkspace::quadratic_forms_panel: inconsistent dimensions");
+        if (nvec == 0) return;
+        // transpose the row-major n x nvec panel into columns of length np
+        const int np = ctx->np;
+        std::vector<double> xt((size_t)nvec * np, 0.0);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < nvec; ++c) xt[(size_t)c * np + r] = x[(size_t)r * nvec + c];
+        double* dsrc = as<double>(ctx->get(ctx->wsrc, xt.size() * 8, "panel"));
+        ctx->h2d(dsrc, xt.data(), xt.size());
+        std::vector<int64_t> pk(nvec);
+        for (int c = 0; c < nvec; ++c) pk[c] = (int64_t)c * np;
+        std::vector<int> col(np);
+        std::iota(col.begin(), col.end(), 0);
+        contract_single(ctx, dsrc, pk.data(), nvec, col.data(), q_re, q_im);
+    });
+}
+
+// ---------------------------------------------------- DevicePqPhPanelView
+int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t nocc) {
+    return guarded(ctx, [&] {
+        need(ctx && eri, "mgw_hip_upload_eri: NULL argument");
+        need(nmo > 0, "MolecularIntegrals: ERI tensor must have shape [nmo,nmo,nmo,nmo]");
+        need(nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
+        const size_t count = (size_t)nmo * nmo * nmo * nmo;
+        const PhLayout L = ph_layout(nmo, nocc);
+        ctx->nmo = 0;
+        double* d = as<double>(ctx->get(ctx->eri, count * 8, "ERI"));
+        int* col = as<int>(ctx->get(ctx->colidx, (size_t)L.np * 4, "colidx"));
+        int64_t* rb = as<int64_t>(ctx->get(ctx->rowbase, (size_t)L.np * 8, "rowbase"));
+        ctx->h2d(d, eri, count);
+        ctx->h2d(col, L.colidx.data(), L.np);
+        ctx->h2d(rb, L.rowbase.data(), L.np);
+        ctx->sync();
+        ctx->nmo = nmo;
+        ctx->nocc = nocc;
+    });
+}
+
+int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width, double* q_re,
+                            double* q_im) {
+    return guarded(ctx, [&] {
+        need(ctx && q_re, "mgw_hip_quadratic_forms: NULL argument");
+        if (!ctx->nmo) fail(MGW_ERR_STATE, "fill_panel before the ERI was uploaded");
+        if (!ctx->wc_ready) fail(MGW_ERR_STATE, "quadratic_forms_panel before compute_w_c");
+        const int nmo = ctx->nmo;
+        need(p >= 0 && p < nmo, "DevicePqPhPanelView::fill_panel: p index out of range");
+        need(k0 >= 0 && width >= 0 && k0 <= nmo && width <= nmo - k0,
+             "DevicePqPhPanelView::fill_panel: k panel exceeds nmo");
+        const PhLayout L = ph_layout(nmo, ctx->nocc);
+        need(L.n == ctx->n, "quadratic forms: V_ph dimension %d does not match the ERI (%d)",
+             ctx->n, L.n);
+        if (width == 0) return;
+        std::vector<int64_t> pk(width);
+        const int64_t n2 = (int64_t)nmo * nmo;
+        for (int c = 0; c < width; ++c) pk[c] = ((int64_t)p * nmo + k0 + c) * n2;
+        contract_single(ctx, as<double>(ctx->eri.p), pk.data(), width, L.colidx.data(), q_re,
+                        q_im);
+    });
+}
+
+// ---------------------------------------------------- hoisted G0W0
+int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
+    return guarded(ctx, [&] {
+        need(ctx && job && out, "mgw_hip_g0w0: NULL argument");
+        const int nmo = job->nmo, nocc = job->nocc, nfreq = job->nfreq, npar = job->npar;
+        need(nmo > 1 && nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
+        if (ctx->nmo != nmo || ctx->nocc != nocc)
+            fail(MGW_ERR_STATE, "mgw_hip_g0w0: resident ERI has nmo=%d nocc=%d, job nmo=%d nocc=%d",
+                 ctx->nmo, ctx->nocc, nmo, nocc);
+        need(nfreq > 0, "generate_transformed_legendre_grid: num_freq_points must be positive");
+        need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
+        need(job->mo_energy && job->vxc_diag, "mgw_hip_g0w0: mo_energy / vxc_diag are NULL");
+        need(out->qp != nullptr, "mgw_hip_g0w0: out->qp is NULL");
+        if (job->eta != 0.0)
+            fail(MGW_ERR_UNSUPPORTED, "eta != 0 (complex screening) is not supported by the hip backend");
+        need(job->precision == MGW_PREC_FP64 || job->precision == MGW_PREC_MIXED,
+             "mgw_hip_g0w0: unknown precision %d", job->precision);
+        std::vector<int> states;
+        if (job->nstates <= 0 || !job->states) {
+            states.resize(nmo);
+            std::iota(states.begin(), states.end(), 0);
+        } else {
+            states.assign(job->states, job->states + job->nstates);
+            for (size_t i = 0; i < states.size(); ++i) {
+                need(states[i] >= 0 && states[i] < nmo, "Selected state is out of range");
+                need(i == 0 || states[i] > states[i - 1], "states must be ascending and unique");
+            }
+        }
+        const int ns = (int)states.size();
+        const PhLayout L = ph_layout(nmo, nocc);
+        const int n = L.n, np = L.np, nblk = np / mgw::kNB;
+        const int m = ns * nmo, mpad = mgw::round_up(m, mgw::kCT);
+        std::vector<double> delta(np, 0.0);
+        for (int i = 0; i < nocc; ++i)
+            for (int a = 0; a < L.nvir; ++a) {
+                const double d = job->mo_energy[nocc + a] - job->mo_energy[i];
+                if (!(d > 0.0))
+                    fail(MGW_ERR_UNSUPPORTED,
+                         "particle-hole gap e_%d - e_%d = %g is not positive", nocc + a, i, d);
+                delta[i * L.nvir + a] = d;
+            }
+        std::vector<double> omega(nfreq), weight(nfreq);
+        transformed_legendre(nfreq, omega.data(), weight.data());
+        std::vector<int64_t> pkrow(mpad, 0);
+        const int64_t n2 = (int64_t)nmo * nmo;
+        for (int s = 0; s < ns; ++s)
+            for (int k = 0; k < nmo; ++k) pkrow[(size_t)s * nmo + k] = ((int64_t)states[s] * nmo + k) * n2;
+
+        hipStream_t st = ctx->stream;
+        hipEvent_t ev[8];
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        struct EvGuard {
+            hipEvent_t* e;
+            ~EvGuard() {
+                for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
+            }
+        } evg{ev};
+        hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
+
+        // resident per-molecule data
+        double* dV = as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
+        double* dom = as<double>(ctx->get(ctx->homega, (size_t)nfreq * 8, "omega"));
+        double* dwt = as<double>(ctx->get(ctx->hweight, (size_t)nfreq * 8, "weight"));
+        double* dmo = as<double>(ctx->get(ctx->hmo, (size_t)(nmo + np) * 8, "mo/delta"));
+        double* ddelta = dmo + nmo;
+        double* dvxc = as<double>(ctx->get(ctx->hvxc, (size_t)nmo * 8, "vxc"));
+        int* dstates = as<int>(ctx->get(ctx->hstates, (size_t)ns * 4, "states"));
+        int64_t* dpk = as<int64_t>(ctx->get(ctx->hpkrow, (size_t)mpad * 8, "pkrow"));
+        double* dq = as<double>(ctx->get(ctx->hq, (size_t)nfreq * m * 8, "q"));
+        ctx->h2d(dom, omega.data(), nfreq);
+        ctx->h2d(dwt, weight.data(), nfreq);
+        ctx->h2d(dmo, job->mo_energy, nmo);
+        ctx->h2d(ddelta, delta.data(), np);
+        ctx->h2d(dvxc, job->vxc_diag, nmo);
+        ctx->h2d(dstates, states.data(), ns);
+        ctx->h2d(dpk, pkrow.data(), mpad);
+        hip_check(mgw::launch_build_v(as<double>(ctx->eri.p), as<int64_t>(ctx->rowbase.p),
+                                      as<int>(ctx->colidx.p), dV, n, np, st),
+                  "build V_ph");
+        ctx->n = n;
+        ctx->np = np;
+        ctx->v_ready = true;
+        ctx->wc_ready = false;
+        const bool mixed = job->precision == MGW_PREC_MIXED;
+        const mgw::Split16 fmt = mgw::Split16::FP16;
+        uint16_t *xh = nullptr, *xl = nullptr;
+        double* xs = nullptr;
+        if (mixed) {
+            xh = as<uint16_t>(ctx->get(ctx->xhi, (size_t)mpad * np * 2, "X hi"));
+            xl = as<uint16_t>(ctx->get(ctx->xlo, (size_t)mpad * np * 2, "X lo"));
+            xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
+            hip_check(mgw::launch_split_x(fmt, as<double>(ctx->eri.p), dpk,
+                                          as<int>(ctx->colidx.p), xh, xl, xs, n, np, mpad, st),
+                      "split X");
+        }
+        hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
+
+        // frequency batch size from the device budget
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        size_t room = free_b > (size_t)(2ull << 30) ? free_b - (size_t)(2ull << 30) : free_b / 2;
+        if (ctx->budget) room = std::min(room, ctx->budget > ctx->held ? ctx->budget - ctx->held : 0);
+        const size_t per = batch_bytes(np, mpad, job->precision);
+        int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
+        nb = std::min(nb, 64);
+        double* A = as<double>(ctx->get(ctx->hA, (size_t)nb * np * np * 8, "dielectric batch"));
+        double* S = as<double>(ctx->get(ctx->hS, (size_t)nb * np * 8, "scale batch"));
+        double* dinv = as<double>(ctx->get(ctx->hdinv, (size_t)nb * nblk * 64 * 64 * 8, "dinv"));
+        double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * np * 64 * 8, "trtri scratch"));
+        int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
+        double* qpart = as<double>(
+            ctx->get(ctx->hqpart, (size_t)nb * (np / mgw::kCT) * mpad * 8, "qpart"));
+        uint16_t *mh = nullptr, *ml = nullptr;
+        double* ms = nullptr;
+        if (mixed) {
+            mh = as<uint16_t>(ctx->get(ctx->mhi, (size_t)nb * np * np * 2, "M hi"));
+            ml = as<uint16_t>(ctx->get(ctx->mlo, (size_t)nb * np * np * 2, "M lo"));
+            ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
+        }
+        hip_check(hipMemsetAsync(info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
+
+        float t_diel = 0, t_fact = 0, t_contr = 0;
+        for (int f0 = 0; f0 < nfreq; f0 += nb) {
+            const int b = std::min(nb, nfreq - f0);
+            hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+            hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0, b, st), "scale");
+            hip_check(mgw::launch_dielectric(dV, S, A, np, b, st), "dielectric");
+            hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+            factor_batch(ctx, A, dinv, info + f0, tbuf, np, b);
+            hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
+            if (mixed) {
+                hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
+                hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b, st),
+                          "contract split");
+                hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dq, np, m, mpad, f0, b, st),
+                          "reduce q");
+            } else {
+                hip_check(mgw::launch_contract_f64(A, S, as<double>(ctx->eri.p), dpk,
+                                                   as<int>(ctx->colidx.p), qpart, np, mpad, b, st),
+                          "contract");
+                hip_check(mgw::launch_reduce_q(qpart, dq, np, m, mpad, f0, b, st), "reduce q");
+            }
+            hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
+            hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
+            float a = 0, bb = 0, c = 0;
+            hip_check(hipEventElapsedTime(&a, ev[2], ev[3]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&bb, ev[3], ev[4]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&c, ev[4], ev[5]), "hipEventElapsedTime");
+            t_diel += a;
+            t_fact += bb;
+            t_contr += c;
+        }
+        check_info(ctx, info, nfreq, "potrf");
+        ctx->wc_ready = false;
+
+        hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+        double* sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
+        double* sim = as<double>(ctx->get(ctx->hsig_im, (size_t)ns * nfreq * 8, "sigma im"));
+        double* sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
+        double* qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
+        int* iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
+        hip_check(mgw::launch_sigma_c(dq, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, sre, sim,
+                                      st),
+                  "sigma_c");
+        hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+        hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), sx, nmo, nocc, st), "sigma_x");
+        {
+            std::vector<double> nan(nmo, std::numeric_limits<double>::quiet_NaN());
+            std::vector<int> zero(nmo, 0);
+            ctx->h2d(qp, nan.data(), nmo);
+            ctx->h2d(iters, zero.data(), nmo);
+            hip_check(mgw::launch_pade_qp(sre, sim, dom, dmo, sx, dvxc, dstates, job->fermi_energy,
+                                          ns, nfreq, npar, qp, iters, st),
+                      "pade/qp");
+            hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
+            if (out->sigma_c_re) ctx->d2h(out->sigma_c_re, sre, (size_t)ns * nfreq);
+            if (out->sigma_c_im) ctx->d2h(out->sigma_c_im, sim, (size_t)ns * nfreq);
+            if (out->sigma_x_diag) ctx->d2h(out->sigma_x_diag, sx, nmo);
+            ctx->d2h(out->qp, qp, nmo);
+            if (out->qp_iterations) ctx->d2h(out->qp_iterations, iters, nmo);
+            ctx->sync();
+        }
+        if (out->omega) std::copy(omega.begin(), omega.end(), out->omega);
+        if (out->weight) std::copy(weight.begin(), weight.end(), out->weight);
+        float tv = 0, tsig = 0, tqp = 0, ttot = 0;
+        hip_check(hipEventElapsedTime(&tv, ev[0], ev[1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&tsig, ev[2], ev[3]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&tqp, ev[3], ev[4]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&ttot, ev[0], ev[4]), "hipEventElapsedTime");
+        const double t[7] = {tv, t_diel, t_fact, t_contr, tsig, tqp, ttot};
+        std::copy(t, t + 7, ctx->timings);
+        ctx->timings[7] = nb;
+    });
+}
+
+int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, double* c) {
+    return guarded(ctx, [&] {
+        need(ctx && a && b && c, "mgw_hip_selftest_tile: NULL argument");
+        double* d = as<double>(ctx->get(ctx->wsrc, 3 * 64 * 64 * 8, "selftest"));
+        ctx->h2d(d, a, 64 * 64);
+        ctx->h2d(d + 4096, b, 64 * 64);
+        hip_check(mgw::launch_tile_selftest(d, d + 4096, d + 8192, ctx->stream), "tile selftest");
+        ctx->d2h(c, d + 8192, 64 * 64);
+        ctx->sync();
+    });
+}
+
+int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n) {
+    if (!ctx || !ms || n < 0) {
+        g_thread_error = "mgw_hip_last_timings: bad argument";
+        return MGW_ERR_ARG;
+    }
+    for (int i = 0; i < n && i < 8; ++i) ms[i] = ctx->timings[i];
+    return MGW_OK;
+}
+
+size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t nstates,
+                              int32_t nbatch, int32_t precision) {
+    if (nmo <= 1 || nocc <= 0 || nocc >= nmo || nfreq <= 0) return 0;
+    const PhLayout L = ph_layout(nmo, nocc);
+    const int ns = nstates > 0 ? nstates : nmo;
+    const int m = ns * nmo, mpad = mgw::round_up(m, mgw::kCT);
+    const size_t eri = (size_t)nmo * nmo * nmo * nmo * 8;
+    const size_t v = (size_t)L.np * L.np * 8;
+    size_t fixed = eri + v + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
+    if (precision == MGW_PREC_MIXED) fixed += (size_t)mpad * L.np * 4 + (size_t)mpad * 8;
+    const int nb = std::max(1, std::min(nbatch > 0 ? nbatch : nfreq, nfreq));
+    return fixed + (size_t)nb * batch_bytes(L.np, mpad, precision);
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// Device kernels of the screening + Sigma_c path and their launchers.
+// All launchers enqueue on the given stream and return hipGetLastError().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mgw {
+
+constexpr int kNB = 64;    // factorization block (potrf / trtri tiles)
+constexpr int kCT = 128;   // contraction tile: rows of M and columns of X per workgroup
+constexpr int kMaxPade = 64;
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Diagnostic: C = A B^T (64 x 64 row-major) through the factorization's MFMA tile.
+hipError_t launch_tile_selftest(const double* a, const double* b, double* c, hipStream_t s);
+
+// V[r][c] = eri[rowbase[r] + colidx[c]] for r, c < n; 0 in the padding (np x np).
+hipError_t launch_build_v(const double* eri, const int64_t* rowbase, const int* colidx,
+                          double* v, int n, int np, hipStream_t s);
+
+// Pad a host-provided n x n row-major matrix into np x np (zeros outside).
+hipError_t launch_pad_square(const double* src, double* dst, int n, int np, hipStream_t s);
+
+// S[w][r] = sqrt(4 D_r / (w_f^2 + D_r^2)), f = f0 + w, for r < n; 0 for r >= n.
+hipError_t launch_scale_from_grid(const double* delta, const double* omega, double* S, int n,
+                                  int np, int f0, int nb, hipStream_t s);
+
+// S[r] = sqrt(-pi0[r]) for r < n; 0 in the padding.
+hipError_t launch_scale_from_pi0(const double* pi0, double* S, int n, int np, hipStream_t s);
+
+// Symmetrised dielectric, lower triangle: A_w = I + diag(S_w) V diag(S_w).
+hipError_t launch_dielectric(const double* v, const double* S, double* A, int np, int nb,
+                             hipStream_t s);
+
+// In-place blocked Cholesky A_w = L L^T (lower) of nb matrices np x np, plus the
+// inverses of its 64 x 64 diagonal blocks in dinv (nb x (np/64) x 64 x 64).
+// info[w] = 1-based global row of the first non-positive pivot, 0 if none.
+hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipStream_t s);
+
+// In-place inverse of the lower factor: A_w <- L_w^{-1} (lower, zeros above the
+// diagonal inside diagonal blocks).  tbuf: nb x np x 64 scratch.
+hipError_t launch_trtri(double* A, const double* dinv, double* tbuf, int np, int nb,
+                        hipStream_t s);
+
+// qpart[w][rt][c] = sum_{r in row tile rt} (sum_k M_w[r][k] S_w[k] X[k][c])^2 with
+// X[k][c] = src[pkrow[c] + colidx[k]] (k < np, c < mpad), M_w lower triangular.
+hipError_t launch_contract_f64(const double* M, const double* S, const double* src,
+                               const int64_t* pkrow, const int* colidx, double* qpart, int np,
+                               int mpad, int nb, hipStream_t s);
+
+// q[(f0+w) * m + c] = -sum_rt qpart[w][rt][c].
+hipError_t launch_reduce_q(const double* qpart, double* q, int np, int m, int mpad, int f0,
+                           int nb, hipStream_t s);
+
+// Mixed precision: 16-bit hi/lo splits of the contraction operands.
+//   BF16: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); no scaling.
+//   FP16: the same with fp16 after a power-of-two scaling per row of M S and
+//         per column of X (largest entry near 2^14), undone in the epilogue.
+enum class Split16 : int { BF16 = 0, FP16 = 1 };
+
+// X[k][c] = src[pkrow[c] + colidx[k]] for k < n (0 for k >= n) -> planes
+// xhi/xlo laid out [c][k] (mpad x np); xscale[c] = applied scale (1 for BF16).
+hipError_t launch_split_x(Split16 fmt, const double* src, const int64_t* pkrow, const int* colidx,
+                          uint16_t* xhi, uint16_t* xlo, double* xscale, int n, int np, int mpad,
+                          hipStream_t s);
+
+// (M_w S_w)[r][k] for k <= r (0 above) -> planes [w][r][k]; mscale[w][r].
+hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_t* mhi,
+                          uint16_t* mlo, double* mscale, int np, int nb, hipStream_t s);
+
+// Same output as launch_contract_f64 (scaled by xscale[c]^2 when FP16):
+// products hi*hi + hi*lo + lo*hi on 16-bit MFMA (v_mfma_f32_16x16x32) with
+// FP32 accumulation, row scale undone and squares summed in FP64.
+hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_t* mlo,
+                                 const double* mscale, const uint16_t* xhi, const uint16_t* xlo,
+                                 double* qpart, int np, int mpad, int nb, hipStream_t s);
+
+// q[(f0+w) * m + c] = -(sum_rt qpart[w][rt][c]) / xscale[c]^2.
+hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, double* q, int np,
+                                  int m, int mpad, int f0, int nb, hipStream_t s);
+
+// W_c = -(M S)^T (M S) for one matrix (explicit, n x n row-major output).
+hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int n, int np,
+                              hipStream_t s);
+
+// sigma[s][n] = (1/pi) sum_f sum_k -(g0(k, n, f) q[f][s*nmo + k]) w_f
+hipError_t launch_sigma_c(const double* q, const double* mo_energy, const double* omega,
+                          const double* weight, double fermi, int nmo, int nstates, int nfreq,
+                          double* sig_re, double* sig_im, hipStream_t s);
+
+// sx[p] = -sum_{i < nocc} eri[p, i, p, i]
+hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s);
+
+// Pade fit + QP fixed point per state (FP64, no FMA contraction).
+hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const double* omega,
+                          const double* mo_energy, const double* sx, const double* vxc_diag,
+                          const int* states, double fermi, int nstates, int nfreq, int npar,
+                          double* qp, int* iters, hipStream_t s);
+
+}  // namespace mgw

@@ -1,0 +1,656 @@
+// CDNA4 (gfx950) kernels of the screening + Sigma_c path, FP64 form.
+//
+// Method (per imaginary frequency w', docs/formulation.md, reference
+// src/gw/sigma_c_driver.cpp:119-169 and src/workspace/screening_workspace.cpp:21-102):
+//   eps = I - diag(Pi0) V,  W_c = eps^-1 diag(Pi0),  q_pk = x_pk^T W_c x_pk.
+// At eta = 0, Pi0 = -S^2 with S = sqrt(4D/(w^2+D^2)) > 0, and eps is similar to
+// the symmetrised dielectric matrix  E = S^-1 eps S = I + S V S  (symmetric
+// positive definite, V being a Coulomb Gram matrix).  Hence
+//   W_c = eps^-1 diag(Pi0) = -S E^-1 S = -(L^-1 S)^T (L^-1 S),  E = L L^T,
+//   q_pk = -|| L^-1 S x_pk ||^2.
+// The kernels factor E by blocked Cholesky (potrf), invert the factor (trtri),
+// and contract M = L^-1 S against X streamed from the resident ERI, with the
+// sum of squares taken in the GEMM epilogue (Y = W_c X is never written).
+//
+// MFMA: v_mfma_f64_16x16x4_f64.  Operand/result lane layout (wave64):
+//   a: A[i][k], i = lane % 16, k = lane / 16      (16 x 4 block)
+//   b: B[k][j], j = lane % 16, k = lane / 16      (4 x 16 block)
+//   d[r]: D[i][j], i = 4 * (lane / 16) + r, j = lane % 16
+#include "mgw_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mgw {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma_f64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kBK = 16;          // K slab staged through LDS per step
+constexpr int kLdsPad = 4;       // doubles of padding per LDS row
+
+// ----------------------------------------------------------------- simple maps
+__global__ void k_build_v(const double* __restrict__ eri, const int64_t* __restrict__ rowbase,
+                          const int* __restrict__ colidx, double* __restrict__ v, int n, int np) {
+    const size_t total = (size_t)np * np;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / np), c = (int)(e % np);
+        v[e] = (r < n && c < n) ? eri[rowbase[r] + colidx[c]] : 0.0;
+    }
+}
+
+__global__ void k_pad_square(const double* __restrict__ src, double* __restrict__ dst, int n,
+                             int np) {
+    const size_t total = (size_t)np * np;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / np), c = (int)(e % np);
+        dst[e] = (r < n && c < n) ? src[(size_t)r * n + c] : 0.0;
+    }
+}
+
+__global__ void k_scale_grid(const double* __restrict__ delta, const double* __restrict__ omega,
+                             double* __restrict__ S, int n, int np, int f0, int nb) {
+    const int total = np * nb;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int w = e / np, r = e % np;
+        double s = 0.0;
+        if (r < n) {
+            const double d = delta[r], om = omega[f0 + w];
+            s = sqrt(4.0 * d / (om * om + d * d));
+        }
+        S[e] = s;
+    }
+}
+
+__global__ void k_scale_pi0(const double* __restrict__ pi0, double* __restrict__ S, int n, int np) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < np; r += gridDim.x * blockDim.x)
+        S[r] = r < n ? sqrt(-pi0[r]) : 0.0;
+}
+
+// lower triangle (incl. diagonal) of A_w = I + S V S; one matrix per blockIdx.y
+__global__ void k_dielectric(const double* __restrict__ v, const double* __restrict__ S,
+                             double* __restrict__ A, int np) {
+    const int w = blockIdx.y;
+    const double* Sw = S + (size_t)w * np;
+    double* Aw = A + (size_t)w * np * np;
+    const size_t total = (size_t)np * np;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / np), c = (int)(e % np);
+        if (c > r) continue;
+        Aw[e] = (r == c ? 1.0 : 0.0) + (Sw[r] * v[e]) * Sw[c];
+    }
+}
+
+// ------------------------------------------------------- 64 x 64 MFMA tile core
+// LDS slabs: As[k][m], Bs[k][n] for k < kBK.  256 threads = 4 waves in 2 x 2,
+// each wave a 32 x 32 sub-tile (2 x 2 MFMA blocks).
+struct Tile64 {
+    d4 acc[2][2];
+    __device__ void zero() {
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    __device__ void step(const double (*As)[64 + kLdsPad], const double (*Bs)[64 + kLdsPad]) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+        const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 4) {
+            const double a0 = As[kk + lk][wm + li], a1 = As[kk + lk][wm + 16 + li];
+            const double b0 = Bs[kk + lk][wn + li], b1 = Bs[kk + lk][wn + 16 + li];
+            acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+            acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+            acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+            acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+        }
+    }
+    // visit(row, col, value) over this thread's 16 results
+    template <class F>
+    __device__ void for_each(F f) const {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+        for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj)
+                for (int r = 0; r < 4; ++r)
+                    f(wm + bi * 16 + 4 * (lane >> 4) + r, wn + bj * 16 + (lane & 15), acc[bi][bj][r]);
+    }
+};
+
+// As[k][m] = src[(row0 + m) * ld + k0 + k], m < 64, k < kBK (row-major operand, K along a row)
+__device__ inline void load_rows_kmajor(double (*As)[64 + kLdsPad], const double* __restrict__ src,
+                                        size_t ld, int row0, int k0) {
+    const int t = threadIdx.x, m = t >> 2, kq = (t & 3) * 4;
+    const double* p = src + (size_t)(row0 + m) * ld + k0 + kq;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) As[kq + u][m] = p[u];
+}
+
+// Bs[k][n] = src[(k0 + k) * ld + col0 + n], k < kBK, n < 64 (row-major operand, K down the rows)
+__device__ inline void load_kn(double (*Bs)[64 + kLdsPad], const double* __restrict__ src,
+                               size_t ld, int k0, int col0) {
+    const int t = threadIdx.x, k = t >> 4, nq = (t & 15) * 4;
+    const double* p = src + (size_t)(k0 + k) * ld + col0 + nq;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Bs[k][nq + u] = p[u];
+}
+
+// ------------------------------------------------------------------ potrf pieces
+// Diagonal block k of every matrix: Cholesky in LDS, then the inverse of the
+// 64 x 64 factor by column-wise forward substitution.
+__global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* __restrict__ dinv,
+                                               int* __restrict__ info, int np, int k) {
+    __shared__ double L[64][65];
+    __shared__ double X[64][65];
+    const int w = blockIdx.x, t = threadIdx.x;
+    const int nblk = np / kNB;
+    double* Aw = A + (size_t)w * np * np;
+    const size_t off = (size_t)k * kNB * np + (size_t)k * kNB;
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int i = e >> 6, j = e & 63;
+        L[i][j] = (j <= i) ? Aw[off + (size_t)i * np + j] : 0.0;
+    }
+    bool bad = false;
+    for (int j = 0; j < 64; ++j) {
+        __syncthreads();
+        const double piv = L[j][j];
+        if (!(piv > 0.0)) bad = true;
+        const double djj = sqrt(piv > 0.0 ? piv : 1.0);
+        __syncthreads();
+        if (t == j) L[j][j] = djj;
+        else if (t > j && t < 64) L[t][j] /= djj;
+        __syncthreads();
+        for (int e = t; e < 64 * 64; e += 256) {
+            const int i = e >> 6, l = e & 63;
+            if (l > j && l <= i) L[i][l] -= L[i][j] * L[l][j];
+        }
+        if (bad && t == 0 && info[w] == 0) info[w] = k * kNB + j + 1;
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int c = t;
+        for (int i = 0; i < c; ++i) X[i][c] = 0.0;
+        X[c][c] = 1.0 / L[c][c];
+        for (int i = c + 1; i < 64; ++i) {
+            double s = 0.0;
+            for (int q = c; q < i; ++q) s += L[i][q] * X[q][c];
+            X[i][c] = -s / L[i][i];
+        }
+    }
+    __syncthreads();
+    double* D = dinv + ((size_t)w * nblk + k) * 64 * 64;
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int i = e >> 6, j = e & 63;
+        Aw[off + (size_t)i * np + j] = L[i][j];     // zeros above the diagonal
+        D[e] = X[i][j];
+    }
+}
+
+// Panel: A_ik <- A_ik * Dinv_k^T for i > k.
+__global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ A,
+                                                    const double* __restrict__ dinv, int np,
+                                                    int k) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    const int w = blockIdx.y, i = k + 1 + blockIdx.x;
+    const int nblk = np / kNB;
+    double* Aw = A + (size_t)w * np * np;
+    const double* D = dinv + ((size_t)w * nblk + k) * 64 * 64;
+    const double* Aik = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
+    Tile64 tile;
+    tile.zero();
+    for (int k0 = 0; k0 < kNB; k0 += kBK) {
+        load_rows_kmajor(As, Aik, np, 0, k0);
+        load_rows_kmajor(Bs, D, 64, 0, k0);
+        __syncthreads();
+        tile.step(As, Bs);
+        __syncthreads();
+    }
+    double* out = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
+    tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] = v; });
+}
+
+__device__ inline void tri_index(int t, int& ii, int& jj) {
+    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    while (i * (i + 1) / 2 > t) --i;
+    ii = i;
+    jj = t - i * (i + 1) / 2;
+}
+
+// Trailing update: A_ij -= L_ik L_jk^T for k < j <= i.
+__global__ __launch_bounds__(256) void k_syrk(double* __restrict__ A, int np, int k) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    const int w = blockIdx.y;
+    int ii, jj;
+    tri_index(blockIdx.x, ii, jj);
+    const int i = k + 1 + ii, j = k + 1 + jj;
+    double* Aw = A + (size_t)w * np * np;
+    const double* Lik = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
+    const double* Ljk = Aw + (size_t)j * kNB * np + (size_t)k * kNB;
+    Tile64 tile;
+    tile.zero();
+    for (int k0 = 0; k0 < kNB; k0 += kBK) {
+        load_rows_kmajor(As, Lik, np, 0, k0);
+        load_rows_kmajor(Bs, Ljk, np, 0, k0);
+        __syncthreads();
+        tile.step(As, Bs);
+        __syncthreads();
+    }
+    double* out = Aw + (size_t)i * kNB * np + (size_t)j * kNB;
+    tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] -= v; });
+}
+
+// ------------------------------------------------------------------ trtri pieces
+// T_i = sum_{k=j+1}^{i} M_ik L_kj, i > j (M = already inverted columns > j).
+__global__ __launch_bounds__(256) void k_trtri_t(const double* __restrict__ A,
+                                                 double* __restrict__ tbuf, int np, int j) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    const int w = blockIdx.y, i = j + 1 + blockIdx.x;
+    const double* Aw = A + (size_t)w * np * np;
+    const int kbeg = (j + 1) * kNB, kend = (i + 1) * kNB;
+    const double* Arow = Aw + (size_t)i * kNB * np;     // rows of block i
+    const double* Bcol = Aw + (size_t)j * kNB;          // column block j
+    Tile64 tile;
+    tile.zero();
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+        load_rows_kmajor(As, Arow, np, 0, k0);
+        load_kn(Bs, Bcol, np, k0, 0);
+        __syncthreads();
+        tile.step(As, Bs);
+        __syncthreads();
+    }
+    double* out = tbuf + (size_t)w * np * kNB + (size_t)i * kNB * kNB;
+    tile.for_each([&](int r, int c, double v) { out[r * kNB + c] = v; });
+}
+
+// M_jj = Dinv_j ; M_ij = -T_i Dinv_j for i > j.
+__global__ __launch_bounds__(256) void k_trtri_m(double* __restrict__ A,
+                                                 const double* __restrict__ dinv,
+                                                 const double* __restrict__ tbuf, int np, int j) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    const int w = blockIdx.y, nblk = np / kNB;
+    double* Aw = A + (size_t)w * np * np;
+    const double* D = dinv + ((size_t)w * nblk + j) * 64 * 64;
+    if (blockIdx.x == 0) {
+        double* out = Aw + (size_t)j * kNB * np + (size_t)j * kNB;
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) out[(size_t)(e >> 6) * np + (e & 63)] = D[e];
+        return;
+    }
+    const int i = j + blockIdx.x;
+    const double* T = tbuf + (size_t)w * np * kNB + (size_t)i * kNB * kNB;
+    Tile64 tile;
+    tile.zero();
+    for (int k0 = 0; k0 < kNB; k0 += kBK) {
+        load_rows_kmajor(As, T, kNB, 0, k0);
+        load_kn(Bs, D, 64, k0, 0);
+        __syncthreads();
+        tile.step(As, Bs);
+        __syncthreads();
+    }
+    double* out = Aw + (size_t)i * kNB * np + (size_t)j * kNB;
+    tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] = -v; });
+}
+
+// --------------------------------------------------------- fused contraction FP64
+// One workgroup: rows [r0, r0+128) of Z = M S X and columns [c0, c0+128);
+// K runs over [0, r0+128) (M lower triangular).  Epilogue: column sums of Z^2
+// over the tile's rows -> qpart[w][row tile][column].
+// 4 waves in 2 x 2, each a 64 x 64 sub-tile (4 x 4 MFMA blocks).
+__global__ __launch_bounds__(256) void k_contract_f64(
+    const double* __restrict__ M, const double* __restrict__ S, const double* __restrict__ src,
+    const int64_t* __restrict__ pkrow, const int* __restrict__ colidx, double* __restrict__ qpart,
+    int np, int mpad) {
+    __shared__ double As[kBK][kCT + kLdsPad];
+    __shared__ double Bs[kBK][kCT + kLdsPad];
+    __shared__ double red[2][kCT];
+    const int w = blockIdx.z;
+    const int rt = blockIdx.y, r0 = rt * kCT, c0 = blockIdx.x * kCT;
+    const double* Mw = M + (size_t)w * np * np;
+    const double* Sw = S + (size_t)w * np;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int li = lane & 15, lk = lane >> 4;
+
+    // loader coordinates: 128 rows/cols x 16 k, 8 consecutive k per thread
+    const int lrow = t >> 1, lkq = (t & 1) * 8;
+    const double* arow = Mw + (size_t)(r0 + lrow) * np;
+    const int64_t brow = pkrow[c0 + lrow];
+
+    d4 acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const int kend = min(np, r0 + kCT);
+    for (int k0 = 0; k0 < kend; k0 += kBK) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = k0 + lkq + u;
+            const double mv = (kk <= r0 + lrow) ? arow[kk] * Sw[kk] : 0.0;
+            As[lkq + u][lrow] = mv;
+            Bs[lkq + u][lrow] = src[brow + colidx[kk]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 4) {
+            double a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = As[kk + lk][wm + q * 16 + li];
+                b[q] = Bs[kk + lk][wn + q * 16 + li];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[x][y] = mfma_f64(a[x], b[y], acc[x][y]);
+        }
+        __syncthreads();
+    }
+
+    // epilogue: per-column sum of squares over this wave's 64 rows
+    for (int y = 0; y < 4; ++y) {
+        double s = 0.0;
+        for (int x = 0; x < 4; ++x)
+            for (int r = 0; r < 4; ++r) s += acc[x][y][r] * acc[x][y][r];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (lk == 0) red[wave >> 1][wn + y * 16 + li] = s;
+    }
+    __syncthreads();
+    if (t < kCT) {
+        const double v = red[0][t] + red[1][t];
+        qpart[((size_t)w * (np / kCT) + rt) * mpad + c0 + t] = v;
+    }
+}
+
+__global__ void k_reduce_q(const double* __restrict__ qpart, double* __restrict__ q, int nrt,
+                           int m, int mpad, int f0) {
+    const int w = blockIdx.y;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int rt = 0; rt < nrt; ++rt) s += qpart[((size_t)w * nrt + rt) * mpad + c];
+        q[(size_t)(f0 + w) * m + c] = -s;
+    }
+}
+
+// W_c = -(M S)^T (M S), plain FMA (diagnostic / host-panel path, small n).
+__global__ void k_wc_explicit(const double* __restrict__ M, const double* __restrict__ S,
+                              double* __restrict__ wc, int n, int np) {
+    const size_t total = (size_t)n * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / n), c = (int)(e % n);
+        const int t0 = r > c ? r : c;
+        double s = 0.0;
+        for (int t = t0; t < n; ++t) s += M[(size_t)t * np + r] * M[(size_t)t * np + c];
+        wc[e] = -(S[r] * s) * S[c];
+    }
+}
+
+// ------------------------------------------------------------- Sigma_c, Sigma_x
+// Complex helpers with the textbook formulas of the host complex type and
+// Smith's division, evaluated without FMA contraction.
+struct cplx {
+    double re, im;
+};
+__device__ inline cplx cmul(cplx a, cplx b) {
+#pragma clang fp contract(off)
+    return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ inline cplx cdiv(cplx a, cplx b) {
+#pragma clang fp contract(off)
+    // Smith: |c| < |d| branch as libgcc's __divdc3 / CPython's complex division
+    if (fabs(b.re) < fabs(b.im)) {
+        const double ratio = b.re / b.im;
+        const double denom = (b.re * ratio) + b.im;
+        return cplx{((a.re * ratio) + a.im) / denom, ((a.im * ratio) - a.re) / denom};
+    }
+    const double ratio = b.im / b.re;
+    const double denom = (b.im * ratio) + b.re;
+    return cplx{((a.im * ratio) + a.re) / denom, (a.im - (a.re * ratio)) / denom};
+}
+__device__ inline double cabs_(cplx a) { return hypot(a.re, a.im); }
+
+__global__ void k_sigma_c(const double* __restrict__ q, const double* __restrict__ mo,
+                          const double* __restrict__ omega, const double* __restrict__ weight,
+                          double fermi, int nmo, int nstates, int nfreq, double* __restrict__ sre,
+                          double* __restrict__ sim) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nstates * nfreq) return;
+    const int s = e / nfreq, n = e % nfreq;
+    const size_t m = (size_t)nstates * nmo;
+    const double wn = omega[n];
+    double are = 0.0, aim = 0.0;
+    for (int f = 0; f < nfreq; ++f) {
+        const double wp = omega[f], wf = weight[f];
+        const double* qf = q + (size_t)f * m + (size_t)s * nmo;
+        for (int k = 0; k < nmo; ++k) {
+            const cplx d{fermi - mo[k], wn};
+            const cplx dd = cmul(d, d);
+            const cplx den{dd.re + wp * wp, dd.im};
+            const cplx g = cdiv(d, den);
+            const double qk = qf[k];
+            are -= (g.re * qk) * wf;
+            aim -= (g.im * qk) * wf;
+        }
+    }
+    const double pi = 3.141592653589793238462643383279502884;
+    sre[e] = are / pi;
+    sim[e] = aim / pi;
+}
+
+__global__ void k_sigma_x(const double* __restrict__ eri, double* __restrict__ sx, int nmo,
+                          int nocc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nmo) return;
+    const size_t n2 = (size_t)nmo * nmo;
+    double v = 0.0;
+    for (int i = 0; i < nocc; ++i) v -= eri[((size_t)p * nmo + i) * n2 + (size_t)p * nmo + i];
+    sx[p] = v;
+}
+
+// Thiele continued-fraction Pade (src/gw/pade.cpp:10-101) + QP fixed point
+// (src/gw/qp_solver.cpp:10-47), one thread per state.
+__global__ void k_pade_qp(const double* __restrict__ sre, const double* __restrict__ sim,
+                          const double* __restrict__ omega, const double* __restrict__ mo,
+                          const double* __restrict__ sx, const double* __restrict__ vxc,
+                          const int* __restrict__ states, double fermi, int nstates, int nfreq,
+                          int npar, double* __restrict__ qp, int* __restrict__ iters) {
+#pragma clang fp contract(off)
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= nstates) return;
+    const int state = states[si];
+    const int npa = npar < nfreq ? npar : nfreq;
+    cplx xs[kMaxPade], g[kMaxPade], coef[kMaxPade];
+    const int step = npa > 1 ? nfreq / (npa - 1) : 0;
+    for (int i = 0; i < npa; ++i) {
+        int idx;
+        if (npa == 1) idx = 0;
+        else if (step == 0) idx = i;
+        else idx = (i < npa - 1) ? i * step : nfreq - 1;
+        xs[i] = cplx{0.0, omega[idx]};
+        g[i] = cplx{sre[(size_t)si * nfreq + idx], sim[(size_t)si * nfreq + idx]};
+    }
+    coef[0] = g[0];
+    for (int ip = 1; ip < npa; ++ip) {
+        const cplx piv = g[ip - 1];
+        for (int id = ip; id < npa; ++id) {
+            const cplx num{piv.re - g[id].re, piv.im - g[id].im};
+            const cplx dx{xs[id].re - xs[ip - 1].re, xs[id].im - xs[ip - 1].im};
+            const cplx den = cmul(dx, g[id]);
+            g[id] = (cabs_(den) < 1e-20) ? cdiv(num, cplx{1e-20, 0.0}) : cdiv(num, den);
+        }
+        coef[ip] = g[ip];
+    }
+    const double e0 = mo[state];
+    double cur = e0;
+    int used = 200;
+    for (int it = 0; it < 200; ++it) {
+        const double z = cur - fermi;
+        cplx val{1.0, 0.0};
+        cplx res;
+        if (npa == 1) {
+            res = coef[0];
+        } else {
+            for (int k = npa - 2; k >= 0; --k) {
+                const cplx num = cmul(coef[k + 1], cplx{z - xs[k].re, 0.0 - xs[k].im});
+                if (cabs_(val) < 1e-20) val = cplx{1e-20, 0.0};
+                const cplx qv = cdiv(num, val);
+                val = cplx{1.0 + qv.re, 0.0 + qv.im};
+            }
+            res = (cabs_(val) < 1e-20) ? cdiv(coef[0], cplx{1e-20, 0.0}) : cdiv(coef[0], val);
+        }
+        const double nq = e0 + res.re + sx[state] - vxc[state];
+        if (fabs(nq - cur) < 1e-6) {
+            cur = nq;
+            used = it + 1;
+            break;
+        }
+        cur = nq;
+    }
+    qp[state] = cur;
+    iters[state] = used;
+}
+
+// Diagnostic: C = A B^T for 64 x 64 row-major A, B through the Tile64 path.
+__global__ __launch_bounds__(256) void k_tile_selftest(const double* __restrict__ a,
+                                                       const double* __restrict__ b,
+                                                       double* __restrict__ c) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    Tile64 tile;
+    tile.zero();
+    for (int k0 = 0; k0 < 64; k0 += kBK) {
+        load_rows_kmajor(As, a, 64, 0, k0);
+        load_rows_kmajor(Bs, b, 64, 0, k0);
+        __syncthreads();
+        tile.step(As, Bs);
+        __syncthreads();
+    }
+    tile.for_each([&](int r, int col, double v) { c[r * 64 + col] = v; });
+}
+
+inline int grid_for(size_t total, int block = 256, int cap = 65535 * 8) {
+    size_t g = (total + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_tile_selftest(const double* a, const double* b, double* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_selftest, dim3(1), dim3(256), 0, s, a, b, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_v(const double* eri, const int64_t* rowbase, const int* colidx, double* v,
+                          int n, int np, hipStream_t s) {
+    hipLaunchKernelGGL(k_build_v, dim3(grid_for((size_t)np * np)), dim3(256), 0, s, eri, rowbase,
+                       colidx, v, n, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_square(const double* src, double* dst, int n, int np, hipStream_t s) {
+    hipLaunchKernelGGL(k_pad_square, dim3(grid_for((size_t)np * np)), dim3(256), 0, s, src, dst,
+                       n, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_from_grid(const double* delta, const double* omega, double* S, int n,
+                                  int np, int f0, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_grid, dim3(grid_for((size_t)np * nb)), dim3(256), 0, s, delta,
+                       omega, S, n, np, f0, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_from_pi0(const double* pi0, double* S, int n, int np, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_pi0, dim3(grid_for((size_t)np)), dim3(256), 0, s, pi0, S, n, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_dielectric(const double* v, const double* S, double* A, int np, int nb,
+                             hipStream_t s) {
+    const int gx = grid_for((size_t)np * np, 256, 2048);
+    hipLaunchKernelGGL(k_dielectric, dim3(gx, nb), dim3(256), 0, s, v, S, A, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipStream_t s) {
+    const int nblk = np / kNB;
+    for (int k = 0; k < nblk; ++k) {
+        hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k);
+        const int t = nblk - k - 1;
+        if (t > 0) {
+            hipLaunchKernelGGL(k_trsm_panel, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k);
+            hipLaunchKernelGGL(k_syrk, dim3(t * (t + 1) / 2, nb), dim3(256), 0, s, A, np, k);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trtri(double* A, const double* dinv, double* tbuf, int np, int nb,
+                        hipStream_t s) {
+    const int nblk = np / kNB;
+    for (int j = nblk - 1; j >= 0; --j) {
+        const int t = nblk - 1 - j;
+        if (t > 0) hipLaunchKernelGGL(k_trtri_t, dim3(t, nb), dim3(256), 0, s, A, tbuf, np, j);
+        hipLaunchKernelGGL(k_trtri_m, dim3(t + 1, nb), dim3(256), 0, s, A, dinv, tbuf, np, j);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_f64(const double* M, const double* S, const double* src,
+                               const int64_t* pkrow, const int* colidx, double* qpart, int np,
+                               int mpad, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_contract_f64, dim3(mpad / kCT, np / kCT, nb), dim3(256), 0, s, M, S, src,
+                       pkrow, colidx, qpart, np, mpad);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_q(const double* qpart, double* q, int np, int m, int mpad, int f0,
+                           int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_q, dim3(grid_for((size_t)m, 256, 4096), nb), dim3(256), 0, s,
+                       qpart, q, np / kCT, m, mpad, f0);
+    return hipGetLastError();
+}
+
+hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int n, int np,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_wc_explicit, dim3(grid_for((size_t)n * n)), dim3(256), 0, s, M, S, wc, n,
+                       np);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_c(const double* q, const double* mo_energy, const double* omega,
+                          const double* weight, double fermi, int nmo, int nstates, int nfreq,
+                          double* sig_re, double* sig_im, hipStream_t s) {
+    const int total = nstates * nfreq;
+    hipLaunchKernelGGL(k_sigma_c, dim3((total + 127) / 128), dim3(128), 0, s, q, mo_energy, omega,
+                       weight, fermi, nmo, nstates, nfreq, sig_re, sig_im);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s) {
+    hipLaunchKernelGGL(k_sigma_x, dim3((nmo + 63) / 64), dim3(64), 0, s, eri, sx, nmo, nocc);
+    return hipGetLastError();
+}
+
+hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const double* omega,
+                          const double* mo_energy, const double* sx, const double* vxc_diag,
+                          const int* states, double fermi, int nstates, int nfreq, int npar,
+                          double* qp, int* iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_pade_qp, dim3((nstates + 63) / 64), dim3(64), 0, s, sig_re, sig_im, omega,
+                       mo_energy, sx, vxc_diag, states, fermi, nstates, nfreq, npar, qp, iters);
+    return hipGetLastError();
+}
+
+}  // namespace mgw

@@ -1,0 +1,360 @@
+"""CPU oracle for the per-frequency screening + Sigma_c path (test infrastructure).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product (minigw_amd/) imports this
+module; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+use it, and only as the checker / CPU baseline.
+
+It restates the reference's BLAS/LAPACK-backend algorithm in numpy + scipy
+(LAPACK getrf/getrs through scipy.linalg.lu_factor / lu_solve, BLAS gemm
+through numpy), each function citing the reference lines it follows.  One
+change of schedule, exact in arithmetic: the reference rebuilds W_c(w') for
+every external frequency w_n (src/gw/sigma_c_driver.cpp:186-195); q[p,k,w']
+does not depend on w_n, so it is computed once per w' ("hoisted") and the
+f' -> k summation order of the Sigma_c accumulation is kept.
+
+At eta == 0, Pi0 is real (2[1/(iw-D) - 1/(iw+D)] = -4D/(w^2+D^2)), so the
+screening chain is evaluated in real arithmetic; eta != 0 uses complex.
+
+Pinning (tests/test_oracle.py): regression_tests/h2o_serial fixtures
+(E_c_before_Pade.out, E_c.out, gw.out row), all-state outputs of the
+reference's own path (oracle/_ref via tests/golden/make_golden.py), numpy's
+Gauss-Legendre routine, closed forms and invariants.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+
+K_PI = 3.141592653589793238462643383279502884          # include/types.hpp:13
+K_HARTREE_TO_EV = 27.211386245988                       # include/types.hpp:12
+
+
+# --------------------------------------------------------------------------
+# a1: particle-hole basis, ph = i*nvir + a  (include/gw/particle_hole.hpp:22-35)
+# --------------------------------------------------------------------------
+@dataclass
+class PhBasis:
+    nmo: int
+    nocc: int
+    i_occ: np.ndarray
+    a_vir: np.ndarray
+    a_mo: np.ndarray
+    delta: np.ndarray
+
+    @property
+    def size(self) -> int:
+        return int(self.i_occ.shape[0])
+
+
+def particle_hole_basis(mo_energy: np.ndarray, nocc: int) -> PhBasis:
+    nmo = mo_energy.shape[0]
+    if nocc == 0 or nocc >= nmo:                       # include/gw/orbital_space.hpp:16-21
+        raise ValueError("OrbitalSpace: invalid number of occupied orbitals")
+    nvir = nmo - nocc
+    i_occ = np.repeat(np.arange(nocc), nvir)
+    a_vir = np.tile(np.arange(nvir), nocc)
+    a_mo = nocc + a_vir
+    delta = mo_energy[a_mo] - mo_energy[i_occ]
+    return PhBasis(nmo, nocc, i_occ, a_vir, a_mo, delta)
+
+
+# --------------------------------------------------------------------------
+# a2: transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:12-44,62-93)
+# --------------------------------------------------------------------------
+def gauleg(x1: float, x2: float, n: int):
+    """Numerical-Recipes Gauss-Legendre nodes by Newton iteration, eps = 3e-14."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    eps = 3e-14
+    m = (n + 1) // 2
+    xm = 0.5 * (x2 + x1)
+    xl = 0.5 * (x2 - x1)
+    for i in range(m):
+        z = math.cos(K_PI * ((i + 1) - 0.25) / (n + 0.5))
+        pp = 0.0
+        for _ in range(100):
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p3 = p2
+                p2 = p1
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j
+            pp = n * (z * p1 - p2) / (z * z - 1.0)
+            z_old = z
+            z = z_old - p1 / pp
+            if abs(z - z_old) <= eps:
+                break
+        x[i] = xm - xl * z
+        x[n - 1 - i] = xm + xl * z
+        w[i] = 2.0 * xl / ((1.0 - z * z) * pp * pp)
+        w[n - 1 - i] = w[i]
+    return x, w
+
+
+def transformed_legendre_grid(nfreq: int):
+    """x = (1+y)/(2(1-y)), w = w_y/(1-y)^2, sorted ascending, x0 clamped >= 1e-12."""
+    if nfreq <= 0:
+        raise ValueError("generate_transformed_legendre_grid: num_freq_points must be positive")
+    y, wy = gauleg(-1.0, 1.0, nfreq)
+    tv = 0.5 * (1.0 + y) / (1.0 - y)
+    tw = wy / ((1.0 - y) * (1.0 - y))
+    order = np.argsort(tv, kind="stable")
+    sv, sw = tv[order].copy(), tw[order].copy()
+    if sv[0] < 1e-12:
+        sv[0] = 1e-12
+    return sv, sw
+
+
+# --------------------------------------------------------------------------
+# a4: Pi0 diagonal (src/gw/sigma.cpp:41-54)
+# --------------------------------------------------------------------------
+def pi0_diag(omega: float, delta: np.ndarray, eta: float) -> np.ndarray:
+    """2 [1/(i w - D + i eta) - 1/(i w + D - i eta)] (complex, as the reference)."""
+    iw = 1j * omega
+    ieta = 1j * eta
+    return 2.0 * (1.0 / (iw - delta + ieta) - 1.0 / (iw + delta - ieta))
+
+
+def pi0_diag_real(omega: float, delta: np.ndarray) -> np.ndarray:
+    """Closed form of pi0_diag at eta == 0: -4 D / (w^2 + D^2)."""
+    return -4.0 * delta / (omega * omega + delta * delta)
+
+
+# --------------------------------------------------------------------------
+# a5 / a10 / a15: gathers from the C-order ERI (src/gw/sigma.cpp:22-39,56-71;
+# src/workspace/pq_ph_panel.cpp:25-47)
+# --------------------------------------------------------------------------
+def v_ph_matrix(eri: np.ndarray, ph: PhBasis) -> np.ndarray:
+    """V[jb, ia] = (j b | i a): row = jb, col = ia."""
+    return _v_ph(eri, ph)
+
+
+def _v_ph(eri: np.ndarray, ph: PhBasis) -> np.ndarray:
+    sub = eri[:ph.nocc, ph.nocc:, :ph.nocc, ph.nocc:]           # (i, a, j, b) blocks
+    n = ph.size
+    return np.ascontiguousarray(sub.reshape(n, n))
+
+
+def exchange_matrix(eri: np.ndarray, nocc: int) -> np.ndarray:
+    """Sigma_x[p, q] = -sum_{i<nocc} (p i | q i)."""
+    nmo = eri.shape[0]
+    out = np.zeros((nmo, nmo))
+    for q in range(nmo):
+        for p in range(nmo):
+            v = 0.0
+            for k in range(nocc):
+                v -= eri[p, k, q, k]
+            out[p, q] = v
+    return out
+
+
+def exchange_diag(eri: np.ndarray, nocc: int) -> np.ndarray:
+    nmo = eri.shape[0]
+    out = np.zeros(nmo)
+    for p in range(nmo):
+        v = 0.0
+        for k in range(nocc):
+            v -= eri[p, k, p, k]
+        out[p] = v
+    return out
+
+
+def pq_panel(eri: np.ndarray, ph: PhBasis, p: int, k0: int, width: int) -> np.ndarray:
+    """X[ph, kk] = (p, k0+kk | i a), shape n_ph x width."""
+    nmo = eri.shape[0]
+    if p >= nmo:
+        raise ValueError("PqPhPanelView::fill_panel: p index out of range")
+    if k0 > nmo or width > nmo - k0:
+        raise ValueError("PqPhPanelView::fill_panel: k panel exceeds nmo")
+    blk = eri[p, k0:k0 + width, :ph.nocc, ph.nocc:]              # (width, nocc, nvir)
+    return np.ascontiguousarray(blk.reshape(width, ph.size).T)
+
+
+# --------------------------------------------------------------------------
+# a7-a9: W_c = (I - diag(Pi0) V)^-1 diag(Pi0)   (src/workspace/screening_workspace.cpp:21-66,
+#        BlasLapackBackend::inverse = getrf + getrs against I, src/linalg/linalg_blas_lapack.cpp:106-140)
+# --------------------------------------------------------------------------
+def compute_w_c(v: np.ndarray, pi0: np.ndarray) -> np.ndarray:
+    n = v.shape[0]
+    if pi0.shape[0] != n:
+        raise ValueError("HostScreeningWorkspace::compute_w_c: inconsistent pi0 dimension")
+    eps = -pi0[:, None] * v
+    eps[np.diag_indices(n)] += 1.0
+    lu, piv = sla.lu_factor(eps, check_finite=False)
+    if np.any(np.diag(lu) == 0):
+        raise np.linalg.LinAlgError("getrf: matrix is singular")
+    inv = sla.lu_solve((lu, piv), np.eye(n, dtype=eps.dtype), check_finite=False)
+    return inv * pi0[None, :]
+
+
+# a11: q_j = sum_i X[i, j] (W X)[i, j]   (src/workspace/screening_workspace.cpp:69-102)
+def quadratic_forms_panel(w_c: np.ndarray, x: np.ndarray) -> np.ndarray:
+    n = w_c.shape[0]
+    if w_c.shape != (n, n) or x.shape[0] != n:
+        raise ValueError("HostScreeningWorkspace::quadratic_forms_panel: inconsistent dimensions")
+    y = w_c @ x
+    return np.einsum("ij,ij->j", x, y)
+
+
+# --------------------------------------------------------------------------
+# a12: Sigma_c(p, i w_n), hoisted over w'   (src/gw/sigma_c_driver.cpp:119-169)
+# --------------------------------------------------------------------------
+def quadratic_forms_all(eri: np.ndarray, ph: PhBasis, omegas: np.ndarray, states, eta: float,
+                        progress=None) -> np.ndarray:
+    """q[f', s, k] = x_{p_s k}^T W_c(w'_f) x_{p_s k} for every grid point, state, k."""
+    nmo = eri.shape[0]
+    v = _v_ph(eri, ph)
+    x = eri[np.asarray(states)][:, :, :ph.nocc, ph.nocc:].reshape(len(states) * nmo, ph.size).T
+    x = np.ascontiguousarray(x)
+    real = eta == 0.0
+    q = np.zeros((len(omegas), len(states), nmo), dtype=np.float64 if real else np.complex128)
+    for f, w in enumerate(omegas):
+        pi0 = pi0_diag_real(w, ph.delta) if real else pi0_diag(w, ph.delta, eta)
+        wc = compute_w_c(v, pi0)
+        q[f] = quadratic_forms_panel(wc, x).reshape(len(states), nmo)
+        if progress:
+            progress(f)
+    return q
+
+
+def sigma_from_q(q: np.ndarray, mo_energy: np.ndarray, fermi: float, omegas: np.ndarray,
+                 weights: np.ndarray) -> np.ndarray:
+    """Sigma[s, n] = (1/pi) * sum_{f'} sum_k -g0(k, n, f') q[f', s, k] w_{f'}, f' outer, k inner.
+
+    g0 = d / (d^2 - (i w')^2), d = i w_n + E_F - e_k (sigma_c_driver.cpp:154-160).
+    """
+    nf, ns, nmo = q.shape
+    d = (fermi - mo_energy)[:, None] + 1j * omegas[None, :]          # (k, n)
+    acc = np.zeros((ns, nf), dtype=np.complex128)
+    for f in range(nf):
+        iwp = 1j * omegas[f]
+        g0 = d / (d * d - iwp * iwp)                                   # (k, n)
+        for k in range(nmo):
+            acc -= (g0[k][None, :] * q[f, :, k][:, None]) * weights[f]
+    return acc / K_PI
+
+
+# --------------------------------------------------------------------------
+# a16/a17: Thiele continued-fraction Pade (src/gw/pade.cpp:10-101)
+# --------------------------------------------------------------------------
+def pade_coefficients(x, y, npar: int):
+    n = len(x)
+    if len(y) != n:
+        raise ValueError("Pade: x and y size mismatch")
+    if npar == 0:
+        raise ValueError("Pade: npar must be positive")
+    if n == 0:
+        raise ValueError("Pade: no input points")
+    npa = min(npar, n)
+    if npa == 1:
+        return [complex(y[0])], [complex(x[0])]
+    step = n // (npa - 1)
+    if step == 0:
+        sx = [complex(x[i]) for i in range(npa)]
+        sy = [complex(y[i]) for i in range(npa)]
+    else:
+        sx, sy = [], []
+        idx = 0
+        for _ in range(npa - 1):
+            sx.append(complex(x[idx]))
+            sy.append(complex(y[idx]))
+            idx += step
+        sx.append(complex(x[n - 1]))
+        sy.append(complex(y[n - 1]))
+    g = [[0j] * npa for _ in range(npa)]
+    for i in range(npa):
+        g[i][0] = sy[i]
+    for ip in range(1, npa):
+        for idat in range(ip, npa):
+            num = g[ip - 1][ip - 1] - g[idat][ip - 1]
+            den = (sx[idat] - sx[ip - 1]) * g[idat][ip - 1]
+            g[idat][ip] = num / (1e-20 + 0j) if abs(den) < 1e-20 else num / den
+    return [g[i][i] for i in range(npa)], sx
+
+
+def pade_evaluate(coeffs, z: float, xs) -> complex:
+    npar = len(coeffs)
+    if npar == 0:
+        return 0j
+    if npar == 1:
+        return coeffs[0]
+    zt = complex(z, 0.0)
+    val = 1 + 0j
+    for k in range(npar - 2, -1, -1):
+        num = coeffs[k + 1] * (zt - xs[k])
+        if abs(val) < 1e-20:
+            val = 1e-20 + 0j
+        val = 1 + num / val
+    if abs(val) < 1e-20:
+        return coeffs[0] / (1e-20 + 0j)
+    return coeffs[0] / val
+
+
+# --------------------------------------------------------------------------
+# a18: diagonal QP fixed point (src/gw/qp_solver.cpp:10-47)
+# --------------------------------------------------------------------------
+def solve_qp(mo_energy, fermi, sigma_x_diag, vxc_diag, sigma_c, omegas, states, npar,
+             max_iterations: int = 200, tolerance: float = 1e-6):
+    """Returns (qp[nmo] with NaN for unselected states, iterations[nmo])."""
+    nmo = len(mo_energy)
+    qp = np.full(nmo, np.nan)
+    iters = np.zeros(nmo, dtype=np.int64)
+    xs_im = [complex(0.0, w) for w in omegas]
+    for si, s in enumerate(states):
+        cur = float(mo_energy[s])
+        coeffs, sx = pade_coefficients(xs_im, list(sigma_c[si]), npar)
+        it_used = max_iterations
+        for it in range(max_iterations):
+            sc = pade_evaluate(coeffs, cur - fermi, sx)
+            new = float(mo_energy[s]) + sc.real + float(sigma_x_diag[s]) - float(vxc_diag[s])
+            if abs(new - cur) < tolerance:
+                cur = new
+                it_used = it + 1
+                break
+            cur = new
+        qp[s] = cur
+        iters[s] = it_used
+    return qp, iters
+
+
+# --------------------------------------------------------------------------
+# run_g0w0 (src/gw/gw.cpp:125-191), hoisted
+# --------------------------------------------------------------------------
+@dataclass
+class OracleResult:
+    omegas: np.ndarray
+    weights: np.ndarray
+    states: list
+    sigma_x_diag: np.ndarray
+    q: np.ndarray                 # (nfreq, nstates, nmo)
+    sigma_c: np.ndarray           # (nstates, nfreq)
+    qp: np.ndarray                # (nmo,), NaN for unselected
+    qp_iterations: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+
+def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.0,
+             progress=None) -> OracleResult:
+    ph = particle_hole_basis(inp.mo_energy, inp.nocc)
+    omegas, weights = transformed_legendre_grid(nfreq)
+    states = list(range(inp.nmo)) if states is None else list(states)
+    sx = exchange_diag(inp.eri, inp.nocc)
+    q = quadratic_forms_all(inp.eri, ph, omegas, states, eta, progress)
+    sigma = sigma_from_q(q, inp.mo_energy, inp.fermi_energy, omegas, weights)
+    qp, iters = solve_qp(inp.mo_energy, inp.fermi_energy, sx, np.diag(inp.vxc), sigma,
+                         omegas, states, npar)
+    return OracleResult(omegas, weights, states, sx, q, sigma, qp, iters)
+
+
+def pade_real_axis(sigma_state, omegas, npar, fermi, num_points=19836, x_start=-6.5,
+                   interval=0.00065):
+    """E_c.out rows (src/io.cpp:94-108): (E in eV, Re, Im) of the continued fraction."""
+    coeffs, sx = pade_coefficients([complex(0.0, w) for w in omegas], list(sigma_state), npar)
+    out = np.zeros((num_points, 3))
+    for i in range(num_points):
+        e_iter = x_start + i * interval - fermi
+        v = pade_evaluate(coeffs, e_iter, sx)
+        out[i] = ((e_iter + fermi) * K_HARTREE_TO_EV, v.real, v.imag)
+    return out

@@ -1,0 +1,103 @@
+// Test-infrastructure driver for the reference's own G0W0 path (oracle/_ref).
+//
+// Compiled only by oracle/Makefile against the reference sources where they
+// lie; it is never part of the product library.  It feeds a raw binary input
+// to gw::run_g0w0 (src/gw/gw.cpp:125-191) with the self-contained
+// ReferenceBackend (src/linalg/linalg.cpp:44-91, Gauss-Jordan inverse) -- the
+// backend that produced regression_tests/h2o_serial -- and writes everything
+// the parity tests compare: the frequency grid, Sigma_x, Sigma_c(i w) for the
+// selected states and the QP energies.
+//
+// Input file (little endian):
+//   int64 nmo, nocc, nfreq, npar, state (-1 = all states, else 0-based),
+//         threads (OpenMP frequency parallelism if > 1)
+//   double fermi_energy, eta
+//   double mo_energy[nmo], vxc[nmo*nmo], eri[nmo^4]  (C order)
+// Output file:
+//   int64 nmo, nfreq
+//   double omega[nfreq], weight[nfreq], sigma_x_diag[nmo],
+//          sigma_c[nmo*nfreq*2] (row-major state x freq, re/im interleaved),
+//          qp_energy[nmo] (NaN for unselected states)
+#include "gw/gw.hpp"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <vector>
+
+namespace {
+
+template <class T>
+void read_into(std::ifstream& in, T* dst, std::size_t count) {
+    in.read(reinterpret_cast<char*>(dst), static_cast<std::streamsize>(count * sizeof(T)));
+    if (!in) throw std::runtime_error("ref_driver: short read");
+}
+
+template <class T>
+void write_from(std::ofstream& out, const T* src, std::size_t count) {
+    out.write(reinterpret_cast<const char*>(src), static_cast<std::streamsize>(count * sizeof(T)));
+}
+
+} // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        std::cerr << "usage: gw_ref <input.bin> <output.bin>\n";
+        return 2;
+    }
+    try {
+        std::ifstream in(argv[1], std::ios::binary);
+        if (!in) throw std::runtime_error("cannot open input");
+        std::int64_t hdr[6];
+        read_into(in, hdr, 6);
+        double fe[2];
+        read_into(in, fe, 2);
+        const std::size_t nmo = static_cast<std::size_t>(hdr[0]);
+        gw::GwInput input;
+        input.nocc = static_cast<std::size_t>(hdr[1]);
+        input.fermi_energy = fe[0];
+        input.mo_energy.resize(nmo);
+        read_into(in, input.mo_energy.data(), nmo);
+        std::vector<double> vxc(nmo * nmo);
+        read_into(in, vxc.data(), vxc.size());
+        input.vxc_mo = gw::MatrixReal(nmo, nmo, std::move(vxc));
+        std::vector<double> eri(nmo * nmo * nmo * nmo);
+        read_into(in, eri.data(), eri.size());
+        input.eri_mo = gw::Tensor4Real(nmo, nmo, nmo, nmo, std::move(eri));
+
+        gw::GwSettings settings;
+        settings.num_freq_points_total = static_cast<std::size_t>(hdr[2]);
+        settings.num_pade_params = static_cast<std::size_t>(hdr[3]);
+        if (hdr[4] < 0) {
+            settings.selected_state_0based.reset();
+        } else {
+            settings.selected_state_0based = static_cast<std::size_t>(hdr[4]);
+        }
+        settings.eta = fe[1];
+        if (hdr[5] > 1) {
+            settings.execution.frequency_parallel_mode = gw::FrequencyParallelMode::OpenMP;
+            settings.execution.openmp_frequency_parallel = true;
+        }
+
+        const gw::GwResult result = gw::run_g0w0(input, settings);
+
+        std::ofstream out(argv[2], std::ios::binary);
+        const std::int64_t ohdr[2] = {static_cast<std::int64_t>(nmo), hdr[2]};
+        write_from(out, ohdr, 2);
+        write_from(out, result.omegas.data(), result.omegas.size());
+        write_from(out, result.weights.data(), result.weights.size());
+        std::vector<double> sx(nmo);
+        for (std::size_t p = 0; p < nmo; ++p) sx[p] = result.sigma_x(p, p);
+        write_from(out, sx.data(), nmo);
+        write_from(out, reinterpret_cast<const double*>(result.sigma_c_im_points.data().data()),
+                   2 * result.sigma_c_im_points.data().size());
+        write_from(out, result.qp_energy.data(), result.qp_energy.size());
+    } catch (const std::exception& ex) {
+        std::cerr << "ref_driver error: " << ex.what() << '\n';
+        return 1;
+    }
+    return 0;
+}

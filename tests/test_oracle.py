@@ -1,0 +1,302 @@
+"""Pins the CPU oracle (oracle/oracle.py) before anything is checked against it.
+
+Each check is chosen so that a plausible slip (a dropped term, a transposed
+operand, a wrong sign or index) fails: reference fixtures, outputs of the
+reference's own path, library routines and closed forms.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from minigw_amd.synthetic import synthetic_input, GwInput
+from oracle import oracle as orc
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def isclose_components(a, b, rtol, atol):
+    """math.isclose on real and imaginary parts separately, as the reference's
+    comparison harness does (scripts/compare_reference.py:93-96)."""
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    assert a.shape == b.shape
+    bad = []
+    for i, (x, y) in enumerate(zip(a, b)):
+        if not (math.isclose(x.real, y.real, rel_tol=rtol, abs_tol=atol)
+                and math.isclose(x.imag, y.imag, rel_tol=rtol, abs_tol=atol)):
+            bad.append((i, x, y))
+    return bad
+
+
+def h2o_input():
+    d = np.load(os.path.join(GOLD, "h2o_input.npz"))
+    return GwInput(mo_energy=d["mo_energy"], eri=d["eri"], vxc=d["vxc"],
+                   nocc=int(d["nocc"]), fermi_energy=float(d["fermi_energy"]))
+
+
+def ref_case(name):
+    return dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
+
+
+# ---------------------------------------------------------------- generator
+def test_generator_deterministic_and_symmetric():
+    a = synthetic_input(9, 3, 123)
+    b = synthetic_input(9, 3, 123)
+    c = synthetic_input(9, 3, 124)
+    assert np.array_equal(a.eri, b.eri) and np.array_equal(a.mo_energy, b.mo_energy)
+    assert not np.array_equal(a.eri, c.eri)
+    e = a.eri
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.array_equal(e, e.transpose(perm))
+    m = e.reshape(81, 81)
+    assert np.linalg.eigvalsh(m).min() > -1e-12
+    assert np.linalg.matrix_rank(m, tol=1e-10) <= 3 * 9
+    occ, vir = a.mo_energy[:3], a.mo_energy[3:]
+    assert np.all(np.diff(occ) >= 0) and np.all(np.diff(vir) >= 0)
+    assert occ.min() >= -1.2 and occ.max() <= -0.3 and vir.min() >= 0.05 and vir.max() <= 3.0
+    assert a.fermi_energy == 0.5 * (occ[-1] + vir[0])
+    assert np.array_equal(a.vxc, a.vxc.T)
+
+
+def test_generator_splitmix_known_answer():
+    # SplitMix64 with seed 0: the published first outputs of the algorithm.
+    from minigw_amd.synthetic import SplitMix64
+    r = SplitMix64(0).u64(3)
+    assert [int(x) for x in r] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_generator_normal_moments():
+    from minigw_amd.synthetic import SplitMix64
+    z = SplitMix64(5).normal(200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+    assert abs((z ** 4).mean() - 3.0) < 0.06
+
+
+# ---------------------------------------------------------------- a1, a2, a4
+def test_ph_basis_order():
+    mo = np.array([-1.0, -0.5, 0.2, 0.9, 1.7])
+    ph = orc.particle_hole_basis(mo, 2)
+    pairs = [(i, a) for i in range(2) for a in range(2, 5)]
+    assert list(zip(ph.i_occ, ph.a_mo)) == pairs
+    assert np.allclose(ph.delta, [mo[a] - mo[i] for i, a in pairs])
+    with pytest.raises(ValueError):
+        orc.particle_hole_basis(mo, 5)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 16, 200])
+def test_legendre_grid_vs_numpy(n):
+    y, wy = np.polynomial.legendre.leggauss(n)
+    x, w = orc.gauleg(-1.0, 1.0, n)
+    assert np.allclose(x, y, rtol=0, atol=1e-14)
+    # NR Newton stops at |dz| <= 3e-14, so its weights carry ~3e-11 relative error
+    assert np.allclose(w, wy, rtol=1e-10, atol=1e-15)
+    om, ow = orc.transformed_legendre_grid(n)
+    tv = 0.5 * (1 + y) / (1 - y)
+    tw = wy / (1 - y) ** 2
+    assert np.all(np.diff(om) > 0)
+    assert np.allclose(om[1:], np.sort(tv)[1:], rtol=1e-12)
+    assert np.allclose(ow, tw[np.argsort(tv)], rtol=1e-10)
+    # integral of 1/(1+x^2) over [0, inf) = pi/2 (n large enough)
+    if n >= 16:
+        assert abs(np.sum(ow / (1 + om ** 2)) - math.pi / 2) < 1e-6
+
+
+def test_pi0_closed_form():
+    rng = np.random.default_rng(0)
+    delta = rng.uniform(0.01, 5.0, 1000)
+    for w in [1e-12, 0.1, 3.0, 500.0]:
+        c = orc.pi0_diag(w, delta, 0.0)
+        r = orc.pi0_diag_real(w, delta)
+        assert np.all(c.imag == 0.0)
+        assert np.allclose(c.real, r, rtol=1e-14, atol=0)
+        assert np.all(r < 0)
+
+
+# ---------------------------------------------------------------- gathers
+def test_gathers_brute_force():
+    inp = synthetic_input(7, 2, 3)
+    ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+    v = orc.v_ph_matrix(inp.eri, ph)
+    for r in range(ph.size):
+        for c in range(ph.size):
+            assert v[r, c] == inp.eri[ph.i_occ[r], ph.a_mo[r], ph.i_occ[c], ph.a_mo[c]]
+    x = orc.pq_panel(inp.eri, ph, 3, 2, 4)
+    for r in range(ph.size):
+        for kk in range(4):
+            assert x[r, kk] == inp.eri[3, 2 + kk, ph.i_occ[r], ph.a_mo[r]]
+    sx = orc.exchange_matrix(inp.eri, inp.nocc)
+    assert sx[4, 5] == -(inp.eri[4, 0, 5, 0] + inp.eri[4, 1, 5, 1])
+    assert np.array_equal(np.diag(sx), orc.exchange_diag(inp.eri, inp.nocc))
+    with pytest.raises(ValueError):
+        orc.pq_panel(inp.eri, ph, 7, 0, 1)
+    with pytest.raises(ValueError):
+        orc.pq_panel(inp.eri, ph, 0, 5, 3)
+
+
+# ---------------------------------------------------------------- W_c invariants
+def test_wc_invariants():
+    inp = synthetic_input(9, 3, 99)
+    ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+    v = orc.v_ph_matrix(inp.eri, ph)
+    for w in [1e-12, 0.3, 10.0]:
+        pi0 = orc.pi0_diag_real(w, ph.delta)
+        wc = orc.compute_w_c(v, pi0)
+        alt = np.linalg.inv(np.diag(1.0 / pi0) - v)             # (Pi0^-1 - V)^-1
+        assert np.allclose(wc, alt, rtol=1e-10, atol=1e-12)
+        assert np.allclose(wc, wc.T, rtol=1e-10, atol=1e-13)
+        assert np.linalg.eigvalsh(0.5 * (wc + wc.T)).max() < 0
+        eps = np.eye(ph.size) - pi0[:, None] * v
+        assert np.allclose(eps @ wc, np.diag(pi0), atol=1e-12)
+    # n = 1 closed form Pi0 / (1 - Pi0 V)
+    wc1 = orc.compute_w_c(np.array([[0.7]]), np.array([-0.4]))
+    assert abs(wc1[0, 0] - (-0.4 / (1 + 0.4 * 0.7))) < 1e-16
+    # a dropped Pi0 scale or a transposed V is caught (V non-symmetric here)
+    vn = v + 0.01 * np.triu(np.ones_like(v), 1)
+    pi0 = orc.pi0_diag_real(0.3, ph.delta)
+    wc = orc.compute_w_c(vn, pi0)
+    assert np.allclose(wc, np.linalg.inv(np.diag(1.0 / pi0) - vn), atol=1e-12)
+    assert not np.allclose(wc, np.linalg.inv(np.diag(1.0 / pi0) - vn.T), atol=1e-8)
+
+
+def test_quadratic_forms_brute_force():
+    rng = np.random.default_rng(1)
+    w = rng.normal(size=(6, 6))
+    x = rng.normal(size=(6, 3))
+    q = orc.quadratic_forms_panel(w, x)
+    for j in range(3):
+        assert abs(q[j] - x[:, j] @ w @ x[:, j]) < 1e-12
+
+
+# ---------------------------------------------------------------- Pade / QP
+def test_pade_reproduces_sampled_points():
+    rng = np.random.default_rng(2)
+    om = np.sort(rng.uniform(0.01, 20, 40))
+    xs = [complex(0, w) for w in om]
+    ys = [complex(1.0 / (1j * w - 0.5)) + 0.3 / (1j * w + 1.2) for w in om]
+    coeffs, sx = orc.pade_coefficients(xs, ys, 8)
+    step = 40 // 7
+    idx = [i * step for i in range(7)] + [39]
+    for i in idx:
+        # evaluate at an imaginary point by hand (continued fraction, same recursion)
+        z = xs[i]
+        val = 1 + 0j
+        for k in range(len(coeffs) - 2, -1, -1):
+            val = 1 + coeffs[k + 1] * (z - sx[k]) / val
+        assert abs(coeffs[0] / val - ys[i]) < 1e-9 * max(1, abs(ys[i]))
+    # a two-pole function is represented exactly on the real axis too
+    f = lambda e: 1.0 / (e - 0.5) + 0.3 / (e + 1.2)
+    for e in [-2.0, 0.1, 0.9, 3.0]:
+        assert abs(orc.pade_evaluate(coeffs, e, sx) - f(e)) < 1e-8
+
+
+def test_qp_one_pole_fixed_point():
+    # Sigma_c(w) = a / (w - b): QP solves E = e + a/(E - EF - b) + sx - vxc
+    a, b, e0, ef, sx, vx = 0.05, -0.8, -0.4, -0.1, -0.3, -0.25
+    om, _ = orc.transformed_legendre_grid(60)
+    sig = np.array([[a / (1j * w - b) for w in om]])
+    qp, it = orc.solve_qp(np.array([e0]), ef, np.array([sx]), np.array([vx]), sig, om, [0], 16)
+    c = e0 + sx - vx
+    # E^2 - (c + EF + b) E + (c (EF + b) - a) = 0, the root near e0
+    B = c + ef + b
+    disc = math.sqrt(B * B - 4 * (c * (ef + b) - a))
+    roots = [(B + disc) / 2, (B - disc) / 2]
+    exact = min(roots, key=lambda r: abs(r - e0))
+    assert abs(qp[0] - exact) < 2e-6
+    assert 1 <= it[0] < 200
+
+
+# ---------------------------------------------------------------- reference fixtures
+def test_h2o_serial_sigma_c_fixture():
+    inp = h2o_input()
+    fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
+    res = orc.run_g0w0(inp, nfreq=200, npar=16, states=[4])
+    bad = isclose_components(res.sigma_c[0], fx["e_c_before_pade"], 1e-8, 1e-8)
+    assert not bad, bad[:3]
+    diff = np.abs(res.sigma_c[0] - fx["e_c_before_pade"]).max()
+    assert diff < 1e-11, diff
+    # QP summary row (eV, 6 significant digits) -- compare_reference.py energy check
+    row = fx["summary_row_orbital5_ev"]
+    qp_ev = res.qp[4] * orc.K_HARTREE_TO_EV
+    sc_ev = (res.qp[4] + inp.vxc[4, 4] - inp.mo_energy[4] - res.sigma_x_diag[4]) * orc.K_HARTREE_TO_EV
+    for got, want in [(inp.mo_energy[4] * orc.K_HARTREE_TO_EV, row[0]),
+                      (res.sigma_x_diag[4] * orc.K_HARTREE_TO_EV, row[1]), (sc_ev, row[2]),
+                      (inp.vxc[4, 4] * orc.K_HARTREE_TO_EV, row[3]), (qp_ev, row[4])]:
+        assert math.isclose(got, want, rel_tol=1e-5, abs_tol=1e-5), (got, want)
+
+
+def test_h2o_serial_pade_fixture():
+    inp = h2o_input()
+    fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
+    om, _ = orc.transformed_legendre_grid(200)
+    ec = orc.pade_real_axis(fx["e_c_before_pade"], om, 16, inp.fermi_energy)
+    ref = fx["e_c"]
+    assert ec.shape == ref.shape
+    assert np.allclose(ec[:, 0], ref[:, 0], rtol=1e-15, atol=1e-12)
+    err = np.abs(ec[:, 1:] - ref[:, 1:]) / np.maximum(1.0, np.abs(ref[:, 1:]))
+    assert err.max() < 1e-8, err.max()
+
+
+def test_backend_spread_of_fixtures():
+    fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
+    base = fx["e_c_before_pade"]
+    for k in fx.files:
+        if k.startswith("e_c_before_pade_"):
+            assert np.abs(fx[k] - base).max() < 1e-11
+
+
+def _check_chain(inp, r, sigma_tol):
+    res = orc.run_g0w0(inp, nfreq=int(r["nfreq"]), npar=int(r["npar"]), eta=float(r["eta"]))
+    assert np.allclose(res.omegas, r["omega"], rtol=1e-15, atol=0)
+    assert np.allclose(res.weights, r["weight"], rtol=1e-14)
+    assert np.allclose(res.sigma_x_diag, r["sigma_x_diag"], rtol=1e-14, atol=1e-15)
+    assert np.abs(res.sigma_c - r["sigma_c"]).max() < sigma_tol
+    # Pade + QP restated exactly: on the reference's own Sigma_c it gives the
+    # reference's QP energies bit for bit.
+    qp, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, r["sigma_x_diag"], np.diag(inp.vxc),
+                         r["sigma_c"], r["omega"], list(range(inp.nmo)), int(r["npar"]))
+    assert np.array_equal(qp, r["qp"])
+    # Full chain: QP within 1e-8 Ha wherever the QP energy is well conditioned,
+    # and inside the envelope that ulp-level noise on Sigma_c produces elsewhere.
+    env = qp_envelope(inp, r)
+    dq = np.abs(res.qp - r["qp"])
+    assert np.all(dq <= np.maximum(1e-8, 10.0 * env)), (dq, env)
+    return res, env
+
+
+def qp_envelope(inp, r, rel=1e-13, trials=4):
+    """Per-state spread of the QP energy under relative noise `rel` on Sigma_c."""
+    rng = np.random.default_rng(12345)
+    out = np.zeros(inp.nmo)
+    for _ in range(trials):
+        sc = r["sigma_c"] * (1 + rel * rng.standard_normal(r["sigma_c"].shape))
+        qp, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, r["sigma_x_diag"],
+                             np.diag(inp.vxc), sc, r["omega"], list(range(inp.nmo)), int(r["npar"]))
+        out = np.maximum(out, np.abs(qp - r["qp"]))
+    return out
+
+
+@pytest.mark.parametrize("name", ["syn_small", "syn_odd", "syn_small_eta"])
+def test_oracle_vs_reference_small(name):
+    r = ref_case(name)
+    inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
+                  fermi_energy=float(r["fermi_energy"]))
+    _check_chain(inp, r, 1e-12)
+
+
+def test_oracle_vs_reference_h2o_all_states():
+    _, env = _check_chain(h2o_input(), ref_case("h2o_all"), 1e-11)
+    assert np.sum(env < 1e-8) >= 3          # some states are well conditioned
+
+
+def test_oracle_vs_reference_h2o_eta():
+    _check_chain(h2o_input(), ref_case("h2o_eta"), 1e-11)
+
+
+def test_oracle_vs_reference_synthetic_c1():
+    r = ref_case("syn_c1")
+    from minigw_amd.synthetic import CONFIGS
+    inp = synthetic_input(*CONFIGS["c1"])
+    assert int(r["nmo"]) == inp.nmo and float(r["fermi_energy"]) == inp.fermi_energy
+    _check_chain(inp, r, 1e-11)
