@@ -148,9 +148,11 @@ typedef struct mgw_g0w0_out {
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
 
 /* Phase times of the last mgw_hip_g0w0 call in milliseconds (HIP events on the
- * context's stream): [0] V build, [1] dielectric build, [2] factorization
- * (potrf + trtri), [3] contraction, [4] Sigma_c quadrature, [5] Pade + QP,
- * [6] total.  n is the number of entries the caller provides (<= 8). */
+ * context's stream): [0] V build (+ X split), [1] dielectric build,
+ * [2] factorization (potrf + trtri), [3] contraction kernel, [4] Sigma_c
+ * quadrature, [5] Pade + QP, [6] total, [7] frequencies per batch,
+ * [8] contraction launches, [9] M split + q reduction.  n is the number of
+ * entries the caller provides (<= 12). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 
 /* Diagnostic with no counterpart in the reference: C = A B^T for 64 x 64

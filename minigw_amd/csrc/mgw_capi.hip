@@ -49,7 +49,7 @@ struct mgw_hip_ctx {
     std::string error;
     size_t budget = 0;     // 0 = no cap
     size_t held = 0;
-    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double timings[12] = {0};
 
     // grow-only device buffer with budget accounting
     struct Buf {
@@ -634,7 +634,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         }
         hip_check(hipMemsetAsync(info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
 
-        float t_diel = 0, t_fact = 0, t_contr = 0;
+        float t_diel = 0, t_fact = 0, t_contr = 0, t_aux = 0;
+        int n_contr = 0;
         for (int f0 = 0; f0 < nfreq; f0 += nb) {
             const int b = std::min(nb, nfreq - f0);
             hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
@@ -643,27 +644,35 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
             factor_batch(ctx, A, dinv, info + f0, tbuf, np, b);
             hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
-            if (mixed) {
+            if (mixed)
                 hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
+            hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
+            if (mixed)
                 hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b, st),
                           "contract split");
-                hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dq, np, m, mpad, f0, b, st),
-                          "reduce q");
-            } else {
+            else
                 hip_check(mgw::launch_contract_f64(A, S, as<double>(ctx->eri.p), dpk,
                                                    as<int>(ctx->colidx.p), qpart, np, mpad, b, st),
                           "contract");
+            hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
+            if (mixed)
+                hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dq, np, m, mpad, f0, b, st),
+                          "reduce q");
+            else
                 hip_check(mgw::launch_reduce_q(qpart, dq, np, m, mpad, f0, b, st), "reduce q");
-            }
             hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
             hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
-            float a = 0, bb = 0, c = 0;
+            float a = 0, bb = 0, c = 0, d = 0, e = 0;
             hip_check(hipEventElapsedTime(&a, ev[2], ev[3]), "hipEventElapsedTime");
             hip_check(hipEventElapsedTime(&bb, ev[3], ev[4]), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&c, ev[4], ev[5]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&c, ev[6], ev[7]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&d, ev[4], ev[6]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&e, ev[7], ev[5]), "hipEventElapsedTime");
             t_diel += a;
             t_fact += bb;
             t_contr += c;
+            t_aux += d + e;
+            ++n_contr;
         }
         check_info(ctx, info, nfreq, "potrf");
         ctx->wc_ready = false;
@@ -705,6 +714,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const double t[7] = {tv, t_diel, t_fact, t_contr, tsig, tqp, ttot};
         std::copy(t, t + 7, ctx->timings);
         ctx->timings[7] = nb;
+        ctx->timings[8] = n_contr;
+        ctx->timings[9] = t_aux;
     });
 }
 
@@ -725,7 +736,7 @@ int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n) {
         g_thread_error = "mgw_hip_last_timings: bad argument";
         return MGW_ERR_ARG;
     }
-    for (int i = 0; i < n && i < 8; ++i) ms[i] = ctx->timings[i];
+    for (int i = 0; i < n && i < 12; ++i) ms[i] = ctx->timings[i];
     return MGW_OK;
 }
 

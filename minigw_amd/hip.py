@@ -1,0 +1,275 @@
+"""Python mirror of the `hip` backend's boundary, over the C ABI (include/mgw_hip.h).
+
+Names and argument meaning follow the reference's device screening interface
+so tests read like the reference's own:
+
+* ``HipScreeningWorkspace(v_ph)`` -- DeviceScreeningWorkspace
+  (include/workspace/device_screening_workspace.hpp:27-59): ``compute_w_c(pi0)``,
+  ``quadratic_forms_panel(x_panel)``, ``nph``, ``estimated_device_bytes``.
+* ``HipPqPhPanelView(eri, nocc)`` -- DevicePqPhPanelView with a resident ERI
+  (include/workspace/device_pq_ph_panel.hpp:67-99), whose panels feed
+  ``quadratic_forms(p, k0, width)``.
+* ``run_g0w0(inp, ...)`` -- run_g0w0 (src/gw/gw.cpp:125-191) with the hoisted,
+  frequency-batched device path.
+
+Errors raise ``HipError("mgw_hip_<fn> failed: <message>")`` like the
+reference's check_* helpers (src/workspace/device_screening_workspace.cu:25-43).
+The library is loaded from minigw_amd/lib/libmgw_hip.so; if it is missing the
+import fails -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmgw_hip.so")
+
+MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_MEMORY, \
+    MGW_ERR_STATE = range(7)
+PREC_FP64, PREC_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class HipError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed: {msg}")
+        self.code = code
+
+
+class G0W0Job(C.Structure):
+    _fields_ = [("nmo", C.c_int32), ("nocc", C.c_int32), ("nfreq", C.c_int32),
+                ("npar", C.c_int32), ("nstates", C.c_int32), ("states", _ip),
+                ("mo_energy", _dp), ("vxc_diag", _dp), ("fermi_energy", C.c_double),
+                ("eta", C.c_double), ("precision", C.c_int32)]
+
+
+class G0W0Out(C.Structure):
+    _fields_ = [("omega", _dp), ("weight", _dp), ("sigma_x_diag", _dp), ("sigma_c_re", _dp),
+                ("sigma_c_im", _dp), ("qp", _dp), ("qp_iterations", _ip)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"HIP library not built: {LIB_PATH} (run minigw_amd/build.py)")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "mgw_hip_version": (C.c_char_p, []),
+        "mgw_hip_last_error": (C.c_char_p, [vp]),
+        "mgw_hip_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "mgw_hip_device_for_local_rank": (C.c_int, [C.c_size_t, C.c_size_t, C.POINTER(C.c_int)]),
+        "mgw_hip_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "mgw_hip_destroy": (C.c_int, [vp]),
+        "mgw_hip_device_bytes": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
+        "mgw_hip_set_memory_budget": (C.c_int, [vp, C.c_size_t]),
+        "mgw_hip_upload_vph": (C.c_int, [vp, _dp, C.c_int32]),
+        "mgw_hip_compute_wc": (C.c_int, [vp, _dp, _dp, C.c_int32]),
+        "mgw_hip_download_wc": (C.c_int, [vp, _dp, _dp, C.c_int32]),
+        "mgw_hip_quadratic_forms_host": (C.c_int, [vp, _dp, C.c_int32, C.c_int32, _dp, _dp]),
+        "mgw_hip_upload_eri": (C.c_int, [vp, _dp, C.c_int32, C.c_int32]),
+        "mgw_hip_build_vph_from_eri": (C.c_int, [vp]),
+        "mgw_hip_quadratic_forms": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
+        "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
+        "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
+        "mgw_hip_selftest_tile": (C.c_int, [vp, _dp, _dp, _dp]),
+        "mgw_grid_transformed_legendre": (C.c_int, [C.c_int32, _dp, _dp]),
+        "mgw_hip_estimate_bytes": (C.c_size_t, [C.c_int32] * 6),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a: np.ndarray | None, kind=_dp):
+    return None if a is None else a.ctypes.data_as(kind)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _check(fn: str, rc: int, ctx=None):
+    if rc != MGW_OK:
+        msg = lib.mgw_hip_last_error(ctx)
+        raise HipError(fn, rc, msg.decode() if msg else "")
+
+
+def transformed_legendre_grid(nfreq: int):
+    om = np.zeros(nfreq)
+    w = np.zeros(nfreq)
+    _check("mgw_grid_transformed_legendre", lib.mgw_grid_transformed_legendre(nfreq, _ptr(om), _ptr(w)))
+    return om, w
+
+
+def estimate_bytes(nmo: int, nocc: int, nfreq: int, nstates: int = 0, nbatch: int = 0,
+                   precision: int = PREC_FP64) -> int:
+    return int(lib.mgw_hip_estimate_bytes(nmo, nocc, nfreq, nstates, nbatch, precision))
+
+
+def device_for_local_rank(local_rank: int, tasks_per_gpu: int) -> int:
+    d = C.c_int(0)
+    _check("mgw_hip_device_for_local_rank", lib.mgw_hip_device_for_local_rank(local_rank, tasks_per_gpu, C.byref(d)))
+    return d.value
+
+
+class HipContext:
+    """One device context (one HIP stream); not thread-safe."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check("mgw_hip_create", lib.mgw_hip_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib.mgw_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def call(self, name: str, *args):
+        _check(name, getattr(lib, name)(self.h, *args), self.h)
+
+    @property
+    def device_bytes(self) -> int:
+        b = C.c_size_t(0)
+        self.call("mgw_hip_device_bytes", C.byref(b))
+        return b.value
+
+    def set_memory_budget(self, nbytes: int):
+        self.call("mgw_hip_set_memory_budget", nbytes)
+
+    def upload_eri(self, eri: np.ndarray, nocc: int):
+        e = _f64(eri)
+        nmo = e.shape[0]
+        assert e.shape == (nmo, nmo, nmo, nmo)
+        self.call("mgw_hip_upload_eri", _ptr(e), nmo, nocc)
+
+    def last_timings(self) -> dict:
+        t = np.zeros(12)
+        self.call("mgw_hip_last_timings", _ptr(t), 12)
+        keys = ["build_v", "dielectric", "factorize", "contract", "sigma_c", "pade_qp", "total",
+                "freq_batch", "contract_launches", "split_reduce"]
+        return dict(zip(keys, t.tolist()))
+
+    def selftest_tile(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        a, b = _f64(a), _f64(b)
+        c = np.zeros((64, 64))
+        self.call("mgw_hip_selftest_tile", _ptr(a), _ptr(b), _ptr(c))
+        return c
+
+
+class HipScreeningWorkspace:
+    """DeviceScreeningWorkspace over the hip backend."""
+
+    def __init__(self, v_ph: np.ndarray, ctx: HipContext | None = None):
+        v = _f64(v_ph)
+        if v.ndim != 2 or v.shape[0] != v.shape[1]:
+            raise HipError("mgw_hip_upload_vph", MGW_ERR_ARG, "V_ph must be square")
+        self.ctx = ctx or HipContext()
+        self._n = v.shape[0]
+        self.ctx.call("mgw_hip_upload_vph", _ptr(v), self._n)
+
+    @property
+    def nph(self) -> int:
+        return self._n
+
+    @property
+    def estimated_device_bytes(self) -> int:
+        return self.ctx.device_bytes
+
+    def compute_w_c(self, pi0: np.ndarray):
+        p = np.asarray(pi0)
+        re = _f64(p.real)
+        im = _f64(p.imag) if np.iscomplexobj(p) else None
+        self.ctx.call("mgw_hip_compute_wc", _ptr(re), _ptr(im), re.shape[0])
+
+    def w_c(self) -> np.ndarray:
+        out = np.zeros((self._n, self._n))
+        self.ctx.call("mgw_hip_download_wc", _ptr(out), None, self._n)
+        return out
+
+    def quadratic_forms_panel(self, x_panel: np.ndarray) -> np.ndarray:
+        x = _f64(x_panel)
+        n, nvec = x.shape
+        q = np.zeros(nvec)
+        self.ctx.call("mgw_hip_quadratic_forms_host", _ptr(x), n, nvec, _ptr(q), None)
+        return q.astype(np.complex128)
+
+
+class HipPqPhPanelView:
+    """DevicePqPhPanelView (full-ERI-resident mode) sharing a workspace context."""
+
+    def __init__(self, eri: np.ndarray, nocc: int, ctx: HipContext):
+        self.ctx = ctx
+        self.nmo = eri.shape[0]
+        self.nocc = nocc
+        ctx.upload_eri(eri, nocc)
+
+    @property
+    def nph(self) -> int:
+        return self.nocc * (self.nmo - self.nocc)
+
+    def build_v_ph(self):
+        self.ctx.call("mgw_hip_build_vph_from_eri")
+
+    def quadratic_forms(self, p: int, k0: int, width: int) -> np.ndarray:
+        q = np.zeros(max(width, 0))
+        self.ctx.call("mgw_hip_quadratic_forms", p, k0, width, _ptr(q), None)
+        return q.astype(np.complex128)
+
+
+@dataclass
+class HipG0W0Result:
+    omegas: np.ndarray
+    weights: np.ndarray
+    states: list
+    sigma_x_diag: np.ndarray
+    sigma_c: np.ndarray        # (nstates, nfreq) complex
+    qp: np.ndarray             # (nmo,), NaN for unselected states
+    qp_iterations: np.ndarray
+    timings: dict
+
+
+def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.0,
+             precision: int = PREC_FP64, ctx: HipContext | None = None,
+             upload: bool = True) -> HipG0W0Result:
+    """G0W0 for one molecule on the device (ERI uploaded unless upload=False)."""
+    ctx = ctx or HipContext()
+    nmo = inp.nmo
+    if upload:
+        ctx.upload_eri(inp.eri, inp.nocc)
+    st = None if states is None else np.ascontiguousarray(states, dtype=np.int32)
+    ns = nmo if st is None else len(st)
+    mo = _f64(inp.mo_energy)
+    vxc = _f64(np.diag(inp.vxc))
+    job = G0W0Job(nmo, inp.nocc, nfreq, npar, 0 if st is None else ns, _ptr(st, _ip),
+                  _ptr(mo), _ptr(vxc), float(inp.fermi_energy), float(eta), precision)
+    om, w, sx = np.zeros(nfreq), np.zeros(nfreq), np.zeros(nmo)
+    sre, sim = np.zeros((ns, nfreq)), np.zeros((ns, nfreq))
+    qp, it = np.zeros(nmo), np.zeros(nmo, dtype=np.int32)
+    out = G0W0Out(_ptr(om), _ptr(w), _ptr(sx), _ptr(sre), _ptr(sim), _ptr(qp), _ptr(it, _ip))
+    ctx.call("mgw_hip_g0w0", C.byref(job), C.byref(out))
+    return HipG0W0Result(om, w, list(range(nmo)) if st is None else st.tolist(), sx,
+                         sre + 1j * sim, qp, it, ctx.last_timings())

@@ -1,0 +1,217 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the
+reference's own outputs.  Needs an MI355X."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from minigw_amd.synthetic import GwInput, synthetic_input, CONFIGS
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from minigw_amd import hip as h
+    return h
+
+
+@pytest.fixture(scope="module")
+def ctx(hip):
+    c = hip.HipContext(0)
+    yield c
+    c.close()
+
+
+def h2o_input():
+    d = np.load(os.path.join(GOLD, "h2o_input.npz"))
+    return GwInput(mo_energy=d["mo_energy"], eri=d["eri"], vxc=d["vxc"], nocc=int(d["nocc"]),
+                   fermi_energy=float(d["fermi_energy"]))
+
+
+def qp_envelope(inp, omegas, sigma_x, sigma_c, states, npar, rel, trials=4):
+    """Per-state QP spread under relative noise `rel` on Sigma_c (oracle Pade/QP)."""
+    rng = np.random.default_rng(7)
+    base, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, sigma_x, np.diag(inp.vxc), sigma_c,
+                           omegas, states, npar)
+    env = np.zeros(inp.nmo)
+    for _ in range(trials):
+        sc = sigma_c * (1 + rel * rng.standard_normal(sigma_c.shape))
+        qp, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, sigma_x, np.diag(inp.vxc), sc,
+                             omegas, states, npar)
+        env = np.maximum(env, np.nan_to_num(np.abs(qp - base)))
+    return env
+
+
+# ------------------------------------------------------------ MFMA layout
+def test_mfma_f64_tile_layout(ctx):
+    say("case: f64 MFMA tile C = A B^T, 64x64")
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((64, 64))
+    b = rng.standard_normal((64, 64))
+    c = ctx.selftest_tile(a, b)
+    assert np.allclose(c, a @ b.T, rtol=1e-13, atol=1e-12)
+
+
+# ------------------------------------------------------------ workspace boundary
+@pytest.mark.parametrize("nmo,nocc,seed", [(10, 3, 7), (24, 5, 20240601), (40, 9, 3)])
+def test_compute_wc_matches_oracle(hip, ctx, nmo, nocc, seed):
+    say(f"case: compute_w_c nmo={nmo} nocc={nocc}")
+    inp = synthetic_input(nmo, nocc, seed)
+    ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+    v = orc.v_ph_matrix(inp.eri, ph)
+    ws = hip.HipScreeningWorkspace(v, ctx)
+    assert ws.nph == ph.size
+    for w in [1e-12, 0.37, 25.0]:
+        pi0 = orc.pi0_diag(w, ph.delta, 0.0)
+        ws.compute_w_c(pi0)
+        want = orc.compute_w_c(v, pi0.real)
+        got = ws.w_c()
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err < 1e-12, err
+        rng = np.random.default_rng(1)
+        x = rng.standard_normal((ph.size, 37))
+        q = ws.quadratic_forms_panel(x)
+        qw = orc.quadratic_forms_panel(want, x)
+        assert np.allclose(q.real, qw, rtol=1e-11, atol=1e-13)
+        assert np.all(q.imag == 0)
+
+
+def test_eri_panels_match_oracle(hip, ctx):
+    say("case: ERI-resident panels nmo=24")
+    inp = synthetic_input(24, 5, 11)
+    ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+    ws = hip.HipScreeningWorkspace(orc.v_ph_matrix(inp.eri, ph), ctx)
+    view = hip.HipPqPhPanelView(inp.eri, inp.nocc, ctx)
+    view.build_v_ph()                     # V_ph rebuilt on the device from the ERI
+    pi0 = orc.pi0_diag_real(0.8, ph.delta)
+    ws.compute_w_c(pi0)
+    wc = orc.compute_w_c(orc.v_ph_matrix(inp.eri, ph), pi0)
+    for p, k0, width in [(0, 0, 24), (4, 0, 24), (23, 20, 4), (7, 3, 1)]:
+        q = view.quadratic_forms(p, k0, width)
+        want = orc.quadratic_forms_panel(wc, orc.pq_panel(inp.eri, ph, p, k0, width))
+        assert np.allclose(q.real, want, rtol=1e-11, atol=1e-13), (p, k0, width)
+    with pytest.raises(hip.HipError) as e:
+        view.quadratic_forms(24, 0, 1)
+    assert "p index out of range" in str(e.value)
+    with pytest.raises(hip.HipError):
+        view.quadratic_forms(0, 20, 5)
+
+
+def test_boundary_errors(hip, ctx):
+    say("case: boundary error paths")
+    with pytest.raises(hip.HipError) as e:
+        hip.HipScreeningWorkspace(np.zeros((3, 4)), ctx)
+    v = np.eye(5) * 0.3
+    ws = hip.HipScreeningWorkspace(v, ctx)
+    with pytest.raises(hip.HipError) as e:
+        ws.compute_w_c(np.full(4, -0.5))
+    assert e.value.code == hip.MGW_ERR_ARG
+    with pytest.raises(hip.HipError) as e:
+        ws.compute_w_c(np.full(5, -0.5) + 0.01j)
+    assert e.value.code == hip.MGW_ERR_UNSUPPORTED
+    ws.compute_w_c(np.full(5, -0.5))
+    assert np.allclose(ws.w_c(), np.eye(5) * (-0.5 / (1 + 0.5 * 0.3)), atol=1e-15)
+
+
+# ------------------------------------------------------------ full chain, FP64
+def test_h2o_state5_fixture(hip, ctx):
+    say("case: H2O state 5, 200 GL, 16 Pade, FP64")
+    inp = h2o_input()
+    fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
+    res = hip.run_g0w0(inp, 200, 16, states=[4], ctx=ctx)
+    sc = res.sigma_c[0]
+    ref = fx["e_c_before_pade"]
+    for x, y in zip(sc, ref):  # compare_reference.py component check at 1e-8
+        assert math.isclose(x.real, y.real, rel_tol=1e-8, abs_tol=1e-8)
+        assert math.isclose(x.imag, y.imag, rel_tol=1e-8, abs_tol=1e-8)
+    assert np.abs(sc - ref).max() < 1e-11
+    row = fx["summary_row_orbital5_ev"]
+    assert math.isclose(res.qp[4] * orc.K_HARTREE_TO_EV, row[4], rel_tol=1e-5, abs_tol=1e-5)
+    assert np.isnan(res.qp[0]) and res.qp_iterations[4] > 0
+
+
+@pytest.mark.parametrize("name", ["h2o_all", "syn_c1", "syn_small", "syn_odd"])
+def test_all_states_vs_reference_path(hip, ctx, name):
+    r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
+    say(f"case: all states vs reference path, {name}")
+    if name == "h2o_all":
+        inp = h2o_input()
+    elif name == "syn_c1":
+        inp = synthetic_input(*CONFIGS["c1"])
+    else:
+        inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
+                      fermi_energy=float(r["fermi_energy"]))
+    nf, npar = int(r["nfreq"]), int(r["npar"])
+    res = hip.run_g0w0(inp, nf, npar, ctx=ctx)
+    assert np.array_equal(res.omegas, r["omega"])
+    assert np.allclose(res.sigma_x_diag, r["sigma_x_diag"], rtol=1e-14, atol=1e-15)
+    dsig = np.abs(res.sigma_c - r["sigma_c"]).max()
+    say(f"   max |dSigma_c| = {dsig:.3e}")
+    assert dsig < 1e-11
+    env = qp_envelope(inp, r["omega"], r["sigma_x_diag"], r["sigma_c"], list(range(inp.nmo)),
+                      npar, max(dsig / np.abs(r["sigma_c"]).max(), 1e-15))
+    dq = np.abs(res.qp - r["qp"])
+    say(f"   QP: {np.sum(dq < 1e-8)}/{inp.nmo} states within 1e-8 Ha; max {dq.max():.2e}")
+    assert np.all(dq <= np.maximum(1e-8, 10 * env)), (dq, env)
+
+
+def test_pade_qp_kernel_exact_on_same_sigma(hip, ctx):
+    """Pade + QP on the device equals the oracle's bit for bit when Sigma_c agrees."""
+    say("case: device Pade/QP vs oracle on the device's own Sigma_c, nmo=24")
+    inp = synthetic_input(*CONFIGS["c1"])
+    res = hip.run_g0w0(inp, 64, 16, ctx=ctx)
+    qp, it = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
+                          res.sigma_c, res.omegas, list(range(inp.nmo)), 16)
+    assert np.array_equal(it, res.qp_iterations)
+    assert np.array_equal(qp, res.qp)
+
+
+@pytest.mark.parametrize("nmo,nocc,nfreq", [(60, 12, 12), (90, 18, 6)])
+def test_multiblock_vs_oracle(hip, ctx, nmo, nocc, nfreq):
+    say(f"case: multi-block factorization nmo={nmo} nocc={nocc} nfreq={nfreq}")
+    inp = synthetic_input(nmo, nocc, 99)
+    res = hip.run_g0w0(inp, nfreq, 6, ctx=ctx)
+    want = orc.run_g0w0(inp, nfreq, 6)
+    d = np.abs(res.sigma_c - want.sigma_c).max() / np.abs(want.sigma_c).max()
+    say(f"   rel |dSigma_c| = {d:.3e}")
+    assert d < 1e-11
+
+
+def test_subset_states_and_errors(hip, ctx):
+    say("case: state subset + job errors")
+    inp = synthetic_input(16, 4, 5)
+    full = hip.run_g0w0(inp, 24, 8, ctx=ctx)
+    sub = hip.run_g0w0(inp, 24, 8, states=[2, 9, 15], ctx=ctx, upload=False)
+    assert np.allclose(sub.sigma_c, full.sigma_c[[2, 9, 15]], rtol=1e-13, atol=1e-15)
+    assert np.isnan(sub.qp[0]) and not np.isnan(sub.qp[9])
+    with pytest.raises(hip.HipError) as e:
+        hip.run_g0w0(inp, 24, 8, eta=0.01, ctx=ctx, upload=False)
+    assert e.value.code == hip.MGW_ERR_UNSUPPORTED
+    with pytest.raises(hip.HipError) as e:
+        hip.run_g0w0(inp, 24, 8, states=[3, 3], ctx=ctx, upload=False)
+    assert e.value.code == hip.MGW_ERR_ARG
+    with pytest.raises(hip.HipError):
+        hip.run_g0w0(inp, 24, 65, ctx=ctx, upload=False)
+
+
+# ------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("nmo,nocc", [(24, 5), (60, 12)])
+def test_mixed_vs_fp64(hip, ctx, nmo, nocc):
+    say(f"case: mixed vs FP64 nmo={nmo}")
+    inp = synthetic_input(nmo, nocc, 31)
+    f64 = hip.run_g0w0(inp, 48, 16, ctx=ctx)
+    mix = hip.run_g0w0(inp, 48, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED)
+    d = np.abs(mix.sigma_c - f64.sigma_c).max()
+    dq = np.abs(mix.qp - f64.qp) * orc.K_HARTREE_TO_EV * 1000
+    say(f"   max |dSigma_c| = {d:.2e} Ha; QP max {dq.max():.3e} meV, "
+        f"{np.sum(dq <= 1.0)}/{nmo} within 1 meV")
+    assert d < 1e-5
+    assert np.median(dq) < 1.0
