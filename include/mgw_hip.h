@@ -160,6 +160,14 @@ int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
  * the operand / result lane layout on a new device). */
 int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, double* c);
 
+/* Diagnostic with no counterpart in the reference: ntests single-MFMA runs
+ * with per-lane inputs, raw per-lane results (lane layout check).
+ * kind 0: v_mfma_f64_16x16x4, a/b double[ntests][64], d double[ntests][64][4];
+ * kind 1/2: v_mfma_f32_16x16x32 bf16/f16, a/b uint16[ntests][64][8],
+ * d float[ntests][64][4]. */
+int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const void* a,
+                       const void* b, void* d);
+
 /* Transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:62-93). */
 int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight);
 

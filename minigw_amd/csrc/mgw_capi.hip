@@ -731,6 +731,31 @@ int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, do
     });
 }
 
+int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const void* a,
+                       const void* b, void* d) {
+    return guarded(ctx, [&] {
+        need(ctx && a && b && d && ntests > 0 && ntests <= 65536, "mgw_hip_probe_mfma: bad argument");
+        need(kind >= 0 && kind <= 2, "mgw_hip_probe_mfma: kind must be 0 (f64), 1 (bf16), 2 (f16)");
+        const size_t in_bytes = kind == 0 ? 64 * 8 : 64 * 16;
+        const size_t out_bytes = kind == 0 ? 64 * 4 * 8 : 64 * 4 * 4;
+        char* buf = as<char>(ctx->get(ctx->wsrc, (size_t)ntests * (2 * in_bytes + out_bytes), "probe"));
+        char* da = buf;
+        char* db = buf + (size_t)ntests * in_bytes;
+        char* dd = db + (size_t)ntests * in_bytes;
+        ctx->h2d(da, static_cast<const char*>(a), (size_t)ntests * in_bytes);
+        ctx->h2d(db, static_cast<const char*>(b), (size_t)ntests * in_bytes);
+        if (kind == 0)
+            hip_check(mgw::launch_probe_f64(as<double>(da), as<double>(db), as<double>(dd), ntests,
+                                            ctx->stream), "probe f64");
+        else
+            hip_check(mgw::launch_probe16(kind == 1 ? mgw::Split16::BF16 : mgw::Split16::FP16,
+                                          as<uint16_t>(da), as<uint16_t>(db), as<float>(dd), ntests,
+                                          ctx->stream), "probe 16");
+        ctx->d2h(static_cast<char*>(d), dd, (size_t)ntests * out_bytes);
+        ctx->sync();
+    });
+}
+
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n) {
     if (!ctx || !ms || n < 0) {
         g_thread_error = "mgw_hip_last_timings: bad argument";

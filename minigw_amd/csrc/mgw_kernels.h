@@ -18,6 +18,10 @@ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // Diagnostic: C = A B^T (64 x 64 row-major) through the factorization's MFMA tile.
 hipError_t launch_tile_selftest(const double* a, const double* b, double* c, hipStream_t s);
 
+// Diagnostics: raw MFMA results for per-lane inputs (lane layout checks).
+hipError_t launch_probe_f64(const double* a, const double* b, double* d, int ntests,
+                           hipStream_t s);
+
 // V[r][c] = eri[rowbase[r] + colidx[c]] for r, c < n; 0 in the padding (np x np).
 hipError_t launch_build_v(const double* eri, const int64_t* rowbase, const int* colidx,
                           double* v, int n, int np, hipStream_t s);
@@ -78,6 +82,9 @@ hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_
 hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_t* mlo,
                                  const double* mscale, const uint16_t* xhi, const uint16_t* xlo,
                                  double* qpart, int np, int mpad, int nb, hipStream_t s);
+
+hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
+                         int ntests, hipStream_t s);
 
 // q[(f0+w) * m + c] = -(sum_rt qpart[w][rt][c]) / xscale[c]^2.
 hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, double* q, int np,

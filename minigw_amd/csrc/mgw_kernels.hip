@@ -15,7 +15,7 @@
 // MFMA: v_mfma_f64_16x16x4_f64.  Operand/result lane layout (wave64):
 //   a: A[i][k], i = lane % 16, k = lane / 16      (16 x 4 block)
 //   b: B[k][j], j = lane % 16, k = lane / 16      (4 x 16 block)
-//   d[r]: D[i][j], i = 4 * (lane / 16) + r, j = lane % 16
+//   d[r]: D[i][j], i = lane / 16 + 4 * r, j = lane % 16   (checked on the device)
 #include "mgw_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -118,7 +118,7 @@ struct Tile64 {
         for (int bi = 0; bi < 2; ++bi)
             for (int bj = 0; bj < 2; ++bj)
                 for (int r = 0; r < 4; ++r)
-                    f(wm + bi * 16 + 4 * (lane >> 4) + r, wn + bj * 16 + (lane & 15), acc[bi][bj][r]);
+                    f(wm + bi * 16 + (lane >> 4) + 4 * r, wn + bj * 16 + (lane & 15), acc[bi][bj][r]);
     }
 };
 
@@ -538,6 +538,17 @@ __global__ __launch_bounds__(256) void k_tile_selftest(const double* __restrict_
     tile.for_each([&](int r, int col, double v) { c[r * 64 + col] = v; });
 }
 
+// Diagnostic: one v_mfma_f64_16x16x4 per wave with per-lane inputs a[t][lane],
+// b[t][lane]; raw results d[t][lane][r] (lane layout check).
+__global__ __launch_bounds__(64) void k_probe_f64(const double* __restrict__ a,
+                                                  const double* __restrict__ b,
+                                                  double* __restrict__ d) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+    acc = mfma_f64(a[t * 64 + lane], b[t * 64 + lane], acc);
+    for (int r = 0; r < 4; ++r) d[((size_t)t * 64 + lane) * 4 + r] = acc[r];
+}
+
 inline int grid_for(size_t total, int block = 256, int cap = 65535 * 8) {
     size_t g = (total + block - 1) / block;
     if (g < 1) g = 1;
@@ -546,6 +557,12 @@ inline int grid_for(size_t total, int block = 256, int cap = 65535 * 8) {
 }
 
 }  // namespace
+
+hipError_t launch_probe_f64(const double* a, const double* b, double* d, int ntests,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_f64, dim3(ntests), dim3(64), 0, s, a, b, d);
+    return hipGetLastError();
+}
 
 hipError_t launch_tile_selftest(const double* a, const double* b, double* c, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_selftest, dim3(1), dim3(256), 0, s, a, b, c);

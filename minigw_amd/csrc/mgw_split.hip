@@ -205,6 +205,17 @@ __global__ __launch_bounds__(256) void k_contract_split(
     if (t < kCT) qpart[((size_t)w * (np / kCT) + rt) * mpad + c0 + t] = red[0][t] + red[1][t];
 }
 
+// Diagnostic: one 16x16x32 MFMA per wave; a/b: [t][lane][8] 16-bit patterns.
+template <Split16 F>
+__global__ __launch_bounds__(64) void k_probe16(const uint4* __restrict__ a,
+                                                const uint4* __restrict__ b,
+                                                float* __restrict__ d) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+    acc = mfma16<F>(a[t * 64 + lane], b[t * 64 + lane], acc);
+    for (int r = 0; r < 4; ++r) d[((size_t)t * 64 + lane) * 4 + r] = acc[r];
+}
+
 __global__ void k_reduce_q_scaled(const double* __restrict__ qpart,
                                   const double* __restrict__ xscale, double* __restrict__ q,
                                   int nrt, int m, int mpad, int f0) {
@@ -254,6 +265,17 @@ hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_
     else
         hipLaunchKernelGGL(k_contract_split<Split16::FP16>, grid, dim3(256), 0, s, mhi, mlo, mscale,
                            xhi, xlo, qpart, np, mpad);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
+                         int ntests, hipStream_t s) {
+    const uint4* a4 = reinterpret_cast<const uint4*>(a);
+    const uint4* b4 = reinterpret_cast<const uint4*>(b);
+    if (fmt == Split16::BF16)
+        hipLaunchKernelGGL(k_probe16<Split16::BF16>, dim3(ntests), dim3(64), 0, s, a4, b4, d);
+    else
+        hipLaunchKernelGGL(k_probe16<Split16::FP16>, dim3(ntests), dim3(64), 0, s, a4, b4, d);
     return hipGetLastError();
 }
 

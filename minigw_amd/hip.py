@@ -77,6 +77,7 @@ def _load():
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
         "mgw_hip_selftest_tile": (C.c_int, [vp, _dp, _dp, _dp]),
+        "mgw_hip_probe_mfma": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, vp]),
         "mgw_grid_transformed_legendre": (C.c_int, [C.c_int32, _dp, _dp]),
         "mgw_hip_estimate_bytes": (C.c_size_t, [C.c_int32] * 6),
     }
@@ -172,6 +173,18 @@ class HipContext:
         keys = ["build_v", "dielectric", "factorize", "contract", "sigma_c", "pade_qp", "total",
                 "freq_batch", "contract_launches", "split_reduce"]
         return dict(zip(keys, t.tolist()))
+
+    def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """Raw single-MFMA results: kind 0 f64 (a, b: [t, 64]); 1 bf16 / 2 f16 (a, b: [t, 64, 8] uint16)."""
+        if kind == 0:
+            a, b = _f64(a), _f64(b)
+            d = np.zeros((a.shape[0], 64, 4))
+        else:
+            a = np.ascontiguousarray(a, dtype=np.uint16)
+            b = np.ascontiguousarray(b, dtype=np.uint16)
+            d = np.zeros((a.shape[0], 64, 4), dtype=np.float32)
+        self.call("mgw_hip_probe_mfma", kind, a.shape[0], a.ctypes.data, b.ctypes.data, d.ctypes.data)
+        return d
 
     def selftest_tile(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         a, b = _f64(a), _f64(b)

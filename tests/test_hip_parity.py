@@ -57,6 +57,9 @@ def test_mfma_f64_tile_layout(ctx):
     a = rng.standard_normal((64, 64))
     b = rng.standard_normal((64, 64))
     c = ctx.selftest_tile(a, b)
+    out = os.path.join(os.path.dirname(GOLD), "..", "gpurun_out")
+    if os.path.isdir(out):
+        np.savez(os.path.join(out, "tile_selftest.npz"), a=a, b=b, c=c)
     assert np.allclose(c, a @ b.T, rtol=1e-13, atol=1e-12)
 
 
