@@ -206,15 +206,25 @@ def test_subset_states_and_errors(hip, ctx):
 
 
 # ------------------------------------------------------------ mixed precision
-@pytest.mark.parametrize("nmo,nocc", [(24, 5), (60, 12)])
-def test_mixed_vs_fp64(hip, ctx, nmo, nocc):
-    say(f"case: mixed vs FP64 nmo={nmo}")
-    inp = synthetic_input(nmo, nocc, 31)
-    f64 = hip.run_g0w0(inp, 48, 16, ctx=ctx)
-    mix = hip.run_g0w0(inp, 48, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED)
-    d = np.abs(mix.sigma_c - f64.sigma_c).max()
-    dq = np.abs(mix.qp - f64.qp) * orc.K_HARTREE_TO_EV * 1000
-    say(f"   max |dSigma_c| = {d:.2e} Ha; QP max {dq.max():.3e} meV, "
-        f"{np.sum(dq <= 1.0)}/{nmo} within 1 meV")
-    assert d < 1e-5
-    assert np.median(dq) < 1.0
+@pytest.mark.parametrize("case", ["h2o", "syn24", "syn60"])
+def test_mixed_vs_fp64(hip, ctx, case):
+    """Mixed form against the FP64 form: Sigma_c(i w) to 1e-6 relative; QP
+    energies within 1 meV, or inside the spread that noise of the measured
+    Sigma_c error level produces through the Pade/QP step (the QP energies of
+    the synthetic molecules are ill-conditioned)."""
+    say(f"case: mixed vs FP64 {case}")
+    inp = h2o_input() if case == "h2o" else synthetic_input(24 if case == "syn24" else 60,
+                                                             5 if case == "syn24" else 12, 31)
+    nf = 200 if case == "h2o" else 48
+    f64 = hip.run_g0w0(inp, nf, 16, ctx=ctx)
+    mix = hip.run_g0w0(inp, nf, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED)
+    scale = np.abs(f64.sigma_c).max()
+    d = np.abs(mix.sigma_c - f64.sigma_c).max() / scale
+    dq = np.abs(mix.qp - f64.qp)
+    mev = dq * orc.K_HARTREE_TO_EV * 1000
+    env = qp_envelope(inp, f64.omegas, f64.sigma_x_diag, f64.sigma_c, list(range(inp.nmo)), 16,
+                      max(d, 1e-15))
+    say(f"   rel |dSigma_c| = {d:.2e}; QP within 1 meV: {np.sum(mev <= 1.0)}/{inp.nmo}; "
+        f"median {np.median(mev):.3g} meV; noise envelope median {np.median(env) * 27211.4:.3g} meV")
+    assert d < 1e-6
+    assert np.all(dq <= np.maximum(1.0 / 27211.386, 10 * env)), (mev, env)
