@@ -32,7 +32,8 @@ def log(*a):
 
 def contraction_flops(n: int, m: int, precision: int) -> float:
     """Algorithmic flops of one frequency's contraction Z = (L^-1 S) X, L^-1
-    lower triangular: 2 * (n(n+1)/2) * m; three products in the split form."""
+    lower triangular: 2 * (n(n+1)/2) * m for the m contracted columns (the
+    unordered (p,k) pairs when q(p,k) = q(k,p)); three products in the split form."""
     f = float(n) * (n + 1) * m
     return 3.0 * f if precision == 1 else f
 
@@ -124,7 +125,8 @@ def main() -> int:
     launches = max(ph["contract_launches"], 1.0)
     nb = ph["freq_batch"]
     t_launch = ph["contract"] / launches * 1e-3
-    flops_launch = contraction_flops(n, m, prec) * min(nb, args.nfreq)
+    cols = ph.get("contract_columns", m) or m      # unordered (p,k) pairs when the ERI is symmetric
+    flops_launch = contraction_flops(n, cols, prec) * min(nb, args.nfreq)
     peak = PEAK["fp16"] if prec == hip.PREC_MIXED else PEAK["fp64"]
     achieved = flops_launch / t_launch / 1e12
     out = {

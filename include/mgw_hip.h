@@ -151,7 +151,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
  * context's stream): [0] V build (+ X split), [1] dielectric build,
  * [2] factorization (potrf + trtri), [3] contraction kernel, [4] Sigma_c
  * quadrature, [5] Pade + QP, [6] total, [7] frequencies per batch,
- * [8] contraction launches, [9] M split + q reduction.  n is the number of
+ * [8] contraction launches, [9] M split + q reduction, [10] contraction
+ * columns (unordered (p,k) pairs when the ERI is exactly pk-symmetric).  n is the number of
  * entries the caller provides (<= 12). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 

@@ -89,6 +89,8 @@ struct mgw_hip_ctx {
     // resident inputs
     Buf eri, v, colidx, rowbase, delta;
     int nmo = 0, nocc = 0;        // of the resident ERI (0 = none)
+    bool eri_pk_sym = false;      // (pk|ia) == (kp|ia) exactly for the resident ERI
+    Buf flag, hcolmap;
     int n = 0, np = 0;            // ph dimension of the resident V
     bool v_ready = false;
     // single-frequency workspace (compute_wc / quadratic forms)
@@ -100,7 +102,7 @@ struct mgw_hip_ctx {
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
 
     mgw_hip_ctx() {
-        all = {&eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
+        all = {&flag, &hcolmap, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale};
@@ -218,7 +220,7 @@ PhLayout ph_layout(int nmo, int nocc) {
     L.nocc = nocc;
     L.nvir = nmo - nocc;
     L.n = nocc * L.nvir;
-    L.np = mgw::round_up(std::max(L.n, 1), mgw::kCT);
+    L.np = mgw::round_up(std::max(L.n, 1), mgw::kPad);
     L.colidx.assign(L.np, 0);
     L.rowbase.assign(L.np, 0);
     const int64_t n2 = (int64_t)nmo * nmo;
@@ -345,7 +347,7 @@ int mgw_hip_upload_vph(mgw_hip_ctx* ctx, const double* v, int32_t n) {
     return guarded(ctx, [&] {
         need(ctx && v, "mgw_hip_upload_vph: NULL argument");
         need(n > 0, "DeviceScreeningWorkspace: V_ph must be non-empty and square");
-        const int np = mgw::round_up(n, mgw::kCT);
+        const int np = mgw::round_up(n, mgw::kPad);
         double* tmp = as<double>(ctx->get(ctx->wsrc, (size_t)n * n * 8, "V_ph staging"));
         double* dv = as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
         ctx->h2d(tmp, v, (size_t)n * n);
@@ -427,7 +429,7 @@ namespace {
 void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow, int m,
                      const int* hcolidx, double* q_re, double* q_im) {
     const int np = ctx->np;
-    const int mpad = mgw::round_up(std::max(m, 1), mgw::kCT);
+    const int mpad = mgw::round_up(std::max(m, 1), mgw::kPad);
     std::vector<int64_t> pk(mpad, 0);
     std::copy(hpkrow, hpkrow + m, pk.begin());
     int64_t* dpk = as<int64_t>(ctx->get(ctx->wpkrow, (size_t)mpad * 8, "pkrow"));
@@ -440,7 +442,8 @@ void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow,
     hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), src, dpk,
                                        dcol, qpart, np, mpad, 1, ctx->stream),
               "contract");
-    hip_check(mgw::launch_reduce_q(qpart, q, np, m, mpad, 0, 1, ctx->stream), "reduce q");
+    hip_check(mgw::launch_reduce_q(qpart, nullptr, q, np / mgw::kCT, m, mpad, 0, 1, ctx->stream),
+              "reduce q");
     ctx->d2h(q_re, q, m);
     ctx->sync();
     if (q_im) std::fill(q_im, q_im + m, 0.0);
@@ -485,7 +488,13 @@ int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t
         ctx->h2d(d, eri, count);
         ctx->h2d(col, L.colidx.data(), L.np);
         ctx->h2d(rb, L.rowbase.data(), L.np);
+        int* dflag = as<int>(ctx->get(ctx->flag, sizeof(int), "flag"));
+        hip_check(hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
+        hip_check(mgw::launch_check_pk_symmetry(d, nmo, nocc, dflag, ctx->stream), "symmetry check");
+        int asym = 1;
+        ctx->d2h(&asym, dflag, 1);
         ctx->sync();
+        ctx->eri_pk_sym = (asym == 0);
         ctx->nmo = nmo;
         ctx->nocc = nocc;
     });
@@ -544,7 +553,28 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const int ns = (int)states.size();
         const PhLayout L = ph_layout(nmo, nocc);
         const int n = L.n, np = L.np, nblk = np / mgw::kNB;
-        const int m = ns * nmo, mpad = mgw::round_up(m, mgw::kCT);
+        const int m = ns * nmo;
+        // Contraction columns.  With (pk|ia) == (kp|ia) exactly, q(p,k) = q(k,p):
+        // each unordered pair {p,k} with p or k among the states is contracted once
+        // and colmap sends every (state, k) to its column.
+        std::vector<int> colmap(m);
+        std::vector<std::pair<int, int>> cols;
+        {
+            std::vector<int> slot(nmo, -1);
+            for (int s = 0; s < ns; ++s) slot[states[s]] = s;
+            cols.reserve(m);
+            for (int s = 0; s < ns; ++s)
+                for (int k = 0; k < nmo; ++k) {
+                    const int p = states[s];
+                    if (ctx->eri_pk_sym && k < p && slot[k] >= 0) {
+                        colmap[(size_t)s * nmo + k] = colmap[(size_t)slot[k] * nmo + p];
+                    } else {
+                        colmap[(size_t)s * nmo + k] = (int)cols.size();
+                        cols.emplace_back(p, k);
+                    }
+                }
+        }
+        const int mu = (int)cols.size(), mpad = mgw::round_up(mu, mgw::kPad);
         std::vector<double> delta(np, 0.0);
         for (int i = 0; i < nocc; ++i)
             for (int a = 0; a < L.nvir; ++a) {
@@ -558,8 +588,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         transformed_legendre(nfreq, omega.data(), weight.data());
         std::vector<int64_t> pkrow(mpad, 0);
         const int64_t n2 = (int64_t)nmo * nmo;
-        for (int s = 0; s < ns; ++s)
-            for (int k = 0; k < nmo; ++k) pkrow[(size_t)s * nmo + k] = ((int64_t)states[s] * nmo + k) * n2;
+        for (int c = 0; c < mu; ++c) pkrow[c] = ((int64_t)cols[c].first * nmo + cols[c].second) * n2;
 
         hipStream_t st = ctx->stream;
         hipEvent_t ev[8];
@@ -589,6 +618,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->h2d(dvxc, job->vxc_diag, nmo);
         ctx->h2d(dstates, states.data(), ns);
         ctx->h2d(dpk, pkrow.data(), mpad);
+        int* dcolmap = as<int>(ctx->get(ctx->hcolmap, (size_t)m * 4, "colmap"));
+        ctx->h2d(dcolmap, colmap.data(), m);
         hip_check(mgw::launch_build_v(as<double>(ctx->eri.p), as<int64_t>(ctx->rowbase.p),
                                       as<int>(ctx->colidx.p), dV, n, np, st),
                   "build V_ph");
@@ -656,10 +687,13 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                           "contract");
             hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
             if (mixed)
-                hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dq, np, m, mpad, f0, b, st),
+                hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dcolmap, dq,
+                                                      mgw::contract_split_row_tiles(np), m, mpad,
+                                                      f0, b, st),
                           "reduce q");
             else
-                hip_check(mgw::launch_reduce_q(qpart, dq, np, m, mpad, f0, b, st), "reduce q");
+                hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0, b, st),
+                          "reduce q");
             hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
             hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
             float a = 0, bb = 0, c = 0, d = 0, e = 0;
@@ -716,6 +750,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->timings[7] = nb;
         ctx->timings[8] = n_contr;
         ctx->timings[9] = t_aux;
+        ctx->timings[10] = mu;
     });
 }
 
@@ -770,7 +805,7 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
     if (nmo <= 1 || nocc <= 0 || nocc >= nmo || nfreq <= 0) return 0;
     const PhLayout L = ph_layout(nmo, nocc);
     const int ns = nstates > 0 ? nstates : nmo;
-    const int m = ns * nmo, mpad = mgw::round_up(m, mgw::kCT);
+    const int m = ns * nmo, mpad = mgw::round_up(m, mgw::kPad);
     const size_t eri = (size_t)nmo * nmo * nmo * nmo * 8;
     const size_t v = (size_t)L.np * L.np * 8;
     size_t fixed = eri + v + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;

@@ -10,7 +10,8 @@
 namespace mgw {
 
 constexpr int kNB = 64;    // factorization block (potrf / trtri tiles)
-constexpr int kCT = 128;   // contraction tile: rows of M and columns of X per workgroup
+constexpr int kCT = 128;   // FP64 contraction tile: rows of M and columns of X per workgroup
+constexpr int kPad = 256;  // np and the contraction column count are padded to this
 constexpr int kMaxPade = 64;
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -56,9 +57,15 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* s
                                const int64_t* pkrow, const int* colidx, double* qpart, int np,
                                int mpad, int nb, hipStream_t s);
 
-// q[(f0+w) * m + c] = -sum_rt qpart[w][rt][c].
-hipError_t launch_reduce_q(const double* qpart, double* q, int np, int m, int mpad, int f0,
-                           int nb, hipStream_t s);
+// q[(f0+w) * m + c] = -sum_rt qpart[w][rt][colmap[c]] (colmap NULL = identity).
+// nrt = row tiles of the contraction that wrote qpart (np / 128 for FP64,
+// contract_split_row_tiles(np) for the split form).
+hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
+                           int mpad, int f0, int nb, hipStream_t s);
+
+// flag = 1 if (pk|ia) != (kp|ia) for some p, k, i < nocc <= a (flag must start at 0).
+hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int* flag,
+                                    hipStream_t s);
 
 // Mixed precision: 16-bit hi/lo splits of the contraction operands.
 //   BF16: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); no scaling.
@@ -76,7 +83,8 @@ hipError_t launch_split_x(Split16 fmt, const double* src, const int64_t* pkrow, 
 hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_t* mhi,
                           uint16_t* mlo, double* mscale, int np, int nb, hipStream_t s);
 
-// Same output as launch_contract_f64 (scaled by xscale[c]^2 when FP16):
+// Same output as launch_contract_f64 but with 256-row tiles (scaled by
+// xscale[c]^2 when FP16); needs np and mpad multiples of 256:
 // products hi*hi + hi*lo + lo*hi on 16-bit MFMA (v_mfma_f32_16x16x32) with
 // FP32 accumulation, row scale undone and squares summed in FP64.
 hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_t* mlo,
@@ -86,9 +94,11 @@ hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_
 hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
                          int ntests, hipStream_t s);
 
-// q[(f0+w) * m + c] = -(sum_rt qpart[w][rt][c]) / xscale[c]^2.
-hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, double* q, int np,
-                                  int m, int mpad, int f0, int nb, hipStream_t s);
+// q[(f0+w) * m + c] = -(sum_rt qpart[w][rt][cu]) / xscale[cu]^2, cu = colmap[c].
+hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, const int* colmap,
+                                  double* q, int nrt, int m, int mpad, int f0, int nb,
+                                  hipStream_t s);
+int contract_split_row_tiles(int np);
 
 // W_c = -(M S)^T (M S) for one matrix (explicit, n x n row-major output).
 hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int n, int np,

@@ -313,32 +313,47 @@ __global__ __launch_bounds__(256) void k_contract_f64(
     __shared__ double Bs[kBK][kCT + kLdsPad];
     __shared__ double red[2][kCT];
     const int w = blockIdx.z;
-    const int rt = blockIdx.y, r0 = rt * kCT, c0 = blockIdx.x * kCT;
+    // large-K row tiles first; consecutive tiles of one row panel on one XCD
+    const int ncol = gridDim.x, nrow = gridDim.y, T = ncol * nrow;
+    int lin = blockIdx.x + ncol * blockIdx.y;
+    if (T % 8 == 0) lin = (lin % 8) * (T / 8) + lin / 8;
+    const int rt = nrow - 1 - lin / ncol, r0 = rt * kCT, c0 = (lin % ncol) * kCT;
     const double* Mw = M + (size_t)w * np * np;
     const double* Sw = S + (size_t)w * np;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int li = lane & 15, lk = lane >> 4;
 
-    // loader coordinates: 128 rows/cols x 16 k, 8 consecutive k per thread
+    // loader coordinates: 128 rows/cols x 16 k, 8 consecutive k per thread;
+    // the next slab is fetched into registers while the current one is used
     const int lrow = t >> 1, lkq = (t & 1) * 8;
     const double* arow = Mw + (size_t)(r0 + lrow) * np;
-    const int64_t brow = pkrow[c0 + lrow];
+    const double* bsrc = src + pkrow[c0 + lrow];
+    double ra[8], rb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = k0 + lkq + u;
+            ra[u] = (kk <= r0 + lrow) ? arow[kk] * Sw[kk] : 0.0;
+            rb[u] = bsrc[colidx[kk]];
+        }
+    };
 
     d4 acc[4][4];
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
     const int kend = min(np, r0 + kCT);
+    fetch(0);
     for (int k0 = 0; k0 < kend; k0 += kBK) {
+        __syncthreads();
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int kk = k0 + lkq + u;
-            const double mv = (kk <= r0 + lrow) ? arow[kk] * Sw[kk] : 0.0;
-            As[lkq + u][lrow] = mv;
-            Bs[lkq + u][lrow] = src[brow + colidx[kk]];
+            As[lkq + u][lrow] = ra[u];
+            Bs[lkq + u][lrow] = rb[u];
         }
         __syncthreads();
+        if (k0 + kBK < kend) fetch(k0 + kBK);
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 4) {
             double a[4], b[4];
@@ -352,8 +367,8 @@ __global__ __launch_bounds__(256) void k_contract_f64(
 #pragma unroll
                 for (int y = 0; y < 4; ++y) acc[x][y] = mfma_f64(a[x], b[y], acc[x][y]);
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     // epilogue: per-column sum of squares over this wave's 64 rows
     for (int y = 0; y < 4; ++y) {
@@ -371,13 +386,32 @@ __global__ __launch_bounds__(256) void k_contract_f64(
     }
 }
 
-__global__ void k_reduce_q(const double* __restrict__ qpart, double* __restrict__ q, int nrt,
-                           int m, int mpad, int f0) {
+__global__ void k_reduce_q(const double* __restrict__ qpart, const int* __restrict__ colmap,
+                           double* __restrict__ q, int nrt, int m, int mpad, int f0) {
     const int w = blockIdx.y;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x) {
+        const int cu = colmap ? colmap[c] : c;
         double s = 0.0;
-        for (int rt = 0; rt < nrt; ++rt) s += qpart[((size_t)w * nrt + rt) * mpad + c];
+        for (int rt = 0; rt < nrt; ++rt) s += qpart[((size_t)w * nrt + rt) * mpad + cu];
         q[(size_t)(f0 + w) * m + c] = -s;
+    }
+}
+
+// flag = 1 if (pk|ia) != (kp|ia) for any p, k and occupied-virtual (i, a).
+__global__ void k_check_pk_symmetry(const double* __restrict__ eri, int nmo, int nocc,
+                                    int* __restrict__ flag) {
+    const int nvir = nmo - nocc;
+    const size_t total = (size_t)nmo * nmo * nocc * nvir;
+    const size_t n2 = (size_t)nmo * nmo;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(e % nvir);
+        const int i = (int)((e / nvir) % nocc);
+        const size_t pk = e / ((size_t)nocc * nvir);
+        const int p = (int)(pk / nmo), k = (int)(pk % nmo);
+        if (p >= k) continue;
+        const size_t col = (size_t)i * nmo + nocc + a;
+        if (eri[((size_t)p * nmo + k) * n2 + col] != eri[((size_t)k * nmo + p) * n2 + col]) *flag = 1;
     }
 }
 
@@ -633,10 +667,18 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* s
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_q(const double* qpart, double* q, int np, int m, int mpad, int f0,
-                           int nb, hipStream_t s) {
+hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
+                           int mpad, int f0, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_q, dim3(grid_for((size_t)m, 256, 4096), nb), dim3(256), 0, s,
-                       qpart, q, np / kCT, m, mpad, f0);
+                       qpart, colmap, q, nrt, m, mpad, f0);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int* flag,
+                                    hipStream_t s) {
+    const size_t total = (size_t)nmo * nmo * nocc * (nmo - nocc);
+    hipLaunchKernelGGL(k_check_pk_symmetry, dim3(grid_for(total, 256, 65536)), dim3(256), 0, s, eri,
+                       nmo, nocc, flag);
     return hipGetLastError();
 }
 

@@ -128,81 +128,114 @@ __device__ inline f4 mfma16(uint4 a, uint4 b, f4 c) {
     }
 }
 
-// Tile: rows [r0, r0+128) x columns [c0, c0+128); 4 waves in 2 x 2 of 64 x 64.
-// Operand fragments are loaded straight from HBM/L2 into registers (16 B per lane).
+// Workgroup tile: rows [r0, r0+256) x columns [c0, c0+256), 512 threads = 8
+// waves in 4 (rows) x 2 (columns), each wave 64 x 128 (4 x 8 MFMA blocks).
+// Per K step of 32 the four operand planes (M hi/lo rows, X hi/lo columns,
+// 16 B per thread-chunk, 8 chunks per thread) are loaded into registers one
+// step ahead and stored to LDS (row stride 40 halves: 80 B keeps the 16 B
+// fragment reads of a 16-lane group on distinct banks).  M is lower
+// triangular, so the K loop stops at r0 + 256.  Tiles are visited large-K
+// first, and consecutive tiles of one row panel go to the same XCD.
+constexpr int kSBM = 256, kSBN = 256, kSBK = 32, kSLd = kSBK + 8;
+
 template <Split16 F>
-__global__ __launch_bounds__(256) void k_contract_split(
+__global__ __launch_bounds__(512) void k_contract_split(
     const uint16_t* __restrict__ mhi, const uint16_t* __restrict__ mlo,
     const double* __restrict__ mscale, const uint16_t* __restrict__ xhi,
     const uint16_t* __restrict__ xlo, double* __restrict__ qpart, int np, int mpad) {
-    __shared__ double red[2][kCT];
-    const int w = blockIdx.z, rt = blockIdx.y, r0 = rt * kCT, c0 = blockIdx.x * kCT;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[4][kSBM][kSLd];
+    const int w = blockIdx.z;
+    const int ncol = gridDim.x, nrow = gridDim.y, T = ncol * nrow;
+    int lin = blockIdx.x + ncol * blockIdx.y;
+    if (T % 8 == 0) lin = (lin % 8) * (T / 8) + lin / 8;     // XCD-contiguous tiles
+    const int rt = nrow - 1 - lin / ncol, ct = lin % ncol;
+    const int r0 = rt * kSBM, c0 = ct * kSBN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
     const int li = lane & 15, lk = lane >> 4;
-    const size_t mo = (size_t)w * np * np;
 
-    const uint16_t* arow_h[4];
-    const uint16_t* arow_l[4];
-    const uint16_t* bcol_h[4];
-    const uint16_t* bcol_l[4];
+    // loader: chunk u of thread t -> plane u / 2, row (t + 512 (u % 2)) / 4, k (idx % 4) * 8
+    const uint16_t* src[8];
+    int lrow[8], lkq[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const size_t ro = mo + (size_t)(r0 + wm + q * 16 + li) * np + 8 * lk;
-        const size_t co = (size_t)(c0 + wn + q * 16 + li) * np + 8 * lk;
-        arow_h[q] = mhi + ro;
-        arow_l[q] = mlo + ro;
-        bcol_h[q] = xhi + co;
-        bcol_l[q] = xlo + co;
+    for (int u = 0; u < 8; ++u) {
+        const int plane = u >> 1, idx = t + 512 * (u & 1);
+        lrow[u] = idx >> 2;
+        lkq[u] = (idx & 3) * 8;
+        const uint16_t* base = plane == 0 ? mhi : plane == 1 ? mlo : plane == 2 ? xhi : xlo;
+        const size_t row = plane < 2 ? (size_t)w * np * np + (size_t)(r0 + lrow[u]) * np
+                                     : (size_t)(c0 + lrow[u]) * np;
+        src[u] = base + row + lkq[u];
     }
-    f4 acc[4][4];
+    uint4 pre[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pre[u] = *reinterpret_cast<const uint4*>(src[u]);
+
+    f4 acc[4][8];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int y = 0; y < 8; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
 
-    const int kend = min(np, r0 + kCT);
-    for (int k0 = 0; k0 < kend; k0 += 32) {
-        uint4 ah[4], al[4], bh[4], bl[4];
+    const int kend = min(np, r0 + kSBM);
+    for (int k0 = 0; k0 < kend; k0 += kSBK) {
+        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            ah[q] = *reinterpret_cast<const uint4*>(arow_h[q] + k0);
-            al[q] = *reinterpret_cast<const uint4*>(arow_l[q] + k0);
-            bh[q] = *reinterpret_cast<const uint4*>(bcol_h[q] + k0);
-            bl[q] = *reinterpret_cast<const uint4*>(bcol_l[q] + k0);
+        for (int u = 0; u < 8; ++u)
+            *reinterpret_cast<uint4*>(&lds[u >> 1][lrow[u]][lkq[u]]) = pre[u];
+        __syncthreads();
+        if (k0 + kSBK < kend) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                pre[u] = *reinterpret_cast<const uint4*>(src[u] + k0 + kSBK);
+        }
+        uint4 ah[4], al[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            ah[x] = *reinterpret_cast<const uint4*>(&lds[0][wm + x * 16 + li][8 * lk]);
+            al[x] = *reinterpret_cast<const uint4*>(&lds[1][wm + x * 16 + li][8 * lk]);
         }
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 8; ++y) {
+            const uint4 bh = *reinterpret_cast<const uint4*>(&lds[2][wn + y * 16 + li][8 * lk]);
+            const uint4 bl = *reinterpret_cast<const uint4*>(&lds[3][wn + y * 16 + li][8 * lk]);
 #pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                acc[x][y] = mfma16<F>(ah[x], bh[y], acc[x][y]);
-                acc[x][y] = mfma16<F>(ah[x], bl[y], acc[x][y]);
-                acc[x][y] = mfma16<F>(al[x], bh[y], acc[x][y]);
+            for (int x = 0; x < 4; ++x) {
+                acc[x][y] = mfma16<F>(ah[x], bh, acc[x][y]);
+                acc[x][y] = mfma16<F>(ah[x], bl, acc[x][y]);
+                acc[x][y] = mfma16<F>(al[x], bh, acc[x][y]);
             }
+        }
     }
 
+    // epilogue: undo the row scale, sum squares per column over the tile's rows
     double inv_rs[4][4];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             inv_rs[x][r] = 1.0 / mscale[(size_t)w * np + r0 + wm + x * 16 + 4 * lk + r];
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(&lds[0][0][0]);      // [4][256] doubles
 #pragma unroll
-    for (int y = 0; y < 4; ++y) {
-        double s = 0.0;
+    for (int y = 0; y < 8; ++y) {
+        double sum = 0.0;
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const double z = (double)acc[x][y][r] * inv_rs[x][r];
-                s += z * z;
+                sum += z * z;
             }
-        s += __shfl_xor(s, 16);
-        s += __shfl_xor(s, 32);
-        if (lk == 0) red[wave >> 1][wn + y * 16 + li] = s;
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        if (lk == 0) red[(wave >> 1) * kSBN + wn + y * 16 + li] = sum;
     }
     __syncthreads();
-    if (t < kCT) qpart[((size_t)w * (np / kCT) + rt) * mpad + c0 + t] = red[0][t] + red[1][t];
+    if (t < kSBN) {
+        const double v = red[t] + red[kSBN + t] + red[2 * kSBN + t] + red[3 * kSBN + t];
+        qpart[((size_t)w * (np / kSBM) + rt) * mpad + c0 + t] = v;
+    }
 }
 
 // Diagnostic: one 16x16x32 MFMA per wave; a/b: [t][lane][8] 16-bit patterns.
@@ -217,13 +250,14 @@ __global__ __launch_bounds__(64) void k_probe16(const uint4* __restrict__ a,
 }
 
 __global__ void k_reduce_q_scaled(const double* __restrict__ qpart,
-                                  const double* __restrict__ xscale, double* __restrict__ q,
-                                  int nrt, int m, int mpad, int f0) {
+                                  const double* __restrict__ xscale, const int* __restrict__ colmap,
+                                  double* __restrict__ q, int nrt, int m, int mpad, int f0) {
     const int w = blockIdx.y;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x) {
+        const int cu = colmap ? colmap[c] : c;
         double s = 0.0;
-        for (int rt = 0; rt < nrt; ++rt) s += qpart[((size_t)w * nrt + rt) * mpad + c];
-        const double xs = xscale[c];
+        for (int rt = 0; rt < nrt; ++rt) s += qpart[((size_t)w * nrt + rt) * mpad + cu];
+        const double xs = xscale[cu];
         q[(size_t)(f0 + w) * m + c] = -s / (xs * xs);
     }
 }
@@ -258,15 +292,18 @@ hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_
 hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_t* mlo,
                                  const double* mscale, const uint16_t* xhi, const uint16_t* xlo,
                                  double* qpart, int np, int mpad, int nb, hipStream_t s) {
-    const dim3 grid(mpad / kCT, np / kCT, nb);
+    if (np % kSBM != 0 || mpad % kSBN != 0) return hipErrorInvalidValue;
+    const dim3 grid(mpad / kSBN, np / kSBM, nb);
     if (fmt == Split16::BF16)
-        hipLaunchKernelGGL(k_contract_split<Split16::BF16>, grid, dim3(256), 0, s, mhi, mlo, mscale,
+        hipLaunchKernelGGL(k_contract_split<Split16::BF16>, grid, dim3(512), 0, s, mhi, mlo, mscale,
                            xhi, xlo, qpart, np, mpad);
     else
-        hipLaunchKernelGGL(k_contract_split<Split16::FP16>, grid, dim3(256), 0, s, mhi, mlo, mscale,
+        hipLaunchKernelGGL(k_contract_split<Split16::FP16>, grid, dim3(512), 0, s, mhi, mlo, mscale,
                            xhi, xlo, qpart, np, mpad);
     return hipGetLastError();
 }
+
+int contract_split_row_tiles(int np) { return np / kSBM; }
 
 hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
                          int ntests, hipStream_t s) {
@@ -279,11 +316,12 @@ hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, flo
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, double* q, int np,
-                                  int m, int mpad, int f0, int nb, hipStream_t s) {
+hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, const int* colmap,
+                                  double* q, int nrt, int m, int mpad, int f0, int nb,
+                                  hipStream_t s) {
     const int gx = (int)((m + 255) / 256 < 4096 ? (m + 255) / 256 : 4096);
     hipLaunchKernelGGL(k_reduce_q_scaled, dim3(gx > 0 ? gx : 1, nb), dim3(256), 0, s, qpart, xscale,
-                       q, np / kCT, m, mpad, f0);
+                       colmap, q, nrt, m, mpad, f0);
     return hipGetLastError();
 }
 

@@ -171,7 +171,7 @@ class HipContext:
         t = np.zeros(12)
         self.call("mgw_hip_last_timings", _ptr(t), 12)
         keys = ["build_v", "dielectric", "factorize", "contract", "sigma_c", "pade_qp", "total",
-                "freq_batch", "contract_launches", "split_reduce"]
+                "freq_batch", "contract_launches", "split_reduce", "contract_columns"]
         return dict(zip(keys, t.tolist()))
 
     def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
