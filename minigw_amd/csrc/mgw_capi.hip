@@ -90,7 +90,7 @@ struct mgw_hip_ctx {
     Buf eri, v, colidx, rowbase, delta;
     int nmo = 0, nocc = 0;        // of the resident ERI (0 = none)
     bool eri_pk_sym = false;      // (pk|ia) == (kp|ia) exactly for the resident ERI
-    Buf flag, hcolmap;
+    Buf flag, hcolmap, wxp, hxp;
     int n = 0, np = 0;            // ph dimension of the resident V
     bool v_ready = false;
     // single-frequency workspace (compute_wc / quadratic forms)
@@ -102,7 +102,7 @@ struct mgw_hip_ctx {
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
 
     mgw_hip_ctx() {
-        all = {&flag, &hcolmap, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
+        all = {&flag, &hcolmap, &wxp, &hxp, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale};
@@ -439,8 +439,10 @@ void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow,
     double* q = as<double>(ctx->get(ctx->wq, (size_t)mpad * 8, "q"));
     ctx->h2d(dpk, pk.data(), mpad);
     ctx->h2d(dcol, hcolidx, np);
-    hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), src, dpk,
-                                       dcol, qpart, np, mpad, 1, ctx->stream),
+    double* xp = as<double>(ctx->get(ctx->wxp, (size_t)mpad * np * 8, "packed panel"));
+    hip_check(mgw::launch_pack_x(src, dpk, dcol, xp, ctx->n, np, mpad, ctx->stream), "pack X");
+    hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), xp, qpart, np,
+                                       mpad, 1, ctx->stream),
               "contract");
     hip_check(mgw::launch_reduce_q(qpart, nullptr, q, np / mgw::kCT, m, mpad, 0, 1, ctx->stream),
               "reduce q");
@@ -631,6 +633,13 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const mgw::Split16 fmt = mgw::Split16::FP16;
         uint16_t *xh = nullptr, *xl = nullptr;
         double* xs = nullptr;
+        double* xp = nullptr;
+        if (!mixed) {
+            xp = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X"));
+            hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, n,
+                                         np, mpad, st),
+                      "pack X");
+        }
         if (mixed) {
             xh = as<uint16_t>(ctx->get(ctx->xhi, (size_t)mpad * np * 2, "X hi"));
             xl = as<uint16_t>(ctx->get(ctx->xlo, (size_t)mpad * np * 2, "X lo"));
@@ -682,9 +691,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b, st),
                           "contract split");
             else
-                hip_check(mgw::launch_contract_f64(A, S, as<double>(ctx->eri.p), dpk,
-                                                   as<int>(ctx->colidx.p), qpart, np, mpad, b, st),
-                          "contract");
+                hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");
             hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dcolmap, dq,
@@ -809,7 +816,8 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
     const size_t eri = (size_t)nmo * nmo * nmo * nmo * 8;
     const size_t v = (size_t)L.np * L.np * 8;
     size_t fixed = eri + v + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
-    if (precision == MGW_PREC_MIXED) fixed += (size_t)mpad * L.np * 4 + (size_t)mpad * 8;
+    fixed += precision == MGW_PREC_MIXED ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8
+                                         : (size_t)mpad * L.np * 8;
     const int nb = std::max(1, std::min(nbatch > 0 ? nbatch : nfreq, nfreq));
     return fixed + (size_t)nb * batch_bytes(L.np, mpad, precision);
 }

@@ -51,13 +51,16 @@ hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipS
 hipError_t launch_trtri(double* A, const double* dinv, double* tbuf, int np, int nb,
                         hipStream_t s);
 
-// qpart[w][rt][c] = sum_{r in row tile rt} (sum_k M_w[r][k] S_w[k] X[k][c])^2 with
-// X[k][c] = src[pkrow[c] + colidx[k]] (k < np, c < mpad), M_w lower triangular.
-hipError_t launch_contract_f64(const double* M, const double* S, const double* src,
-                               const int64_t* pkrow, const int* colidx, double* qpart, int np,
-                               int mpad, int nb, hipStream_t s);
+// xp[c][k] = src[pkrow[c] + colidx[k]] for k < n, 0 for n <= k < np (mpad x np):
+// the P panels packed once per molecule, K-contiguous per column.
+hipError_t launch_pack_x(const double* src, const int64_t* pkrow, const int* colidx, double* xp,
+                         int n, int np, int mpad, hipStream_t s);
 
-// q[(f0+w) * m + c] = -sum_rt qpart[w][rt][colmap[c]] (colmap NULL = identity).
+// qpart[w][rt][c] = sum_{r in row tile rt} (sum_k M_w[r][k] S_w[k] xp[c][k])^2,
+// M_w lower triangular, 128-row tiles (np, mpad multiples of 128).
+hipError_t launch_contract_f64(const double* M, const double* S, const double* xp, double* qpart,
+                               int np, int mpad, int nb, hipStream_t s);
+
 // nrt = row tiles of the contraction that wrote qpart (np / 128 for FP64,
 // contract_split_row_tiles(np) for the split form).
 hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
@@ -74,12 +77,14 @@ hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int* f
 enum class Split16 : int { BF16 = 0, FP16 = 1 };
 
 // X[k][c] = src[pkrow[c] + colidx[k]] for k < n (0 for k >= n) -> planes
-// xhi/xlo laid out [c][k] (mpad x np); xscale[c] = applied scale (1 for BF16).
+// xhi/xlo (mpad x np halves, blocks of 256 columns x 32 k, each block
+// row-major, blocks ordered column tile then k); xscale[c] = applied scale.
 hipError_t launch_split_x(Split16 fmt, const double* src, const int64_t* pkrow, const int* colidx,
                           uint16_t* xhi, uint16_t* xlo, double* xscale, int n, int np, int mpad,
                           hipStream_t s);
 
-// (M_w S_w)[r][k] for k <= r (0 above) -> planes [w][r][k]; mscale[w][r].
+// (M_w S_w)[r][k] for k <= r (0 above) -> planes [w] (np x np halves in the
+// same 256 x 32 block layout); mscale[w][r].
 hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_t* mhi,
                           uint16_t* mlo, double* mscale, int np, int nb, hipStream_t s);
 

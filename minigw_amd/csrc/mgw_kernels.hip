@@ -215,35 +215,28 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ A,
     tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] = v; });
 }
 
-__device__ inline void tri_index(int t, int& ii, int& jj) {
-    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((i + 1) * (i + 2) / 2 <= t) ++i;
-    while (i * (i + 1) / 2 > t) --i;
-    ii = i;
-    jj = t - i * (i + 1) / 2;
-}
-
-// Trailing update: A_ij -= L_ik L_jk^T for k < j <= i.
-__global__ __launch_bounds__(256) void k_syrk(double* __restrict__ A, int np, int k) {
+// Left-looking update of block column k: A_ik -= sum_{j<k} L_ij L_kj^T, i >= k
+// (one 64 x 64 tile per workgroup, K = 64 k).  Reads only finished columns of
+// L, so each trailing element is updated once per block column instead of
+// being re-read and re-written after every panel.
+__global__ __launch_bounds__(256) void k_potrf_update(double* __restrict__ A, int np, int k) {
     __shared__ double As[kBK][64 + kLdsPad];
     __shared__ double Bs[kBK][64 + kLdsPad];
-    const int w = blockIdx.y;
-    int ii, jj;
-    tri_index(blockIdx.x, ii, jj);
-    const int i = k + 1 + ii, j = k + 1 + jj;
+    const int w = blockIdx.y, i = k + blockIdx.x;
     double* Aw = A + (size_t)w * np * np;
-    const double* Lik = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
-    const double* Ljk = Aw + (size_t)j * kNB * np + (size_t)k * kNB;
+    const double* Li = Aw + (size_t)i * kNB * np;
+    const double* Lk = Aw + (size_t)k * kNB * np;
     Tile64 tile;
     tile.zero();
-    for (int k0 = 0; k0 < kNB; k0 += kBK) {
-        load_rows_kmajor(As, Lik, np, 0, k0);
-        load_rows_kmajor(Bs, Ljk, np, 0, k0);
+    const int kend = k * kNB;
+    for (int k0 = 0; k0 < kend; k0 += kBK) {
+        load_rows_kmajor(As, Li, np, 0, k0);
+        load_rows_kmajor(Bs, Lk, np, 0, k0);
         __syncthreads();
         tile.step(As, Bs);
         __syncthreads();
     }
-    double* out = Aw + (size_t)i * kNB * np + (size_t)j * kNB;
+    double* out = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
     tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] -= v; });
 }
 
@@ -306,9 +299,8 @@ __global__ __launch_bounds__(256) void k_trtri_m(double* __restrict__ A,
 // over the tile's rows -> qpart[w][row tile][column].
 // 4 waves in 2 x 2, each a 64 x 64 sub-tile (4 x 4 MFMA blocks).
 __global__ __launch_bounds__(256) void k_contract_f64(
-    const double* __restrict__ M, const double* __restrict__ S, const double* __restrict__ src,
-    const int64_t* __restrict__ pkrow, const int* __restrict__ colidx, double* __restrict__ qpart,
-    int np, int mpad) {
+    const double* __restrict__ M, const double* __restrict__ S, const double* __restrict__ xp,
+    double* __restrict__ qpart, int np, int mpad) {
     __shared__ double As[kBK][kCT + kLdsPad];
     __shared__ double Bs[kBK][kCT + kLdsPad];
     __shared__ double red[2][kCT];
@@ -328,14 +320,19 @@ __global__ __launch_bounds__(256) void k_contract_f64(
     // the next slab is fetched into registers while the current one is used
     const int lrow = t >> 1, lkq = (t & 1) * 8;
     const double* arow = Mw + (size_t)(r0 + lrow) * np;
-    const double* bsrc = src + pkrow[c0 + lrow];
+    const double* bsrc = xp + (size_t)(c0 + lrow) * np + lkq;
     double ra[8], rb[8];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int kk = k0 + lkq + u;
             ra[u] = (kk <= r0 + lrow) ? arow[kk] * Sw[kk] : 0.0;
-            rb[u] = bsrc[colidx[kk]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(bsrc + k0 + u);
+            rb[u] = v.x;
+            rb[u + 1] = v.y;
         }
     };
 
@@ -384,6 +381,17 @@ __global__ __launch_bounds__(256) void k_contract_f64(
         const double v = red[0][t] + red[1][t];
         qpart[((size_t)w * (np / kCT) + rt) * mpad + c0 + t] = v;
     }
+}
+
+// xp[c][k] = src[pkrow[c] + colidx[k]] for k < n, 0 for k >= n (mpad x np).
+__global__ void k_pack_x(const double* __restrict__ src, const int64_t* __restrict__ pkrow,
+                         const int* __restrict__ colidx, double* __restrict__ xp, int n, int np,
+                         int mpad) {
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= mpad) return;
+    const int64_t base = pkrow[c];
+    double* o = xp + (size_t)c * np;
+    for (int k = lane; k < np; k += 64) o[k] = k < n ? src[base + colidx[k]] : 0.0;
 }
 
 __global__ void k_reduce_q(const double* __restrict__ qpart, const int* __restrict__ colmap,
@@ -638,12 +646,11 @@ hipError_t launch_dielectric(const double* v, const double* S, double* A, int np
 hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipStream_t s) {
     const int nblk = np / kNB;
     for (int k = 0; k < nblk; ++k) {
+        if (k > 0)
+            hipLaunchKernelGGL(k_potrf_update, dim3(nblk - k, nb), dim3(256), 0, s, A, np, k);
         hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k);
         const int t = nblk - k - 1;
-        if (t > 0) {
-            hipLaunchKernelGGL(k_trsm_panel, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k);
-            hipLaunchKernelGGL(k_syrk, dim3(t * (t + 1) / 2, nb), dim3(256), 0, s, A, np, k);
-        }
+        if (t > 0) hipLaunchKernelGGL(k_trsm_panel, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k);
     }
     return hipGetLastError();
 }
@@ -659,11 +666,17 @@ hipError_t launch_trtri(double* A, const double* dinv, double* tbuf, int np, int
     return hipGetLastError();
 }
 
-hipError_t launch_contract_f64(const double* M, const double* S, const double* src,
-                               const int64_t* pkrow, const int* colidx, double* qpart, int np,
-                               int mpad, int nb, hipStream_t s) {
-    hipLaunchKernelGGL(k_contract_f64, dim3(mpad / kCT, np / kCT, nb), dim3(256), 0, s, M, S, src,
-                       pkrow, colidx, qpart, np, mpad);
+hipError_t launch_pack_x(const double* src, const int64_t* pkrow, const int* colidx, double* xp,
+                         int n, int np, int mpad, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_x, dim3((mpad + 3) / 4), dim3(256), 0, s, src, pkrow, colidx, xp, n,
+                       np, mpad);
+    return hipGetLastError();
+}
+
+hipError_t launch_contract_f64(const double* M, const double* S, const double* xp, double* qpart,
+                               int np, int mpad, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_contract_f64, dim3(mpad / kCT, np / kCT, nb), dim3(256), 0, s, M, S, xp,
+                       qpart, np, mpad);
     return hipGetLastError();
 }
 

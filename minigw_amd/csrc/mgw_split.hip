@@ -33,6 +33,13 @@ __device__ inline uint16_t to_f16_bits(float f) {
 }
 __device__ inline float from_f16_bits(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// Plane layout: blocks of 256 rows x 32 k (8192 halves, row-major inside),
+// blocks ordered by row tile then k block, so one K step of a 256-row tile is
+// one contiguous 16 KB block per plane.
+__device__ inline size_t tiled_index(int row, int k, int np) {
+    return ((size_t)(row >> 8) * (np >> 5) + (k >> 5)) * 8192 + (size_t)(row & 255) * 32 + (k & 31);
+}
+
 template <Split16 F>
 __device__ inline void split_value(double x, uint16_t& hi, uint16_t& lo) {
     if constexpr (F == Split16::BF16) {
@@ -75,14 +82,13 @@ __global__ __launch_bounds__(256) void k_split_x(const double* __restrict__ src,
         for (int k = lane; k < n; k += 64) mx = fmax(mx, fabs(src[base + colidx[k]]));
         scale = scale_for_max(wave_max(mx));
     }
-    uint16_t* oh = xhi + (size_t)c * np;
-    uint16_t* ol = xlo + (size_t)c * np;
     for (int k = lane; k < np; k += 64) {
         const double x = k < n ? src[base + colidx[k]] * scale : 0.0;
         uint16_t h, l;
         split_value<F>(x, h, l);
-        oh[k] = h;
-        ol[k] = l;
+        const size_t o = tiled_index(c, k, np);
+        xhi[o] = h;
+        xlo[o] = l;
     }
     if (lane == 0) xscale[c] = scale;
 }
@@ -105,14 +111,15 @@ __global__ __launch_bounds__(256) void k_split_m(const double* __restrict__ M,
         for (int k = lane; k <= r; k += 64) mx = fmax(mx, fabs(row[k] * Sw[k]));
         scale = scale_for_max(wave_max(mx));
     }
-    uint16_t* oh = mhi + (size_t)w * np * np + (size_t)r * np;
-    uint16_t* ol = mlo + (size_t)w * np * np + (size_t)r * np;
+    uint16_t* oh = mhi + (size_t)w * np * np;
+    uint16_t* ol = mlo + (size_t)w * np * np;
     for (int k = lane; k < np; k += 64) {
         const double x = k <= r ? (row[k] * Sw[k]) * scale : 0.0;
         uint16_t h, l;
         split_value<F>(x, h, l);
-        oh[k] = h;
-        ol[k] = l;
+        const size_t o = tiled_index(r, k, np);
+        oh[o] = h;
+        ol[o] = l;
     }
     if (lane == 0) mscale[(size_t)w * np + r] = scale;
 }
@@ -163,9 +170,9 @@ __global__ __launch_bounds__(512) void k_contract_split(
         lrow[u] = idx >> 2;
         lkq[u] = (idx & 3) * 8;
         const uint16_t* base = plane == 0 ? mhi : plane == 1 ? mlo : plane == 2 ? xhi : xlo;
-        const size_t row = plane < 2 ? (size_t)w * np * np + (size_t)(r0 + lrow[u]) * np
-                                     : (size_t)(c0 + lrow[u]) * np;
-        src[u] = base + row + lkq[u];
+        const size_t off = plane < 2 ? (size_t)w * np * np + tiled_index(r0 + lrow[u], lkq[u], np)
+                                     : tiled_index(c0 + lrow[u], lkq[u], np);
+        src[u] = base + off;
     }
     uint4 pre[8];
 #pragma unroll
@@ -187,7 +194,7 @@ __global__ __launch_bounds__(512) void k_contract_split(
         if (k0 + kSBK < kend) {
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                pre[u] = *reinterpret_cast<const uint4*>(src[u] + k0 + kSBK);
+                pre[u] = *reinterpret_cast<const uint4*>(src[u] + (size_t)((k0 + kSBK) >> 5) * 8192);
         }
         uint4 ah[4], al[4];
 #pragma unroll
