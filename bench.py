@@ -92,6 +92,9 @@ def main() -> int:
         ctx.upload_eri(inp.eri, inp.nocc)
         log(f"bench: ERI resident ({inp.eri.nbytes / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
 
+        peaks = ctx.measure_peaks()
+        log("bench: measured peaks " + ", ".join(f"{k} {v / 1e12:.1f}T" for k, v in peaks.items()))
+
         def step(p):
             return hip.run_g0w0(inp, args.nfreq, args.npar, precision=p, ctx=ctx, upload=False)
 
@@ -146,7 +149,10 @@ def main() -> int:
                      else "k_contract_f64", "achieved": round(achieved, 2),
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4),
                      "traffic": None, "launch_ms": round(t_launch * 1e3, 3),
-                     "flops_per_launch": flops_launch},
+                     "flops_per_launch": flops_launch,
+                     "measured_peak": round((peaks["fp16_mfma_32x32"] if prec == hip.PREC_MIXED
+                                             else peaks["fp64_mfma"]) / 1e12, 1)},
+        "measured_peaks_tflops_or_tbps": {k: round(v / 1e12, 2) for k, v in peaks.items()},
     }
     if f64 is not None:
         fv = nmo * len(f64_lat) / sum(f64_lat)

@@ -165,9 +165,17 @@ int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, do
  * with per-lane inputs, raw per-lane results (lane layout check).
  * kind 0: v_mfma_f64_16x16x4, a/b double[ntests][64], d double[ntests][64][4];
  * kind 1/2: v_mfma_f32_16x16x32 bf16/f16, a/b uint16[ntests][64][8],
- * d float[ntests][64][4]. */
+ * d float[ntests][64][4]; kind 3/4: v_mfma_f32_32x32x16 bf16/f16, same
+ * inputs, d float[ntests][64][16]. */
 int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const void* a,
                        const void* b, void* d);
+
+/* Diagnostic with no counterpart in the reference: measured peaks of this
+ * device -- out[0] FP64 MFMA, out[1] FP16 MFMA, out[2] BF16 MFMA (flop/s,
+ * 16x16 tiles, register-resident operands, 32 waves per CU), out[3] HBM copy
+ * bandwidth (bytes read + written per second, 2 GiB buffers), out[4] FP16 and
+ * out[5] BF16 MFMA with 32x32x16 tiles.  out holds 6 entries. */
+int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out);
 
 /* Transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:62-93). */
 int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight);

@@ -777,9 +777,10 @@ int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const voi
                        const void* b, void* d) {
     return guarded(ctx, [&] {
         need(ctx && a && b && d && ntests > 0 && ntests <= 65536, "mgw_hip_probe_mfma: bad argument");
-        need(kind >= 0 && kind <= 2, "mgw_hip_probe_mfma: kind must be 0 (f64), 1 (bf16), 2 (f16)");
+        need(kind >= 0 && kind <= 4,
+             "mgw_hip_probe_mfma: kind must be 0 (f64), 1/2 (bf16/f16 16x16), 3/4 (bf16/f16 32x32)");
         const size_t in_bytes = kind == 0 ? 64 * 8 : 64 * 16;
-        const size_t out_bytes = kind == 0 ? 64 * 4 * 8 : 64 * 4 * 4;
+        const size_t out_bytes = kind == 0 ? 64 * 4 * 8 : kind <= 2 ? 64 * 4 * 4 : 64 * 16 * 4;
         char* buf = as<char>(ctx->get(ctx->wsrc, (size_t)ntests * (2 * in_bytes + out_bytes), "probe"));
         char* da = buf;
         char* db = buf + (size_t)ntests * in_bytes;
@@ -789,12 +790,61 @@ int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const voi
         if (kind == 0)
             hip_check(mgw::launch_probe_f64(as<double>(da), as<double>(db), as<double>(dd), ntests,
                                             ctx->stream), "probe f64");
-        else
+        else if (kind <= 2)
             hip_check(mgw::launch_probe16(kind == 1 ? mgw::Split16::BF16 : mgw::Split16::FP16,
                                           as<uint16_t>(da), as<uint16_t>(db), as<float>(dd), ntests,
                                           ctx->stream), "probe 16");
+        else
+            hip_check(mgw::launch_probe32(kind == 3 ? mgw::Split16::BF16 : mgw::Split16::FP16,
+                                          as<uint16_t>(da), as<uint16_t>(db), as<float>(dd), ntests,
+                                          ctx->stream), "probe 32");
         ctx->d2h(static_cast<char*>(d), dd, (size_t)ntests * out_bytes);
         ctx->sync();
+    });
+}
+
+int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        need(ctx && out, "mgw_hip_measure_peaks: NULL argument");
+        int dev = 0, cus = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                  "hipDeviceGetAttribute");
+        const size_t count = (size_t)1 << 28;                 // 2 GiB of doubles
+        char* buf = as<char>(ctx->get(ctx->wsrc, 2 * count * 8 + 4096, "peak scratch"));
+        float* sink = reinterpret_cast<float*>(buf + 2 * count * 8);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const double flop[5] = {2048.0 * 8, 16384.0 * 8, 16384.0 * 8, 32768.0 * 4, 32768.0 * 4};
+        const int iters[5] = {2000, 4000, 4000, 4000, 4000};
+        for (int kind = 0; kind < 5; ++kind) {
+            const int blocks = cus * 8;
+            hip_check(mgw::launch_mfma_peak(kind, blocks, 10, sink, ctx->stream), "mfma peak");
+            hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+            hip_check(mgw::launch_mfma_peak(kind, blocks, iters[kind], sink, ctx->stream), "mfma peak");
+            hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            const double total = (double)blocks * 4 * iters[kind] * flop[kind];
+            out[kind < 3 ? kind : kind + 1] = total / (ms * 1e-3);
+        }
+        double* a = reinterpret_cast<double*>(buf);
+        double* b = a + count;
+        hip_check(hipMemsetAsync(a, 0, count * 8, ctx->stream), "hipMemsetAsync");
+        hip_check(mgw::launch_stream_copy(a, b, count, ctx->stream), "stream copy");
+        hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+        for (int r = 0; r < 5; ++r)
+            hip_check(mgw::launch_stream_copy(a, b, count, ctx->stream), "stream copy");
+        hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        out[3] = 5.0 * 2.0 * count * 8 / (ms * 1e-3);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->release(ctx->wsrc);
     });
 }
 

@@ -98,6 +98,8 @@ hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_
 
 hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
                          int ntests, hipStream_t s);
+hipError_t launch_probe32(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
+                         int ntests, hipStream_t s);
 
 // q[(f0+w) * m + c] = -(sum_rt qpart[w][rt][cu]) / xscale[cu]^2, cu = colmap[c].
 hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, const int* colmap,
@@ -122,5 +124,11 @@ hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const doub
                           const double* mo_energy, const double* sx, const double* vxc_diag,
                           const int* states, double fermi, int nstates, int nfreq, int npar,
                           double* qp, int* iters, hipStream_t s);
+
+// Peak calibration: `blocks` x 256 threads issuing 8 x iters independent MFMAs
+// (kind 0 f64 16x16x4, 1 f16 16x16x32, 2 bf16 16x16x32) on registers; and a
+// streaming copy of `count` doubles (count % 4 == 0).
+hipError_t launch_mfma_peak(int kind, int blocks, int iters, float* sink, hipStream_t s);
+hipError_t launch_stream_copy(const double* src, double* dst, size_t count, hipStream_t s);
 
 }  // namespace mgw

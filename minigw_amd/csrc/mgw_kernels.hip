@@ -140,6 +140,54 @@ __device__ inline void load_kn(double (*Bs)[64 + kLdsPad], const double* __restr
     for (int u = 0; u < 4; ++u) Bs[k][nq + u] = p[u];
 }
 
+// acc += A[0:64, kbeg:kend] * op(B) with A row-major (lda), op(B) = B^T for a
+// row-major 64-row B (NT) or B[kbeg:kend, 0:64] row-major (NN).  The next K
+// slab is fetched into registers while the current one is multiplied.
+template <bool NN>
+__device__ inline void gemm64(const double* __restrict__ a, size_t lda,
+                              const double* __restrict__ b, size_t ldb, int kbeg, int kend,
+                              Tile64& tile, double (*As)[64 + kLdsPad],
+                              double (*Bs)[64 + kLdsPad]) {
+    const int t = threadIdx.x;
+    const int am = t >> 2, akq = (t & 3) * 4;
+    const int bk = t >> 4, bnq = (t & 15) * 4;
+    double2 ra[2], rb[2];
+    auto fetch = [&](int k0) {
+        const double2* pa = reinterpret_cast<const double2*>(a + (size_t)am * lda + k0 + akq);
+        ra[0] = pa[0];
+        ra[1] = pa[1];
+        const double2* pb =
+            NN ? reinterpret_cast<const double2*>(b + (size_t)(k0 + bk) * ldb + bnq)
+               : reinterpret_cast<const double2*>(b + (size_t)am * ldb + k0 + akq);
+        rb[0] = pb[0];
+        rb[1] = pb[1];
+    };
+    if (kbeg >= kend) return;
+    fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+        __syncthreads();
+        As[akq + 0][am] = ra[0].x;
+        As[akq + 1][am] = ra[0].y;
+        As[akq + 2][am] = ra[1].x;
+        As[akq + 3][am] = ra[1].y;
+        if (NN) {
+            Bs[bk][bnq + 0] = rb[0].x;
+            Bs[bk][bnq + 1] = rb[0].y;
+            Bs[bk][bnq + 2] = rb[1].x;
+            Bs[bk][bnq + 3] = rb[1].y;
+        } else {
+            Bs[akq + 0][am] = rb[0].x;
+            Bs[akq + 1][am] = rb[0].y;
+            Bs[akq + 2][am] = rb[1].x;
+            Bs[akq + 3][am] = rb[1].y;
+        }
+        __syncthreads();
+        if (k0 + kBK < kend) fetch(k0 + kBK);
+        tile.step(As, Bs);
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------ potrf pieces
 // Diagonal block k of every matrix: Cholesky in LDS, then the inverse of the
 // 64 x 64 factor by column-wise forward substitution.
@@ -204,13 +252,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ A,
     const double* Aik = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
     Tile64 tile;
     tile.zero();
-    for (int k0 = 0; k0 < kNB; k0 += kBK) {
-        load_rows_kmajor(As, Aik, np, 0, k0);
-        load_rows_kmajor(Bs, D, 64, 0, k0);
-        __syncthreads();
-        tile.step(As, Bs);
-        __syncthreads();
-    }
+    gemm64<false>(Aik, np, D, 64, 0, kNB, tile, As, Bs);
     double* out = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
     tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] = v; });
 }
@@ -228,14 +270,7 @@ __global__ __launch_bounds__(256) void k_potrf_update(double* __restrict__ A, in
     const double* Lk = Aw + (size_t)k * kNB * np;
     Tile64 tile;
     tile.zero();
-    const int kend = k * kNB;
-    for (int k0 = 0; k0 < kend; k0 += kBK) {
-        load_rows_kmajor(As, Li, np, 0, k0);
-        load_rows_kmajor(Bs, Lk, np, 0, k0);
-        __syncthreads();
-        tile.step(As, Bs);
-        __syncthreads();
-    }
+    gemm64<false>(Li, np, Lk, np, 0, k * kNB, tile, As, Bs);
     double* out = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
     tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] -= v; });
 }
@@ -253,13 +288,7 @@ __global__ __launch_bounds__(256) void k_trtri_t(const double* __restrict__ A,
     const double* Bcol = Aw + (size_t)j * kNB;          // column block j
     Tile64 tile;
     tile.zero();
-    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-        load_rows_kmajor(As, Arow, np, 0, k0);
-        load_kn(Bs, Bcol, np, k0, 0);
-        __syncthreads();
-        tile.step(As, Bs);
-        __syncthreads();
-    }
+    gemm64<true>(Arow, np, Bcol, np, kbeg, kend, tile, As, Bs);
     double* out = tbuf + (size_t)w * np * kNB + (size_t)i * kNB * kNB;
     tile.for_each([&](int r, int c, double v) { out[r * kNB + c] = v; });
 }
@@ -305,10 +334,9 @@ __global__ __launch_bounds__(256) void k_contract_f64(
     __shared__ double Bs[kBK][kCT + kLdsPad];
     __shared__ double red[2][kCT];
     const int w = blockIdx.z;
-    // large-K row tiles first; consecutive tiles of one row panel on one XCD
-    const int ncol = gridDim.x, nrow = gridDim.y, T = ncol * nrow;
+    // large-K row tiles first; the column tiles of one row panel run together
+    const int ncol = gridDim.x, nrow = gridDim.y;
     int lin = blockIdx.x + ncol * blockIdx.y;
-    if (T % 8 == 0) lin = (lin % 8) * (T / 8) + lin / 8;
     const int rt = nrow - 1 - lin / ncol, r0 = rt * kCT, c0 = (lin % ncol) * kCT;
     const double* Mw = M + (size_t)w * np * np;
     const double* Sw = S + (size_t)w * np;

@@ -1,0 +1,95 @@
+// Peak calibration kernels: MFMA issue rate on register-resident operands and
+// HBM streaming bandwidth.  Used to state measured roofline peaks beside the
+// vendor figures.
+#include "mgw_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mgw {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+// kind 0: v_mfma_f64_16x16x4 (2048 flop, 8 accumulators), 1: f16 16x16x32 (16384),
+// 2: bf16 16x16x32 (16384), 3: f16 32x32x16 (32768, 4 accumulators), 4: bf16 32x32x16
+template <int KIND>
+__global__ __launch_bounds__(256) void k_mfma_peak(int iters, float* sink) {
+    const int lane = threadIdx.x & 63;
+    if constexpr (KIND == 0) {
+        d4 acc[8];
+        for (int i = 0; i < 8; ++i) acc[i] = d4{0, 0, 0, 0};
+        const double a = 1.0 + lane * 1e-3, b = 1.0 - lane * 1e-3;
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+        double s = 0;
+        for (int i = 0; i < 8; ++i) s += acc[i][0] + acc[i][3];
+        if (s == 12345.0) sink[threadIdx.x] = (float)s;
+    } else if constexpr (KIND >= 3) {
+        typedef float f16v __attribute__((ext_vector_type(16)));
+        f16v acc[4];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 16; ++j) acc[i][j] = 0.f;
+        uint4 av = make_uint4(0x3c003c00u + lane, 0x3c003c00u, 0x3c003c00u, 0x3c003c00u);
+        uint4 bv = make_uint4(0x3c003c00u, 0x3c003c00u + lane, 0x3c003c00u, 0x3c003c00u);
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if constexpr (KIND == 3)
+                    acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, av),
+                                                                    __builtin_bit_cast(h8, bv), acc[i], 0, 0, 0);
+                else
+                    acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, av),
+                                                                     __builtin_bit_cast(b8, bv), acc[i], 0, 0, 0);
+            }
+        float s = 0;
+        for (int i = 0; i < 4; ++i) s += acc[i][0] + acc[i][15];
+        if (s == 12345.0f) sink[threadIdx.x] = s;
+    } else {
+        f4 acc[8];
+        for (int i = 0; i < 8; ++i) acc[i] = f4{0, 0, 0, 0};
+        uint4 av = make_uint4(0x3c003c00u + lane, 0x3c003c00u, 0x3c003c00u, 0x3c003c00u);
+        uint4 bv = make_uint4(0x3c003c00u, 0x3c003c00u + lane, 0x3c003c00u, 0x3c003c00u);
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if constexpr (KIND == 1)
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, av),
+                                                                    __builtin_bit_cast(h8, bv), acc[i], 0, 0, 0);
+                else
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, av),
+                                                                     __builtin_bit_cast(b8, bv), acc[i], 0, 0, 0);
+            }
+        float s = 0;
+        for (int i = 0; i < 8; ++i) s += acc[i][0] + acc[i][3];
+        if (s == 12345.0f) sink[threadIdx.x] = s;
+    }
+}
+
+__global__ void k_stream_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+hipError_t launch_mfma_peak(int kind, int blocks, int iters, float* sink, hipStream_t s) {
+    if (kind == 0) hipLaunchKernelGGL(k_mfma_peak<0>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 1) hipLaunchKernelGGL(k_mfma_peak<1>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 2) hipLaunchKernelGGL(k_mfma_peak<2>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 3) hipLaunchKernelGGL(k_mfma_peak<3>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else hipLaunchKernelGGL(k_mfma_peak<4>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(const double* src, double* dst, size_t count, hipStream_t s) {
+    const size_t n4 = count / 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, s,
+                       reinterpret_cast<const double4*>(src), reinterpret_cast<double4*>(dst), n4);
+    return hipGetLastError();
+}
+
+}  // namespace mgw

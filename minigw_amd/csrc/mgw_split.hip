@@ -2,7 +2,7 @@
 //
 // q_c = -|| M S x_c ||^2 with M S and X each written as hi + lo in a 16-bit
 // format; Z = Mh Xh + Mh Xl + Ml Xh (the lo*lo term, ~u^2 relative, is
-// dropped) runs on v_mfma_f32_16x16x32_{bf16,f16} with FP32 accumulation; the
+// dropped) runs on v_mfma_f32_32x32x16_{bf16,f16} with FP32 accumulation; the
 // row scale is undone and the squares are summed in FP64 in the epilogue.
 //
 // Operand fragments (wave64, 16 x 16 x 32): lane l holds 8 consecutive k of
@@ -135,8 +135,27 @@ __device__ inline f4 mfma16(uint4 a, uint4 b, f4 c) {
     }
 }
 
+typedef float f16acc __attribute__((ext_vector_type(16)));
+
+// v_mfma_f32_32x32x16_{bf16,f16}: a / b as for the 16x16 form but with 32 rows
+// (columns) per operand (lane % 32) and k-group lane / 32; the result d[r] is
+// D[row32(lane, r)][lane % 32].  On gfx950 this form issues at twice the rate
+// of the 16x16x32 one (measured by mgw_hip_measure_peaks).
+template <Split16 F>
+__device__ inline f16acc mfma32(uint4 a, uint4 b, f16acc c) {
+    if constexpr (F == Split16::BF16) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                      __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+    }
+}
+__device__ inline int row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
 // Workgroup tile: rows [r0, r0+256) x columns [c0, c0+256), 512 threads = 8
-// waves in 4 (rows) x 2 (columns), each wave 64 x 128 (4 x 8 MFMA blocks).
+// waves in 4 (rows) x 2 (columns), each wave 64 x 128 (2 x 4 blocks of the
+// 32x32x16 MFMA).
 // Per K step of 32 the four operand planes (M hi/lo rows, X hi/lo columns,
 // 16 B per thread-chunk, 8 chunks per thread) are loaded into registers one
 // step ahead and stored to LDS (row stride 40 halves: 80 B keeps the 16 B
@@ -152,14 +171,12 @@ __global__ __launch_bounds__(512) void k_contract_split(
     const uint16_t* __restrict__ xlo, double* __restrict__ qpart, int np, int mpad) {
     __shared__ __attribute__((aligned(16))) uint16_t lds[4][kSBM][kSLd];
     const int w = blockIdx.z;
-    const int ncol = gridDim.x, nrow = gridDim.y, T = ncol * nrow;
+    const int ncol = gridDim.x, nrow = gridDim.y;
     int lin = blockIdx.x + ncol * blockIdx.y;
-    if (T % 8 == 0) lin = (lin % 8) * (T / 8) + lin / 8;     // XCD-contiguous tiles
     const int rt = nrow - 1 - lin / ncol, ct = lin % ncol;
     const int r0 = rt * kSBM, c0 = ct * kSBN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
-    const int li = lane & 15, lk = lane >> 4;
 
     // loader: chunk u of thread t -> plane u / 2, row (t + 512 (u % 2)) / 4, k (idx % 4) * 8
     const uint16_t* src[8];
@@ -178,12 +195,15 @@ __global__ __launch_bounds__(512) void k_contract_split(
 #pragma unroll
     for (int u = 0; u < 8; ++u) pre[u] = *reinterpret_cast<const uint4*>(src[u]);
 
-    f4 acc[4][8];
+    f16acc acc[2][4];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 8; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[x][y][r] = 0.f;
 
+    const int l32 = lane & 31, kg = lane >> 5;
     const int kend = min(np, r0 + kSBM);
     for (int k0 = 0; k0 < kend; k0 += kSBK) {
         __syncthreads();
@@ -196,47 +216,50 @@ __global__ __launch_bounds__(512) void k_contract_split(
             for (int u = 0; u < 8; ++u)
                 pre[u] = *reinterpret_cast<const uint4*>(src[u] + (size_t)((k0 + kSBK) >> 5) * 8192);
         }
-        uint4 ah[4], al[4];
 #pragma unroll
-        for (int x = 0; x < 4; ++x) {
-            ah[x] = *reinterpret_cast<const uint4*>(&lds[0][wm + x * 16 + li][8 * lk]);
-            al[x] = *reinterpret_cast<const uint4*>(&lds[1][wm + x * 16 + li][8 * lk]);
-        }
+        for (int kc = 0; kc < kSBK; kc += 16) {
+            uint4 ah[2], al[2];
 #pragma unroll
-        for (int y = 0; y < 8; ++y) {
-            const uint4 bh = *reinterpret_cast<const uint4*>(&lds[2][wn + y * 16 + li][8 * lk]);
-            const uint4 bl = *reinterpret_cast<const uint4*>(&lds[3][wn + y * 16 + li][8 * lk]);
+            for (int x = 0; x < 2; ++x) {
+                ah[x] = *reinterpret_cast<const uint4*>(&lds[0][wm + x * 32 + l32][kc + 8 * kg]);
+                al[x] = *reinterpret_cast<const uint4*>(&lds[1][wm + x * 32 + l32][kc + 8 * kg]);
+            }
 #pragma unroll
-            for (int x = 0; x < 4; ++x) {
-                acc[x][y] = mfma16<F>(ah[x], bh, acc[x][y]);
-                acc[x][y] = mfma16<F>(ah[x], bl, acc[x][y]);
-                acc[x][y] = mfma16<F>(al[x], bh, acc[x][y]);
+            for (int y = 0; y < 4; ++y) {
+                const uint4 bh =
+                    *reinterpret_cast<const uint4*>(&lds[2][wn + y * 32 + l32][kc + 8 * kg]);
+                const uint4 bl =
+                    *reinterpret_cast<const uint4*>(&lds[3][wn + y * 32 + l32][kc + 8 * kg]);
+#pragma unroll
+                for (int x = 0; x < 2; ++x) acc[x][y] = mfma32<F>(ah[x], bh, acc[x][y]);
+#pragma unroll
+                for (int x = 0; x < 2; ++x) acc[x][y] = mfma32<F>(ah[x], bl, acc[x][y]);
+#pragma unroll
+                for (int x = 0; x < 2; ++x) acc[x][y] = mfma32<F>(al[x], bh, acc[x][y]);
             }
         }
     }
 
     // epilogue: undo the row scale, sum squares per column over the tile's rows
-    double inv_rs[4][4];
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            inv_rs[x][r] = 1.0 / mscale[(size_t)w * np + r0 + wm + x * 16 + 4 * lk + r];
     __syncthreads();
     double* red = reinterpret_cast<double*>(&lds[0][0][0]);      // [4][256] doubles
+    const double* ms = mscale + (size_t)w * np + r0 + wm;
+    double colsum[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int y = 0; y < 8; ++y) {
-        double sum = 0.0;
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int r = 0; r < 16; ++r) {
+            const double inv = 1.0 / ms[x * 32 + row32(lane, r)];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double z = (double)acc[x][y][r] * inv_rs[x][r];
-                sum += z * z;
+            for (int y = 0; y < 4; ++y) {
+                const double z = (double)acc[x][y][r] * inv;
+                colsum[y] += z * z;
             }
-        sum += __shfl_xor(sum, 16);
-        sum += __shfl_xor(sum, 32);
-        if (lk == 0) red[(wave >> 1) * kSBN + wn + y * 16 + li] = sum;
+        }
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        const double sum = colsum[y] + __shfl_xor(colsum[y], 32);
+        if (kg == 0) red[(wave >> 1) * kSBN + wn + y * 32 + l32] = sum;
     }
     __syncthreads();
     if (t < kSBN) {
@@ -254,6 +277,17 @@ __global__ __launch_bounds__(64) void k_probe16(const uint4* __restrict__ a,
     f4 acc = f4{0.f, 0.f, 0.f, 0.f};
     acc = mfma16<F>(a[t * 64 + lane], b[t * 64 + lane], acc);
     for (int r = 0; r < 4; ++r) d[((size_t)t * 64 + lane) * 4 + r] = acc[r];
+}
+
+template <Split16 F>
+__global__ __launch_bounds__(64) void k_probe32(const uint4* __restrict__ a,
+                                                const uint4* __restrict__ b,
+                                                float* __restrict__ d) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    f16acc acc;
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    acc = mfma32<F>(a[t * 64 + lane], b[t * 64 + lane], acc);
+    for (int r = 0; r < 16; ++r) d[((size_t)t * 64 + lane) * 16 + r] = acc[r];
 }
 
 __global__ void k_reduce_q_scaled(const double* __restrict__ qpart,
@@ -320,6 +354,17 @@ hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, flo
         hipLaunchKernelGGL(k_probe16<Split16::BF16>, dim3(ntests), dim3(64), 0, s, a4, b4, d);
     else
         hipLaunchKernelGGL(k_probe16<Split16::FP16>, dim3(ntests), dim3(64), 0, s, a4, b4, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe32(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
+                         int ntests, hipStream_t s) {
+    const uint4* a4 = reinterpret_cast<const uint4*>(a);
+    const uint4* b4 = reinterpret_cast<const uint4*>(b);
+    if (fmt == Split16::BF16)
+        hipLaunchKernelGGL(k_probe32<Split16::BF16>, dim3(ntests), dim3(64), 0, s, a4, b4, d);
+    else
+        hipLaunchKernelGGL(k_probe32<Split16::FP16>, dim3(ntests), dim3(64), 0, s, a4, b4, d);
     return hipGetLastError();
 }
 

@@ -78,6 +78,7 @@ def _load():
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
         "mgw_hip_selftest_tile": (C.c_int, [vp, _dp, _dp, _dp]),
         "mgw_hip_probe_mfma": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, vp]),
+        "mgw_hip_measure_peaks": (C.c_int, [vp, _dp]),
         "mgw_grid_transformed_legendre": (C.c_int, [C.c_int32, _dp, _dp]),
         "mgw_hip_estimate_bytes": (C.c_size_t, [C.c_int32] * 6),
     }
@@ -175,16 +176,24 @@ class HipContext:
         return dict(zip(keys, t.tolist()))
 
     def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
-        """Raw single-MFMA results: kind 0 f64 (a, b: [t, 64]); 1 bf16 / 2 f16 (a, b: [t, 64, 8] uint16)."""
+        """Raw single-MFMA results: kind 0 f64 (a, b: [t, 64]); 1 bf16 / 2 f16 16x16x32,
+        3 bf16 / 4 f16 32x32x16 (a, b: [t, 64, 8] uint16)."""
         if kind == 0:
             a, b = _f64(a), _f64(b)
             d = np.zeros((a.shape[0], 64, 4))
         else:
             a = np.ascontiguousarray(a, dtype=np.uint16)
             b = np.ascontiguousarray(b, dtype=np.uint16)
-            d = np.zeros((a.shape[0], 64, 4), dtype=np.float32)
+            d = np.zeros((a.shape[0], 64, 4 if kind <= 2 else 16), dtype=np.float32)
         self.call("mgw_hip_probe_mfma", kind, a.shape[0], a.ctypes.data, b.ctypes.data, d.ctypes.data)
         return d
+
+    def measure_peaks(self) -> dict:
+        """Measured FP64 / FP16 / BF16 MFMA flop/s and HBM copy bytes/s of this device."""
+        o = np.zeros(6)
+        self.call("mgw_hip_measure_peaks", _ptr(o))
+        return {"fp64_mfma": o[0], "fp16_mfma": o[1], "bf16_mfma": o[2], "hbm_copy": o[3],
+                "fp16_mfma_32x32": o[4], "bf16_mfma_32x32": o[5]}
 
     def selftest_tile(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         a, b = _f64(a), _f64(b)

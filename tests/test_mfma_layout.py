@@ -86,3 +86,31 @@ def test_16bit_layout(ctx, kind):
             # the paired B slot carries the same k: value 1 + e + 8 * (la // 16)
             vals = set(float(out[l, r]) for l, r in nz)
             assert vals == {1.0 + e + 8 * (la // 16)}, (kind, la, e, vals)
+
+
+@pytest.mark.parametrize("kind", [3, 4])
+def test_32x32_layout(ctx, kind):
+    """32x32x16: a[e] = A[lane % 32][k(lane / 32, e)], b likewise with columns,
+    d[r] = D[8 (r / 4) + 4 (lane / 32) + r % 4][lane % 32]."""
+    bits_kind = 1 if kind == 3 else 2
+    n = 64 * 8
+    a = np.zeros((n, 64, 8), dtype=np.float32)
+    for la in range(64):
+        for e in range(8):
+            a[la * 8 + e, la, e] = 1.0
+    bv = np.zeros((64, 8), dtype=np.float32)
+    for lb in range(64):
+        for e in range(8):
+            bv[lb, e] = 1 + e + 8 * (lb // 32)
+    b = np.broadcast_to(bv, (n, 64, 8))
+    d = ctx.probe_mfma(kind, _bits(a, bits_kind), _bits(b, bits_kind))
+    _dump(f"probe_32_{kind}.npz", d=d)
+    for la in range(64):
+        for e in range(8):
+            out = d[la * 8 + e]
+            nz = sorted(tuple(x) for x in np.argwhere(out != 0))
+            i = la % 32
+            want = sorted((j + 32 * ((i % 8) // 4), 4 * (i // 8) + i % 4) for j in range(32))
+            assert nz == want, (kind, la, e, nz[:4])
+            vals = set(float(out[l, r]) for l, r in nz)
+            assert vals == {1.0 + e + 8 * (la // 32)}, (kind, la, e, vals)
