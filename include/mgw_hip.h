@@ -107,6 +107,20 @@ int mgw_hip_quadratic_forms_host(mgw_hip_ctx* ctx, const double* x_rowmajor, int
  * DevicePqPhPanelView(..., FullEriResident) (src/workspace/device_pq_ph_panel.cu:204-252). */
 int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t nocc);
 
+/* Build the resident ERI on the device from its factor B (nmo^2 x kdim,
+ * row-major, symmetric in (p,q)): eri[pq][rs] = sum_K B[pq][K] B[rs][K],
+ * formed with FP64 MFMA and exactly symmetric in (pq, rs).  Saves the host
+ * product and the nmo^4 upload for factor-form inputs (density fitting /
+ * Cholesky vectors, the seeded synthetic molecules). */
+int mgw_hip_upload_eri_factor(mgw_hip_ctx* ctx, const double* b, int32_t nmo, int32_t nocc,
+                              int32_t kdim);
+
+/* Copy the resident ERI (nmo^4, C order) back to the host. */
+int mgw_hip_download_eri(mgw_hip_ctx* ctx, double* eri);
+
+/* Free every device buffer the context holds (the context stays usable). */
+int mgw_hip_release_buffers(mgw_hip_ctx* ctx);
+
 /* Build the resident V_ph from the resident ERI on the device; replaces the
  * host gather calculate_v_ph_matrix (src/gw/sigma.cpp:56-71) + upload. */
 int mgw_hip_build_vph_from_eri(mgw_hip_ctx* ctx);
@@ -146,6 +160,48 @@ typedef struct mgw_g0w0_out {
 
 /* One molecule, all frequencies, on the ERI made resident by mgw_hip_upload_eri. */
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
+
+/* ---- request loop (continuous batching; no counterpart in the reference,
+ *      which runs one molecule per process) */
+
+typedef struct mgw_hip_server mgw_hip_server;
+
+typedef struct mgw_serve_job {
+    mgw_g0w0_job job;           /* sizes, energies, states, precision         */
+    const double* eri;          /* host ERI (nmo^4), or NULL                  */
+    const double* eri_factor;   /* host factor (nmo^2 x kdim), or NULL;       */
+    int32_t kdim;               /* exactly one of eri / eri_factor is given   */
+} mgw_serve_job;
+
+/* Start `workers` threads on `device`, each with its own context and stream.
+ * Jobs are admitted in arrival order while the bytes reserved by running jobs
+ * plus the next job's estimate fit hbm_budget; batch_freqs caps the
+ * frequencies factored at once per job. */
+int mgw_hip_server_create(int device, int32_t workers, size_t hbm_budget, int32_t batch_freqs,
+                          mgw_hip_server** out);
+
+/* Cancel jobs not yet started, wait for running ones, free everything. */
+int mgw_hip_server_destroy(mgw_hip_server* server);
+
+/* Queue a job.  mo_energy / vxc_diag / states are copied; the ERI or factor
+ * and the output buffers must stay valid until mgw_hip_server_wait reports the
+ * job finished. */
+int mgw_hip_server_submit(mgw_hip_server* server, const mgw_serve_job* job,
+                          const mgw_g0w0_out* out, int64_t* ticket);
+
+/* Wait for a job (timeout_s < 0: no limit).  *status = -1 if still pending or
+ * running at the timeout, else the job's mgw_status; latency_ms = submit to
+ * finish, run_ms = start to finish. */
+int mgw_hip_server_wait(mgw_hip_server* server, int64_t ticket, double timeout_s, int32_t* status,
+                        double* latency_ms, double* run_ms);
+
+/* Error message of a failed job ("" if none). */
+const char* mgw_hip_server_job_error(mgw_hip_server* server, int64_t ticket);
+
+/* Jobs finished so far, jobs running now, most jobs running at once, bytes
+ * reserved by running jobs. */
+int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* running,
+                         int32_t* peak_running, size_t* reserved);
 
 /* Phase times of the last mgw_hip_g0w0 call in milliseconds (HIP events on the
  * context's stream): [0] V build (+ X split), [1] dielectric build,

@@ -9,12 +9,12 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib", "libmgw_hip.so")
-SOURCES = ["mgw_kernels.hip", "mgw_split.hip", "mgw_peak.hip", "mgw_capi.hip"]
+SOURCES = ["mgw_kernels.hip", "mgw_split.hip", "mgw_peak.hip", "mgw_capi.hip", "mgw_serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-result", "-pthread",
     "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
 ]
 

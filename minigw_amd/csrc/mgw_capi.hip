@@ -476,29 +476,81 @@ int mgw_hip_quadratic_forms_host(mgw_hip_ctx* ctx, const double* x, int32_t n, i
 }
 
 // ---------------------------------------------------- DevicePqPhPanelView
+namespace {
+// ERI-resident bookkeeping shared by both upload paths: ph tables and the
+// (pk|ia) == (kp|ia) check.
+void finish_eri_upload(mgw_hip_ctx* ctx, int nmo, int nocc) {
+    const PhLayout L = ph_layout(nmo, nocc);
+    int* col = as<int>(ctx->get(ctx->colidx, (size_t)L.np * 4, "colidx"));
+    int64_t* rb = as<int64_t>(ctx->get(ctx->rowbase, (size_t)L.np * 8, "rowbase"));
+    ctx->h2d(col, L.colidx.data(), L.np);
+    ctx->h2d(rb, L.rowbase.data(), L.np);
+    int* dflag = as<int>(ctx->get(ctx->flag, sizeof(int), "flag"));
+    hip_check(hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
+    hip_check(mgw::launch_check_pk_symmetry(as<double>(ctx->eri.p), nmo, nocc, dflag, ctx->stream),
+              "symmetry check");
+    int asym = 1;
+    ctx->d2h(&asym, dflag, 1);
+    ctx->sync();
+    ctx->eri_pk_sym = (asym == 0);
+    ctx->nmo = nmo;
+    ctx->nocc = nocc;
+}
+}  // namespace
+
 int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t nocc) {
     return guarded(ctx, [&] {
         need(ctx && eri, "mgw_hip_upload_eri: NULL argument");
         need(nmo > 0, "MolecularIntegrals: ERI tensor must have shape [nmo,nmo,nmo,nmo]");
         need(nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
         const size_t count = (size_t)nmo * nmo * nmo * nmo;
-        const PhLayout L = ph_layout(nmo, nocc);
         ctx->nmo = 0;
         double* d = as<double>(ctx->get(ctx->eri, count * 8, "ERI"));
-        int* col = as<int>(ctx->get(ctx->colidx, (size_t)L.np * 4, "colidx"));
-        int64_t* rb = as<int64_t>(ctx->get(ctx->rowbase, (size_t)L.np * 8, "rowbase"));
         ctx->h2d(d, eri, count);
-        ctx->h2d(col, L.colidx.data(), L.np);
-        ctx->h2d(rb, L.rowbase.data(), L.np);
-        int* dflag = as<int>(ctx->get(ctx->flag, sizeof(int), "flag"));
-        hip_check(hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
-        hip_check(mgw::launch_check_pk_symmetry(d, nmo, nocc, dflag, ctx->stream), "symmetry check");
-        int asym = 1;
-        ctx->d2h(&asym, dflag, 1);
+        finish_eri_upload(ctx, nmo, nocc);
+    });
+}
+
+int mgw_hip_upload_eri_factor(mgw_hip_ctx* ctx, const double* b, int32_t nmo, int32_t nocc,
+                              int32_t kdim) {
+    return guarded(ctx, [&] {
+        need(ctx && b, "mgw_hip_upload_eri_factor: NULL argument");
+        need(nmo > 0 && kdim > 0, "mgw_hip_upload_eri_factor: nmo and kdim must be positive");
+        need(nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
+        const int N = nmo * nmo;
+        const int Np = mgw::round_up(N, 64), Kp = mgw::round_up(kdim, 16);
+        std::vector<double> bp((size_t)Np * Kp, 0.0);
+        for (int r = 0; r < N; ++r)
+            std::copy(b + (size_t)r * kdim, b + (size_t)(r + 1) * kdim, bp.begin() + (size_t)r * Kp);
+        ctx->nmo = 0;
+        double* d = as<double>(ctx->get(ctx->eri, (size_t)N * N * 8, "ERI"));
+        double* db = as<double>(ctx->get(ctx->wsrc, bp.size() * 8, "ERI factor"));
+        ctx->h2d(db, bp.data(), bp.size());
+        hip_check(mgw::launch_eri_from_factor(db, N, Kp, d, ctx->stream), "ERI from factor");
         ctx->sync();
-        ctx->eri_pk_sym = (asym == 0);
-        ctx->nmo = nmo;
-        ctx->nocc = nocc;
+        ctx->release(ctx->wsrc);
+        finish_eri_upload(ctx, nmo, nocc);
+    });
+}
+
+int mgw_hip_download_eri(mgw_hip_ctx* ctx, double* eri) {
+    return guarded(ctx, [&] {
+        need(ctx && eri, "mgw_hip_download_eri: NULL argument");
+        if (!ctx->nmo) fail(MGW_ERR_STATE, "mgw_hip_download_eri: no resident ERI");
+        const size_t count = (size_t)ctx->nmo * ctx->nmo * ctx->nmo * ctx->nmo;
+        ctx->d2h(eri, ctx->eri.p, count);
+        ctx->sync();
+    });
+}
+
+int mgw_hip_release_buffers(mgw_hip_ctx* ctx) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_release_buffers: ctx is NULL");
+        ctx->sync();
+        for (auto* b : ctx->all) ctx->release(*b);
+        ctx->nmo = ctx->nocc = 0;
+        ctx->n = ctx->np = 0;
+        ctx->v_ready = ctx->wc_ready = false;
     });
 }
 

@@ -23,6 +23,10 @@ hipError_t launch_tile_selftest(const double* a, const double* b, double* c, hip
 hipError_t launch_probe_f64(const double* a, const double* b, double* d, int ntests,
                            hipStream_t s);
 
+// eri[P][R] = sum_K B[P][K] B[R][K] (P, R < N) from B padded to ((N+63)/64*64) x Kp,
+// Kp % 16 == 0, extra rows and columns zero; exactly symmetric in (P, R).
+hipError_t launch_eri_from_factor(const double* b, int N, int Kp, double* eri, hipStream_t s);
+
 // V[r][c] = eri[rowbase[r] + colidx[c]] for r, c < n; 0 in the padding (np x np).
 hipError_t launch_build_v(const double* eri, const int64_t* rowbase, const int* colidx,
                           double* v, int n, int np, hipStream_t s);

@@ -590,6 +590,38 @@ __global__ void k_pade_qp(const double* __restrict__ sre, const double* __restri
     iters[state] = used;
 }
 
+// ERI from its factor: eri[P][R] = sum_K B[P][K] B[R][K] for P, R < N = nmo^2,
+// B padded to Np x Kp (rows and K zero filled, Np % 64 == 0, Kp % 16 == 0).
+// Only tiles with I <= J are computed; each writes its block and the mirror
+// block, so eri[P][R] == eri[R][P] bit for bit.
+__global__ __launch_bounds__(256) void k_eri_from_factor(const double* __restrict__ b, int N,
+                                                         int Kp, double* __restrict__ eri) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    int I, J;
+    {
+        // upper-triangular tile index t -> (I, J), I <= J, row-major over I
+        const int nt = (N + 63) / 64;
+        int t = blockIdx.x, i = 0;
+        while (t >= nt - i) {
+            t -= nt - i;
+            ++i;
+        }
+        I = i;
+        J = i + t;
+    }
+    Tile64 tile;
+    tile.zero();
+    gemm64<false>(b + (size_t)I * 64 * Kp, Kp, b + (size_t)J * 64 * Kp, Kp, 0, Kp, tile, As, Bs);
+    tile.for_each([&](int r, int c, double v) {
+        const int P = I * 64 + r, R = J * 64 + c;
+        if (P < N && R < N) {
+            eri[(size_t)P * N + R] = v;
+            eri[(size_t)R * N + P] = v;
+        }
+    });
+}
+
 // Diagnostic: C = A B^T for 64 x 64 row-major A, B through the Tile64 path.
 __global__ __launch_bounds__(256) void k_tile_selftest(const double* __restrict__ a,
                                                        const double* __restrict__ b,
@@ -631,6 +663,12 @@ inline int grid_for(size_t total, int block = 256, int cap = 65535 * 8) {
 hipError_t launch_probe_f64(const double* a, const double* b, double* d, int ntests,
                            hipStream_t s) {
     hipLaunchKernelGGL(k_probe_f64, dim3(ntests), dim3(64), 0, s, a, b, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_eri_from_factor(const double* b, int N, int Kp, double* eri, hipStream_t s) {
+    const int nt = (N + 63) / 64;
+    hipLaunchKernelGGL(k_eri_from_factor, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, b, N, Kp, eri);
     return hipGetLastError();
 }
 

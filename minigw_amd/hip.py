@@ -53,6 +53,10 @@ class G0W0Out(C.Structure):
                 ("sigma_c_im", _dp), ("qp", _dp), ("qp_iterations", _ip)]
 
 
+class ServeJob(C.Structure):
+    _fields_ = [("job", G0W0Job), ("eri", _dp), ("eri_factor", _dp), ("kdim", C.c_int32)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"HIP library not built: {LIB_PATH} (run minigw_amd/build.py)")
@@ -72,6 +76,9 @@ def _load():
         "mgw_hip_download_wc": (C.c_int, [vp, _dp, _dp, C.c_int32]),
         "mgw_hip_quadratic_forms_host": (C.c_int, [vp, _dp, C.c_int32, C.c_int32, _dp, _dp]),
         "mgw_hip_upload_eri": (C.c_int, [vp, _dp, C.c_int32, C.c_int32]),
+        "mgw_hip_upload_eri_factor": (C.c_int, [vp, _dp, C.c_int32, C.c_int32, C.c_int32]),
+        "mgw_hip_download_eri": (C.c_int, [vp, _dp]),
+        "mgw_hip_release_buffers": (C.c_int, [vp]),
         "mgw_hip_build_vph_from_eri": (C.c_int, [vp]),
         "mgw_hip_quadratic_forms": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
@@ -81,6 +88,15 @@ def _load():
         "mgw_hip_measure_peaks": (C.c_int, [vp, _dp]),
         "mgw_grid_transformed_legendre": (C.c_int, [C.c_int32, _dp, _dp]),
         "mgw_hip_estimate_bytes": (C.c_size_t, [C.c_int32] * 6),
+        "mgw_hip_server_create": (C.c_int, [C.c_int, C.c_int32, C.c_size_t, C.c_int32, C.POINTER(vp)]),
+        "mgw_hip_server_destroy": (C.c_int, [vp]),
+        "mgw_hip_server_submit": (C.c_int, [vp, C.POINTER(ServeJob), C.POINTER(G0W0Out),
+                                            C.POINTER(C.c_int64)]),
+        "mgw_hip_server_wait": (C.c_int, [vp, C.c_int64, C.c_double, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "mgw_hip_server_job_error": (C.c_char_p, [vp, C.c_int64]),
+        "mgw_hip_server_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -167,6 +183,20 @@ class HipContext:
         nmo = e.shape[0]
         assert e.shape == (nmo, nmo, nmo, nmo)
         self.call("mgw_hip_upload_eri", _ptr(e), nmo, nocc)
+
+    def upload_eri_factor(self, b: np.ndarray, nmo: int, nocc: int):
+        """ERI built on the device from B (nmo^2 x K): eri[pq][rs] = sum_K B[pq,K] B[rs,K]."""
+        bb = _f64(b)
+        assert bb.shape[0] == nmo * nmo
+        self.call("mgw_hip_upload_eri_factor", _ptr(bb), nmo, nocc, bb.shape[1])
+
+    def download_eri(self, nmo: int) -> np.ndarray:
+        out = np.zeros((nmo, nmo, nmo, nmo))
+        self.call("mgw_hip_download_eri", _ptr(out))
+        return out
+
+    def release_buffers(self):
+        self.call("mgw_hip_release_buffers")
 
     def last_timings(self) -> dict:
         t = np.zeros(12)
@@ -295,3 +325,80 @@ def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.
     ctx.call("mgw_hip_g0w0", C.byref(job), C.byref(out))
     return HipG0W0Result(om, w, list(range(nmo)) if st is None else st.tolist(), sx,
                          sre + 1j * sim, qp, it, ctx.last_timings())
+
+
+class HipServer:
+    """Continuous-batching request loop (worker threads, one stream each,
+    HBM-budget admission) over mgw_hip_server_*."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30,
+                 batch_freqs: int = 32):
+        h = C.c_void_p()
+        _check("mgw_hip_server_create",
+               lib.mgw_hip_server_create(device, workers, hbm_budget, batch_freqs, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib.mgw_hip_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, inp=None, *, factor=None, mo_energy=None, vxc=None, nocc=None, fermi=None,
+               nfreq: int = 200, npar: int = 16, states=None, precision: int = PREC_FP64) -> int:
+        """Queue one molecule: either `inp` (GwInput with .eri) or its `factor`
+        B (nmo^2 x K) with mo_energy, vxc, nocc, fermi."""
+        if inp is not None:
+            mo, vx, nocc, fermi = inp.mo_energy, inp.vxc, inp.nocc, inp.fermi_energy
+            eri, fac, kdim = _f64(inp.eri), None, 0
+        else:
+            mo, vx = mo_energy, vxc
+            eri, fac = None, _f64(factor)
+            kdim = fac.shape[1]
+        mo = _f64(mo)
+        nmo = mo.shape[0]
+        vxd = _f64(np.diag(vx))
+        st = None if states is None else np.ascontiguousarray(states, dtype=np.int32)
+        ns = nmo if st is None else len(st)
+        res = {"omega": np.zeros(nfreq), "weight": np.zeros(nfreq), "sx": np.zeros(nmo),
+               "sre": np.zeros((ns, nfreq)), "sim": np.zeros((ns, nfreq)), "qp": np.zeros(nmo),
+               "it": np.zeros(nmo, dtype=np.int32)}
+        job = G0W0Job(nmo, int(nocc), nfreq, npar, 0 if st is None else ns, _ptr(st, _ip),
+                      _ptr(mo), _ptr(vxd), float(fermi), 0.0, precision)
+        sj = ServeJob(job, _ptr(eri), _ptr(fac), kdim)
+        out = G0W0Out(_ptr(res["omega"]), _ptr(res["weight"]), _ptr(res["sx"]), _ptr(res["sre"]),
+                      _ptr(res["sim"]), _ptr(res["qp"]), _ptr(res["it"], _ip))
+        t = C.c_int64(0)
+        _check("mgw_hip_server_submit", lib.mgw_hip_server_submit(self.h, C.byref(sj), C.byref(out), C.byref(t)))
+        self._keep[t.value] = (res, eri, fac, st, nmo)
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0):
+        """Returns (HipG0W0Result, latency_ms, run_ms); raises HipError on failure."""
+        st, lat, run = C.c_int32(0), C.c_double(0), C.c_double(0)
+        _check("mgw_hip_server_wait", lib.mgw_hip_server_wait(self.h, ticket, timeout, C.byref(st),
+                                                               C.byref(lat), C.byref(run)))
+        if st.value == -1:
+            raise TimeoutError(f"job {ticket} not finished")
+        res, _, _, stt, nmo = self._keep.pop(ticket)
+        if st.value != MGW_OK:
+            msg = lib.mgw_hip_server_job_error(self.h, ticket)
+            raise HipError("mgw_hip_server_wait", st.value, msg.decode() if msg else "")
+        out = HipG0W0Result(res["omega"], res["weight"],
+                            list(range(nmo)) if stt is None else stt.tolist(), res["sx"],
+                            res["sre"] + 1j * res["sim"], res["qp"], res["it"], {})
+        return out, lat.value, run.value
+
+    def stats(self) -> dict:
+        c, r, p, b = C.c_int64(0), C.c_int32(0), C.c_int32(0), C.c_size_t(0)
+        _check("mgw_hip_server_stats", lib.mgw_hip_server_stats(self.h, C.byref(c), C.byref(r),
+                                                                 C.byref(p), C.byref(b)))
+        return {"completed": c.value, "running": r.value, "peak_running": p.value,
+                "reserved_bytes": b.value}

@@ -1,0 +1,88 @@
+"""Request loop and device-side ERI construction (needs an MI355X)."""
+import numpy as np
+import pytest
+
+from minigw_amd.synthetic import GwInput, eri_from_factor, synthetic_factor, synthetic_input
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from minigw_amd import hip as h
+    return h
+
+
+def test_eri_from_factor_on_device(hip):
+    say("case: ERI built on the device from its factor, nmo=14")
+    nmo, nocc = 14, 4
+    mo, fermi, b, vxc = synthetic_factor(nmo, nocc, 77)
+    host = eri_from_factor(b, nmo)
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri_factor(b, nmo, nocc)
+        dev = ctx.download_eri(nmo)
+        assert np.abs(dev - host).max() <= 1e-14 * np.abs(host).max()
+        for perm in [(1, 0, 2, 3), (2, 3, 0, 1), (0, 1, 3, 2)]:
+            assert np.array_equal(dev, dev.transpose(perm))
+        inp = GwInput(mo_energy=mo, eri=host, vxc=vxc, nocc=nocc, fermi_energy=fermi)
+        res = hip.run_g0w0(inp, 32, 8, ctx=ctx, upload=False)
+        assert res.timings["contract_columns"] == nmo * (nmo + 1) // 2   # symmetric path taken
+    want = orc.run_g0w0(inp, 32, 8)
+    assert np.abs(res.sigma_c - want.sigma_c).max() < 1e-11 * np.abs(want.sigma_c).max()
+
+
+def _jobs():
+    out = []
+    for i, (nmo, nocc) in enumerate([(12, 3), (20, 4), (28, 6), (16, 4), (36, 7), (24, 5)]):
+        mo, fermi, b, vxc = synthetic_factor(nmo, nocc, 1000 + i)
+        out.append(dict(nmo=nmo, nocc=nocc, mo=mo, fermi=fermi, b=b, vxc=vxc,
+                        precision=i % 2))
+    return out
+
+
+def test_serve_alone_equals_mix(hip):
+    """Each job's result in a mix of concurrent jobs equals the same job alone."""
+    say("case: 6 jobs (nmo 12..36, FP64 and mixed) on 4 workers vs alone")
+    jobs = _jobs()
+    with hip.HipServer(0, workers=4, hbm_budget=64 << 30, batch_freqs=16) as srv:
+        tickets = [srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"],
+                              fermi=j["fermi"], nfreq=40, npar=12, precision=j["precision"])
+                   for j in jobs]
+        mixed = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    say(f"   server stats {st}")
+    assert st["completed"] == len(jobs) and st["running"] == 0 and st["peak_running"] >= 2
+    with hip.HipContext(0) as ctx:
+        for j, (res, lat, run) in zip(jobs, mixed):
+            ctx.upload_eri_factor(j["b"], j["nmo"], j["nocc"])
+            inp = GwInput(mo_energy=j["mo"], eri=None, vxc=j["vxc"], nocc=j["nocc"],
+                          fermi_energy=j["fermi"])
+            alone = hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False, precision=j["precision"])
+            assert np.array_equal(alone.sigma_c, res.sigma_c), j["nmo"]
+            assert np.array_equal(alone.qp, res.qp, equal_nan=True)
+            assert np.array_equal(alone.qp_iterations, res.qp_iterations)
+            assert lat >= run > 0
+
+
+def test_serve_admission_and_rejection(hip):
+    say("case: admission by HBM estimate")
+    jobs = _jobs()[:3]
+    est = max(hip.estimate_bytes(j["nmo"], j["nocc"], 40, 0, 16, j["precision"]) for j in jobs)
+    budget = int(est * 1.5)
+    with hip.HipServer(0, workers=3, hbm_budget=budget, batch_freqs=16) as srv:
+        tickets = [srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"],
+                              fermi=j["fermi"], nfreq=40, npar=12, precision=j["precision"])
+                   for j in jobs]
+        for t in tickets:
+            srv.wait(t)
+        assert srv.stats()["peak_running"] >= 1
+        big = synthetic_input(40, 8, 5)
+        t = srv.submit(big, nfreq=40, npar=12)
+        with pytest.raises(hip.HipError) as e:
+            srv.wait(t)
+        assert e.value.code == hip.MGW_ERR_MEMORY
