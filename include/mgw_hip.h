@@ -208,9 +208,16 @@ int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* ru
  * [2] factorization (potrf + trtri), [3] contraction kernel, [4] Sigma_c
  * quadrature, [5] Pade + QP, [6] total, [7] frequencies per batch,
  * [8] contraction launches, [9] M split + q reduction, [10] contraction
- * columns (unordered (p,k) pairs when the ERI is exactly pk-symmetric).  n is the number of
+ * columns (unordered (p,k) pairs when the ERI is exactly pk-symmetric), [11] time
+ * in replayed batch graphs (phases 1-3 and 9 are then 0).  n is the number of
  * entries the caller provides (<= 12). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
+
+/* Run each frequency batch of mgw_hip_g0w0 as a captured hipGraph, replayed
+ * for later batches and molecules of the same shape on the same buffers
+ * (enable = 1), or launch kernels directly (0, the default, which also gives
+ * the per-phase timings). */
+int mgw_hip_set_graphs(mgw_hip_ctx* ctx, int32_t enable);
 
 /* Diagnostic with no counterpart in the reference: C = A B^T for 64 x 64
  * row-major A, B through the factorization kernels' FP64 MFMA tile (checks

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -43,6 +44,15 @@ void hip_check(hipError_t e, const char* what) {
 }  // namespace
 
 // ------------------------------------------------------------------ context
+struct GraphKey {
+    int np, mpad, m, b, precision;
+    std::array<const void*, 19> ptr;
+    bool operator==(const GraphKey& o) const {
+        return np == o.np && mpad == o.mpad && m == o.m && b == o.b && precision == o.precision &&
+               ptr == o.ptr;
+    }
+};
+
 struct mgw_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -78,6 +88,7 @@ struct mgw_hip_ctx {
         return p;
     }
     void release(Buf& b) {
+        if (b.p && !graphs.empty()) drop_graphs();   // a freed address may be handed out again
         if (b.p) {
             (void)hipFree(b.p);
             held -= b.bytes;
@@ -90,7 +101,24 @@ struct mgw_hip_ctx {
     Buf eri, v, colidx, rowbase, delta;
     int nmo = 0, nocc = 0;        // of the resident ERI (0 = none)
     bool eri_pk_sym = false;      // (pk|ia) == (kp|ia) exactly for the resident ERI
-    Buf flag, hcolmap, wxp, hxp;
+    Buf flag, hcolmap, wxp, hxp, f0buf;
+
+    // captured frequency-batch graphs (hipGraph), keyed by shape and buffers
+    bool use_graphs = false;
+    struct GraphEntry {
+        GraphKey key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    hipGraphExec_t find_graph(const GraphKey& k) const {
+        for (const auto& g : graphs)
+            if (g.key == k) return g.exec;
+        return nullptr;
+    }
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
     int n = 0, np = 0;            // ph dimension of the resident V
     bool v_ready = false;
     // single-frequency workspace (compute_wc / quadratic forms)
@@ -102,13 +130,14 @@ struct mgw_hip_ctx {
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
 
     mgw_hip_ctx() {
-        all = {&flag, &hcolmap, &wxp, &hxp, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
+        all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
+        drop_graphs();
         for (Buf* b : all) release(*b);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -334,6 +363,14 @@ int mgw_hip_set_memory_budget(mgw_hip_ctx* ctx, size_t bytes) {
     });
 }
 
+int mgw_hip_set_graphs(mgw_hip_ctx* ctx, int32_t enable) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_graphs: ctx is NULL");
+        ctx->use_graphs = enable != 0;
+        if (!ctx->use_graphs) ctx->drop_graphs();
+    });
+}
+
 int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight) {
     return guarded(nullptr, [&] {
         need(nfreq > 0, "generate_transformed_legendre_grid: num_freq_points must be positive");
@@ -444,7 +481,9 @@ void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow,
     hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), xp, qpart, np,
                                        mpad, 1, ctx->stream),
               "contract");
-    hip_check(mgw::launch_reduce_q(qpart, nullptr, q, np / mgw::kCT, m, mpad, 0, 1, ctx->stream),
+    int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
+    hip_check(mgw::launch_set_int(f0p, 0, ctx->stream), "set offset");
+    hip_check(mgw::launch_reduce_q(qpart, nullptr, q, np / mgw::kCT, m, mpad, f0p, 1, ctx->stream),
               "reduce q");
     ctx->d2h(q_re, q, m);
     ctx->sync();
@@ -715,6 +754,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         double* dinv = as<double>(ctx->get(ctx->hdinv, (size_t)nb * nblk * 64 * 64 * 8, "dinv"));
         double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * np * 64 * 8, "trtri scratch"));
         int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
+        int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
         double* qpart = as<double>(
             ctx->get(ctx->hqpart, (size_t)nb * (np / mgw::kCT) * mpad * 8, "qpart"));
         uint16_t *mh = nullptr, *ml = nullptr;
@@ -726,34 +766,67 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         }
         hip_check(hipMemsetAsync(info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
 
-        float t_diel = 0, t_fact = 0, t_contr = 0, t_aux = 0;
+        float t_diel = 0, t_fact = 0, t_contr = 0, t_aux = 0, t_graph = 0;
         int n_contr = 0;
-        for (int f0 = 0; f0 < nfreq; f0 += nb) {
-            const int b = std::min(nb, nfreq - f0);
-            hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-            hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0, b, st), "scale");
+        // One frequency batch: kernels only, so that it can also be captured.
+        auto enqueue_batch = [&](int b, bool timed) {
+            if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+            hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
             hip_check(mgw::launch_dielectric(dV, S, A, np, b, st), "dielectric");
-            hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-            factor_batch(ctx, A, dinv, info + f0, tbuf, np, b);
-            hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
+            if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+            factor_batch(ctx, A, dinv, info, tbuf, np, b);
+            if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
-            hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
+            if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b, st),
                           "contract split");
             else
                 hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");
-            hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
+            if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dcolmap, dq,
                                                       mgw::contract_split_row_tiles(np), m, mpad,
-                                                      f0, b, st),
+                                                      f0p, b, st),
                           "reduce q");
             else
-                hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0, b, st),
+                hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0p, b, st),
                           "reduce q");
-            hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
+            if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
+        };
+        for (int f0 = 0; f0 < nfreq; f0 += nb) {
+            const int b = std::min(nb, nfreq - f0);
+            hip_check(mgw::launch_set_int(f0p, f0, st), "set offset");
+            if (ctx->use_graphs) {
+                // Replay a graph of this batch when one with the same shape and
+                // buffers was captured before (the offset is read on the device).
+                const GraphKey key{np, mpad, m, b, job->precision,
+                                   {A, S, dinv, tbuf, info, qpart, dq, xp, xh, xl, xs, mh, ml, ms, dV,
+                                    ddelta, dom, dcolmap, f0p}};
+                hipGraphExec_t exec = ctx->find_graph(key);
+                if (!exec) {
+                    hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+                              "hipStreamBeginCapture");
+                    enqueue_batch(b, false);
+                    hipGraph_t graph = nullptr;
+                    hip_check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+                    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    hip_check(e, "hipGraphInstantiate");
+                    ctx->graphs.push_back({key, exec});
+                }
+                hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+                hip_check(hipGraphLaunch(exec, st), "hipGraphLaunch");
+                hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
+                hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
+                float a = 0;
+                hip_check(hipEventElapsedTime(&a, ev[2], ev[5]), "hipEventElapsedTime");
+                t_graph += a;
+                ++n_contr;
+                continue;
+            }
+            enqueue_batch(b, true);
             hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
             float a = 0, bb = 0, c = 0, d = 0, e = 0;
             hip_check(hipEventElapsedTime(&a, ev[2], ev[3]), "hipEventElapsedTime");
@@ -810,6 +883,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->timings[8] = n_contr;
         ctx->timings[9] = t_aux;
         ctx->timings[10] = mu;
+        ctx->timings[11] = t_graph;
     });
 }
 

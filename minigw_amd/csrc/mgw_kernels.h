@@ -34,9 +34,14 @@ hipError_t launch_build_v(const double* eri, const int64_t* rowbase, const int* 
 // Pad a host-provided n x n row-major matrix into np x np (zeros outside).
 hipError_t launch_pad_square(const double* src, double* dst, int n, int np, hipStream_t s);
 
-// S[w][r] = sqrt(4 D_r / (w_f^2 + D_r^2)), f = f0 + w, for r < n; 0 for r >= n.
+// S[w][r] = sqrt(4 D_r / (w_f^2 + D_r^2)), f = *f0p + w, for r < n; 0 for r >= n.
+// The frequency offset is read from device memory so that a captured batch
+// graph serves every batch.
 hipError_t launch_scale_from_grid(const double* delta, const double* omega, double* S, int n,
-                                  int np, int f0, int nb, hipStream_t s);
+                                  int np, const int* f0p, int nb, hipStream_t s);
+
+// *p = v (one thread); sets the device-side batch offset outside a graph.
+hipError_t launch_set_int(int* p, int v, hipStream_t s);
 
 // S[r] = sqrt(-pi0[r]) for r < n; 0 in the padding.
 hipError_t launch_scale_from_pi0(const double* pi0, double* S, int n, int np, hipStream_t s);
@@ -68,7 +73,7 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* x
 // nrt = row tiles of the contraction that wrote qpart (np / 128 for FP64,
 // contract_split_row_tiles(np) for the split form).
 hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
-                           int mpad, int f0, int nb, hipStream_t s);
+                           int mpad, const int* f0p, int nb, hipStream_t s);
 
 // flag = 1 if (pk|ia) != (kp|ia) for some p, k, i < nocc <= a (flag must start at 0).
 hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int* flag,
@@ -105,9 +110,9 @@ hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, flo
 hipError_t launch_probe32(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
                          int ntests, hipStream_t s);
 
-// q[(f0+w) * m + c] = -(sum_rt qpart[w][rt][cu]) / xscale[cu]^2, cu = colmap[c].
+// q[(*f0p + w) * m + c] = -(sum_rt qpart[w][rt][cu]) / xscale[cu]^2, cu = colmap[c].
 hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, const int* colmap,
-                                  double* q, int nrt, int m, int mpad, int f0, int nb,
+                                  double* q, int nrt, int m, int mpad, const int* f0p, int nb,
                                   hipStream_t s);
 int contract_split_row_tiles(int np);
 

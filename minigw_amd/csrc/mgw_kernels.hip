@@ -54,7 +54,9 @@ __global__ void k_pad_square(const double* __restrict__ src, double* __restrict_
 }
 
 __global__ void k_scale_grid(const double* __restrict__ delta, const double* __restrict__ omega,
-                             double* __restrict__ S, int n, int np, int f0, int nb) {
+                             double* __restrict__ S, int n, int np, const int* __restrict__ f0p,
+                             int nb) {
+    const int f0 = *f0p;
     const int total = np * nb;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
         const int w = e / np, r = e % np;
@@ -423,8 +425,9 @@ __global__ void k_pack_x(const double* __restrict__ src, const int64_t* __restri
 }
 
 __global__ void k_reduce_q(const double* __restrict__ qpart, const int* __restrict__ colmap,
-                           double* __restrict__ q, int nrt, int m, int mpad, int f0) {
-    const int w = blockIdx.y;
+                           double* __restrict__ q, int nrt, int m, int mpad,
+                           const int* __restrict__ f0p) {
+    const int w = blockIdx.y, f0 = *f0p;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x) {
         const int cu = colmap ? colmap[c] : c;
         double s = 0.0;
@@ -651,6 +654,8 @@ __global__ __launch_bounds__(64) void k_probe_f64(const double* __restrict__ a,
     for (int r = 0; r < 4; ++r) d[((size_t)t * 64 + lane) * 4 + r] = acc[r];
 }
 
+__global__ void k_set_int(int* __restrict__ p, int v) { *p = v; }
+
 inline int grid_for(size_t total, int block = 256, int cap = 65535 * 8) {
     size_t g = (total + block - 1) / block;
     if (g < 1) g = 1;
@@ -669,6 +674,11 @@ hipError_t launch_probe_f64(const double* a, const double* b, double* d, int nte
 hipError_t launch_eri_from_factor(const double* b, int N, int Kp, double* eri, hipStream_t s) {
     const int nt = (N + 63) / 64;
     hipLaunchKernelGGL(k_eri_from_factor, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, b, N, Kp, eri);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_int(int* p, int v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v);
     return hipGetLastError();
 }
 
@@ -691,9 +701,9 @@ hipError_t launch_pad_square(const double* src, double* dst, int n, int np, hipS
 }
 
 hipError_t launch_scale_from_grid(const double* delta, const double* omega, double* S, int n,
-                                  int np, int f0, int nb, hipStream_t s) {
+                                  int np, const int* f0p, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_scale_grid, dim3(grid_for((size_t)np * nb)), dim3(256), 0, s, delta,
-                       omega, S, n, np, f0, nb);
+                       omega, S, n, np, f0p, nb);
     return hipGetLastError();
 }
 
@@ -747,9 +757,9 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* x
 }
 
 hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
-                           int mpad, int f0, int nb, hipStream_t s) {
+                           int mpad, const int* f0p, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_q, dim3(grid_for((size_t)m, 256, 4096), nb), dim3(256), 0, s,
-                       qpart, colmap, q, nrt, m, mpad, f0);
+                       qpart, colmap, q, nrt, m, mpad, f0p);
     return hipGetLastError();
 }
 

@@ -64,6 +64,7 @@ struct mgw_hip_server {
     void work() {
         mgw_hip_ctx* ctx = nullptr;
         const int rc = mgw_hip_create(device, &ctx);
+        if (rc == MGW_OK) mgw_hip_set_graphs(ctx, 1);   // batches of a job replay one graph
         for (;;) {
             ServeJob* j = nullptr;
             {

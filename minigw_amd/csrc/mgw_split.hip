@@ -292,8 +292,9 @@ __global__ __launch_bounds__(64) void k_probe32(const uint4* __restrict__ a,
 
 __global__ void k_reduce_q_scaled(const double* __restrict__ qpart,
                                   const double* __restrict__ xscale, const int* __restrict__ colmap,
-                                  double* __restrict__ q, int nrt, int m, int mpad, int f0) {
-    const int w = blockIdx.y;
+                                  double* __restrict__ q, int nrt, int m, int mpad,
+                                  const int* __restrict__ f0p) {
+    const int w = blockIdx.y, f0 = *f0p;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x) {
         const int cu = colmap ? colmap[c] : c;
         double s = 0.0;
@@ -369,11 +370,11 @@ hipError_t launch_probe32(Split16 fmt, const uint16_t* a, const uint16_t* b, flo
 }
 
 hipError_t launch_reduce_q_scaled(const double* qpart, const double* xscale, const int* colmap,
-                                  double* q, int nrt, int m, int mpad, int f0, int nb,
+                                  double* q, int nrt, int m, int mpad, const int* f0p, int nb,
                                   hipStream_t s) {
     const int gx = (int)((m + 255) / 256 < 4096 ? (m + 255) / 256 : 4096);
     hipLaunchKernelGGL(k_reduce_q_scaled, dim3(gx > 0 ? gx : 1, nb), dim3(256), 0, s, qpart, xscale,
-                       colmap, q, nrt, m, mpad, f0);
+                       colmap, q, nrt, m, mpad, f0p);
     return hipGetLastError();
 }
 

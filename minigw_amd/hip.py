@@ -83,6 +83,7 @@ def _load():
         "mgw_hip_quadratic_forms": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
+        "mgw_hip_set_graphs": (C.c_int, [vp, C.c_int32]),
         "mgw_hip_selftest_tile": (C.c_int, [vp, _dp, _dp, _dp]),
         "mgw_hip_probe_mfma": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, vp]),
         "mgw_hip_measure_peaks": (C.c_int, [vp, _dp]),
@@ -195,6 +196,10 @@ class HipContext:
         self.call("mgw_hip_download_eri", _ptr(out))
         return out
 
+    def set_graphs(self, enable: bool):
+        """Capture/replay each frequency batch as a hipGraph (no per-phase timings then)."""
+        self.call("mgw_hip_set_graphs", 1 if enable else 0)
+
     def release_buffers(self):
         self.call("mgw_hip_release_buffers")
 
@@ -202,7 +207,8 @@ class HipContext:
         t = np.zeros(12)
         self.call("mgw_hip_last_timings", _ptr(t), 12)
         keys = ["build_v", "dielectric", "factorize", "contract", "sigma_c", "pade_qp", "total",
-                "freq_batch", "contract_launches", "split_reduce", "contract_columns"]
+                "freq_batch", "contract_launches", "split_reduce", "contract_columns",
+                "graph_batches"]
         return dict(zip(keys, t.tolist()))
 
     def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:

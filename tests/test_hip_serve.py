@@ -86,3 +86,27 @@ def test_serve_admission_and_rejection(hip):
         with pytest.raises(hip.HipError) as e:
             srv.wait(t)
         assert e.value.code == hip.MGW_ERR_MEMORY
+
+
+def test_graph_replays_match_eager(hip):
+    """Frequency batches captured as hipGraphs and replayed (within a molecule
+    and across repeated molecules, with eager work on another context in
+    between) give exactly the eager results."""
+    say("case: hipGraph replay vs eager, nmo=30, 150 frequencies (3 batches)")
+    inp = synthetic_input(30, 6, 41)
+    other = synthetic_input(18, 4, 42)
+    with hip.HipContext(0) as ctx, hip.HipContext(0) as side:
+        ctx.upload_eri(inp.eri, inp.nocc)
+        side.upload_eri(other.eri, other.nocc)
+        for prec in (hip.PREC_FP64, hip.PREC_MIXED):
+            ctx.set_graphs(False)
+            eager = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
+            side_ref = hip.run_g0w0(other, 40, 8, ctx=side, upload=False, precision=prec)
+            ctx.set_graphs(True)
+            for rep in range(3):
+                res = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
+                assert res.timings["graph_batches"] > 0
+                assert np.array_equal(res.sigma_c, eager.sigma_c), (prec, rep)
+                assert np.array_equal(res.qp, eager.qp, equal_nan=True)
+                s2 = hip.run_g0w0(other, 40, 8, ctx=side, upload=False, precision=prec)
+                assert np.array_equal(s2.sigma_c, side_ref.sigma_c)
