@@ -59,6 +59,73 @@ def cpu_baseline(inp, nfreq: int, sample_freqs: int) -> dict:
                       f"Sigma_c/Pade/QP excluded (negligible)"}
 
 
+def _factor_job(nmo: int, nocc: int, seed: int):
+    from minigw_amd.synthetic import synthetic_factor
+    mo, fermi, b, vxc = synthetic_factor(nmo, nocc, seed)
+    return dict(nmo=nmo, nocc=nocc, mo=mo, fermi=fermi, b=b, vxc=vxc)
+
+
+def run_batch(hip, count: int, workers: int, nfreq: int, npar: int, prec: int) -> dict:
+    """Uneven-nmo batch: `count` molecules (nmo log-uniform in [24, 120],
+    nocc = round(nmo / 5)), all submitted at once to the request loop."""
+    from minigw_amd.synthetic import batch_molecules
+    mols = [_factor_job(*m) for m in batch_molecules(count)]
+    with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=64) as srv:
+        # warm-up: compile-free, but first-touch allocations and graph capture
+        t = srv.submit(factor=mols[0]["b"], mo_energy=mols[0]["mo"], vxc=mols[0]["vxc"],
+                       nocc=mols[0]["nocc"], fermi=mols[0]["fermi"], nfreq=nfreq, npar=npar,
+                       precision=prec)
+        srv.wait(t)
+        t0 = time.perf_counter()
+        tickets = [srv.submit(factor=m["b"], mo_energy=m["mo"], vxc=m["vxc"], nocc=m["nocc"],
+                              fermi=m["fermi"], nfreq=nfreq, npar=npar, precision=prec)
+                   for m in mols]
+        lat = [srv.wait(t)[1] for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    solves = sum(m["nmo"] for m in mols)
+    return {"molecules": count, "nmo_range": [min(m["nmo"] for m in mols), max(m["nmo"] for m in mols)],
+            "qp_solves_per_s": round(solves / wall, 2), "molecules_per_s": round(count / wall, 3),
+            "wall_s": round(wall, 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1), "workers": workers,
+            "peak_concurrent_jobs": st["peak_running"]}
+
+
+def run_serve(hip, jobs: int, rate: float, workers: int, npar: int, prec: int) -> dict:
+    """Poisson arrivals at `rate` jobs/s; nmo log-uniform in [24, 200],
+    nocc = round(nmo / 5), 100-200 frequencies; latency = arrival to result."""
+    import math
+    from minigw_amd.synthetic import SplitMix64, SERVE_SEED
+    rng = SplitMix64(SERVE_SEED)
+    u = rng.uniform(3 * jobs)
+    specs = []
+    t_arr = 0.0
+    for i in range(jobs):
+        t_arr += -math.log(1.0 - u[3 * i]) / rate
+        nmo = int(round(math.exp(math.log(24) + u[3 * i + 1] * (math.log(200) - math.log(24)))))
+        nfreq = 100 + int(u[3 * i + 2] * 101)
+        specs.append((t_arr, nmo, max(1, int(round(nmo / 5))), nfreq, SERVE_SEED + 1 + i))
+    data = [_factor_job(nmo, nocc, seed) for _, nmo, nocc, _, seed in specs]
+    with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=32) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for (ta, nmo, nocc, nfreq, _), d in zip(specs, data):
+            delay = ta - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(factor=d["b"], mo_energy=d["mo"], vxc=d["vxc"], nocc=nocc,
+                                      fermi=d["fermi"], nfreq=nfreq, npar=npar, precision=prec))
+        lat = [srv.wait(t)[1] for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    solves = sum(s[1] for s in specs)
+    return {"jobs": jobs, "rate_per_s": rate, "qp_solves_per_s": round(solves / wall, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1), "wall_s": round(wall, 2),
+            "workers": workers, "peak_concurrent_jobs": st["peak_running"],
+            "nmo_range": [min(s[1] for s in specs), max(s[1] for s in specs)]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +138,11 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-sample-freqs", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-molecules", type=int, default=32)
+    ap.add_argument("--serve-jobs", type=int, default=12)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[1.0])
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -98,6 +170,9 @@ def main() -> int:
         def step(p):
             return hip.run_g0w0(inp, args.nfreq, args.npar, precision=p, ctx=ctx, upload=False)
 
+        if args.autotune:
+            tb, tms = hip.autotune(inp, args.nfreq, args.npar, prec, ctx=ctx)
+            log(f"bench: autotuned frequency batch {tb} ({tms:.2f} ms per frequency)")
         for i in range(args.warmup):
             step(prec)
             log(f"bench: warmup {i + 1}/{args.warmup} done")
@@ -164,6 +239,14 @@ def main() -> int:
         out["qp_err_mev"] = {"max": float(dq.max()), "median": float(np.median(dq)),
                              "within_1meV": int(np.sum(dq <= 1.0)), "states": nmo,
                              "sigma_c_max_abs_err_ha": float(ds)}
+    if args.batch_molecules > 0:
+        log("bench: uneven-nmo batch ...")
+        out["batch"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq, args.npar, prec)
+    if args.serve_jobs > 0:
+        out["serve"] = []
+        for rate in args.serve_rates:
+            log(f"bench: serving at {rate} jobs/s ...")
+            out["serve"].append(run_serve(hip, args.serve_jobs, rate, args.workers, args.npar, prec))
     if not args.no_cpu:
         log("bench: cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(inp, args.nfreq, args.cpu_sample_freqs)

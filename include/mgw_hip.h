@@ -213,6 +213,13 @@ int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* ru
  * entries the caller provides (<= 12). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 
+/* Autotuner: times mgw_hip_g0w0 on the resident ERI with 8, 16, 32 and 64
+ * frequencies factored per batch (two batches each), keeps the fastest per
+ * (nmo, nocc, nstates, precision) for every later mgw_hip_g0w0 call of the
+ * process, and returns it with its time per frequency. */
+int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_batch,
+                     double* best_ms_per_freq);
+
 /* Run each frequency batch of mgw_hip_g0w0 as a captured hipGraph, replayed
  * for later batches and molecules of the same shape on the same buffers
  * (enable = 1), or launch kernels directly (0, the default, which also gives

@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -102,6 +104,9 @@ struct mgw_hip_ctx {
     int nmo = 0, nocc = 0;        // of the resident ERI (0 = none)
     bool eri_pk_sym = false;      // (pk|ia) == (kp|ia) exactly for the resident ERI
     Buf flag, hcolmap, wxp, hxp, f0buf;
+
+    // frequency batch forced by the autotuner while it times candidates (0 = auto)
+    int forced_nb = 0;
 
     // captured frequency-batch graphs (hipGraph), keyed by shape and buffers
     bool use_graphs = false;
@@ -293,6 +298,17 @@ void check_info(mgw_hip_ctx* ctx, const int* dinfo, int count, const char* what)
 }
 
 }  // namespace
+
+// Frequency-batch sizes picked by mgw_hip_autotune, per (nmo, nocc, nstates,
+// precision), shared by all contexts of the process.
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, int> g_tune;
+
+int tuned_batch(int nmo, int nocc, int ns, int precision) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find({nmo, nocc, ns, precision});
+    return it == g_tune.end() ? 0 : it->second;
+}
 
 // ================================================================ C ABI
 extern "C" {
@@ -749,6 +765,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const size_t per = batch_bytes(np, mpad, job->precision);
         int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
         nb = std::min(nb, 64);
+        const int pick = ctx->forced_nb ? ctx->forced_nb : tuned_batch(nmo, nocc, ns, job->precision);
+        if (pick > 0) nb = std::min(nb, pick);
         double* A = as<double>(ctx->get(ctx->hA, (size_t)nb * np * np * 8, "dielectric batch"));
         double* S = as<double>(ctx->get(ctx->hS, (size_t)nb * np * 8, "scale batch"));
         double* dinv = as<double>(ctx->get(ctx->hdinv, (size_t)nb * nblk * 64 * 64 * 8, "dinv"));
@@ -971,6 +989,52 @@ int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         ctx->release(ctx->wsrc);
+    });
+}
+
+int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_batch,
+                     double* best_ms_per_freq) {
+    return guarded(ctx, [&] {
+        need(ctx && job && best_batch, "mgw_hip_autotune: NULL argument");
+        const int cands[4] = {8, 16, 32, 64};
+        const int nmo = job->nmo;
+        const int ns = job->nstates > 0 && job->states ? job->nstates : nmo;
+        std::vector<double> qp(nmo), sre((size_t)ns * 128), sim((size_t)ns * 128);
+        int best = 0;
+        double best_t = 0.0;
+        const bool graphs = ctx->use_graphs;
+        ctx->use_graphs = false;
+        for (int c : cands) {
+            if (c > job->nfreq && best > 0) break;
+            mgw_g0w0_job j = *job;
+            j.nfreq = std::min(job->nfreq, 2 * c);
+            sre.resize((size_t)ns * j.nfreq);
+            sim.resize((size_t)ns * j.nfreq);
+            mgw_g0w0_out o{};
+            o.qp = qp.data();
+            o.sigma_c_re = sre.data();
+            o.sigma_c_im = sim.data();
+            ctx->forced_nb = c;
+            const int rc = mgw_hip_g0w0(ctx, &j, &o);
+            ctx->forced_nb = 0;
+            if (rc != MGW_OK) {
+                ctx->use_graphs = graphs;
+                throw MgwError(rc, ctx->error);
+            }
+            if ((int)ctx->timings[7] < c) break;      // memory capped the batch: stop growing
+            const double t = (ctx->timings[1] + ctx->timings[2] + ctx->timings[3] + ctx->timings[9]) / j.nfreq;
+            if (best == 0 || t < best_t) {
+                best = c;
+                best_t = t;
+            }
+        }
+        ctx->use_graphs = graphs;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[{nmo, job->nocc, ns, job->precision}] = best;
+        }
+        *best_batch = best;
+        if (best_ms_per_freq) *best_ms_per_freq = best_t;
     });
 }
 

@@ -84,6 +84,8 @@ def _load():
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
         "mgw_hip_set_graphs": (C.c_int, [vp, C.c_int32]),
+        "mgw_hip_autotune": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double)]),
         "mgw_hip_selftest_tile": (C.c_int, [vp, _dp, _dp, _dp]),
         "mgw_hip_probe_mfma": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, vp]),
         "mgw_hip_measure_peaks": (C.c_int, [vp, _dp]),
@@ -408,3 +410,17 @@ class HipServer:
                                                                  C.byref(p), C.byref(b)))
         return {"completed": c.value, "running": r.value, "peak_running": p.value,
                 "reserved_bytes": b.value}
+
+
+def autotune(inp, nfreq: int = 200, npar: int = 16, precision: int = PREC_FP64,
+             ctx: HipContext | None = None):
+    """Pick the frequency batch size for this molecule's shape on the resident
+    ERI; returns (batch, ms per frequency).  Later run_g0w0 calls use it."""
+    ctx = ctx or HipContext()
+    mo = _f64(inp.mo_energy)
+    vxc = _f64(np.diag(inp.vxc))
+    job = G0W0Job(inp.nmo, inp.nocc, nfreq, npar, 0, None, _ptr(mo), _ptr(vxc),
+                  float(inp.fermi_energy), 0.0, precision)
+    b, t = C.c_int32(0), C.c_double(0)
+    ctx.call("mgw_hip_autotune", C.byref(job), C.byref(b), C.byref(t))
+    return b.value, t.value

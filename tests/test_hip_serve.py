@@ -110,3 +110,16 @@ def test_graph_replays_match_eager(hip):
                 assert np.array_equal(res.qp, eager.qp, equal_nan=True)
                 s2 = hip.run_g0w0(other, 40, 8, ctx=side, upload=False, precision=prec)
                 assert np.array_equal(s2.sigma_c, side_ref.sigma_c)
+
+
+def test_autotune_picks_a_batch_and_keeps_results(hip):
+    say("case: frequency-batch autotuner, nmo=30")
+    inp = synthetic_input(30, 6, 43)
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri(inp.eri, inp.nocc)
+        before = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False)
+        nb, ms = hip.autotune(inp, 100, 16, ctx=ctx)
+        assert nb in (8, 16, 32, 64) and ms > 0
+        after = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False)
+        assert after.timings["freq_batch"] == min(nb, 100)
+        assert np.array_equal(after.sigma_c, before.sigma_c)
