@@ -201,10 +201,10 @@ def main() -> int:
     value = nmo * args.steps / elapsed
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     launches = max(ph["contract_launches"], 1.0)
-    nb = ph["freq_batch"]
     t_launch = ph["contract"] / launches * 1e-3
     cols = ph.get("contract_columns", m) or m      # unordered (p,k) pairs when the ERI is symmetric
-    flops_launch = contraction_flops(n, cols, prec) * min(nb, args.nfreq)
+    # launches are nb frequencies each except the last: average flops per launch
+    flops_launch = contraction_flops(n, cols, prec) * args.nfreq / launches
     peak = PEAK["fp16"] if prec == hip.PREC_MIXED else PEAK["fp64"]
     achieved = flops_launch / t_launch / 1e12
     out = {

@@ -17,6 +17,7 @@ namespace mgw {
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
@@ -161,7 +162,7 @@ __device__ inline int row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >
 // step ahead and stored to LDS (row stride 40 halves: 80 B keeps the 16 B
 // fragment reads of a 16-lane group on distinct banks).  M is lower
 // triangular, so the K loop stops at r0 + 256.  Tiles are visited large-K
-// first, and consecutive tiles of one row panel go to the same XCD.
+// first (bottom row panels), so the short tiles fill the tail of the grid.
 constexpr int kSBM = 256, kSBN = 256, kSBK = 32, kSLd = kSBK + 8;
 
 template <Split16 F>
@@ -191,9 +192,9 @@ __global__ __launch_bounds__(512) void k_contract_split(
                                      : tiled_index(c0 + lrow[u], lkq[u], np);
         src[u] = base + off;
     }
-    uint4 pre[8];
+    u32x4 pre[8];      // a native vector type: an array of HIP's uint4 struct is kept in scratch
 #pragma unroll
-    for (int u = 0; u < 8; ++u) pre[u] = *reinterpret_cast<const uint4*>(src[u]);
+    for (int u = 0; u < 8; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u]);
 
     f16acc acc[2][4];
 #pragma unroll
@@ -209,12 +210,15 @@ __global__ __launch_bounds__(512) void k_contract_split(
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            *reinterpret_cast<uint4*>(&lds[u >> 1][lrow[u]][lkq[u]]) = pre[u];
+            *reinterpret_cast<u32x4*>(&lds[u >> 1][lrow[u]][lkq[u]]) = pre[u];
         __syncthreads();
-        if (k0 + kSBK < kend) {
+        {
+            // Unconditional prefetch (the last step re-reads its own stage,
+            // which stays in bounds since kend >= kSBK).
+            const int kn = min(k0 + kSBK, kend - kSBK);
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                pre[u] = *reinterpret_cast<const uint4*>(src[u] + (size_t)((k0 + kSBK) >> 5) * 8192);
+                pre[u] = *reinterpret_cast<const u32x4*>(src[u] + (size_t)(kn >> 5) * 8192);
         }
 #pragma unroll
         for (int kc = 0; kc < kSBK; kc += 16) {
