@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -107,6 +108,8 @@ struct mgw_hip_ctx {
 
     // frequency batch forced by the autotuner while it times candidates (0 = auto)
     int forced_nb = 0;
+    // frequencies per 32-tile unit of the split contraction (MGW_SPLIT_UNIT_FREQS overrides)
+    int split_uf = 4;
 
     // captured frequency-batch graphs (hipGraph), keyed by shape and buffers
     bool use_graphs = false;
@@ -348,6 +351,10 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         hip_check(hipSetDevice(device), "hipSetDevice");
         auto* ctx = new mgw_hip_ctx();
         ctx->device = device;
+        if (const char* env = std::getenv("MGW_SPLIT_UNIT_FREQS")) {
+            const int uf = std::atoi(env);
+            if (uf == 1 || uf == 2 || uf == 4 || uf == 8 || uf == 16 || uf == 32) ctx->split_uf = uf;
+        }
         const hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete ctx;
@@ -798,7 +805,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
             if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
             if (mixed)
-                hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b, st),
+                hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b,
+                                                           ctx->split_uf, st),
                           "contract split");
             else
                 hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");

@@ -99,11 +99,13 @@ hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_
 
 // Same output as launch_contract_f64 but with 256-row tiles (scaled by
 // xscale[c]^2 when FP16); needs np and mpad multiples of 256:
-// products hi*hi + hi*lo + lo*hi on 16-bit MFMA (v_mfma_f32_16x16x32) with
-// FP32 accumulation, row scale undone and squares summed in FP64.
+// products hi*hi + hi*lo + lo*hi on 16-bit MFMA (v_mfma_f32_32x32x16) with
+// FP32 accumulation, row scale undone and squares summed in FP64.  uf (1, 2,
+// 4, ..., 32) is the number of frequencies in a unit of 32 tiles that share
+// one XCD's L2.
 hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_t* mlo,
                                  const double* mscale, const uint16_t* xhi, const uint16_t* xlo,
-                                 double* qpart, int np, int mpad, int nb, hipStream_t s);
+                                 double* qpart, int np, int mpad, int nb, int uf, hipStream_t s);
 
 hipError_t launch_probe16(Split16 fmt, const uint16_t* a, const uint16_t* b, float* d,
                          int ntests, hipStream_t s);

@@ -165,17 +165,43 @@ __device__ inline int row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >
 // first (bottom row panels), so the short tiles fill the tail of the grid.
 constexpr int kSBM = 256, kSBN = 256, kSBK = 32, kSLd = kSBK + 8;
 
+// Workgroup -> tile map.  Workgroups are dispatched round-robin over the 8
+// XCDs (workgroup L runs on XCD L % 8), each XCD with its own L2 and 32 CUs.
+// The tiles are cut into units of uf frequencies x uc column tiles of one row
+// panel (uf * uc <= 32).  The workgroups of a unit land on one XCD and run
+// side by side, so every K step brings uf M slabs and uc X slabs into that L2
+// (12 for 4 x 8) instead of the ~33 of a row-major order.  Units run row panel
+// by row panel, large K first; the grid is padded to whole rounds of 8 units
+// and the tiles beyond the edges exit at once.
+struct SplitTile {
+    int rt, ct, w;
+    bool valid;
+};
+__device__ inline SplitTile split_tile(int L, int nrow, int ncol, int nb, int uf, int uc) {
+    const int us = uf * uc;
+    const int xcd = L & 7, s = L >> 3;
+    const int u = (s / us) * 8 + xcd, j = s % us;
+    const int ncg = (ncol + uc - 1) / uc, nfg = (nb + uf - 1) / uf;
+    const int cg = u % ncg, fg = (u / ncg) % nfg, rr = u / (ncg * nfg);
+    SplitTile tl;
+    tl.rt = nrow - 1 - rr;
+    tl.w = fg * uf + j / uc;
+    tl.ct = cg * uc + j % uc;
+    tl.valid = rr < nrow && tl.w < nb && tl.ct < ncol;
+    return tl;
+}
+
 template <Split16 F>
 __global__ __launch_bounds__(512) void k_contract_split(
     const uint16_t* __restrict__ mhi, const uint16_t* __restrict__ mlo,
     const double* __restrict__ mscale, const uint16_t* __restrict__ xhi,
-    const uint16_t* __restrict__ xlo, double* __restrict__ qpart, int np, int mpad) {
+    const uint16_t* __restrict__ xlo, double* __restrict__ qpart, int np, int mpad, int nb,
+    int uf, int uc) {
     __shared__ __attribute__((aligned(16))) uint16_t lds[4][kSBM][kSLd];
-    const int w = blockIdx.z;
-    const int ncol = gridDim.x, nrow = gridDim.y;
-    int lin = blockIdx.x + ncol * blockIdx.y;
-    const int rt = nrow - 1 - lin / ncol, ct = lin % ncol;
-    const int r0 = rt * kSBM, c0 = ct * kSBN;
+    const SplitTile tile = split_tile(blockIdx.x, np / kSBM, mpad / kSBN, nb, uf, uc);
+    if (!tile.valid) return;                      // uniform over the workgroup
+    const int w = tile.w, rt = tile.rt;
+    const int r0 = rt * kSBM, c0 = tile.ct * kSBN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 128;
 
@@ -337,15 +363,23 @@ hipError_t launch_split_m(Split16 fmt, const double* M, const double* S, uint16_
 
 hipError_t launch_contract_split(Split16 fmt, const uint16_t* mhi, const uint16_t* mlo,
                                  const double* mscale, const uint16_t* xhi, const uint16_t* xlo,
-                                 double* qpart, int np, int mpad, int nb, hipStream_t s) {
-    if (np % kSBM != 0 || mpad % kSBN != 0) return hipErrorInvalidValue;
-    const dim3 grid(mpad / kSBN, np / kSBM, nb);
+                                 double* qpart, int np, int mpad, int nb, int uf, hipStream_t s) {
+    if (np % kSBM != 0 || mpad % kSBN != 0 || nb <= 0) return hipErrorInvalidValue;
+    if (uf != 1 && uf != 2 && uf != 4 && uf != 8 && uf != 16 && uf != 32) return hipErrorInvalidValue;
+    const int ncol = mpad / kSBN;
+    int uc = 32 / uf;
+    while (uf > 1 && uf / 2 >= nb) uf >>= 1;          // no wider than the batch
+    while (uc > 1 && uc / 2 >= ncol) uc >>= 1;        // nor than the columns
+    const long units = (long)(np / kSBM) * ((nb + uf - 1) / uf) * ((ncol + uc - 1) / uc);
+    const long wgs = (units + 7) / 8 * 8 * uf * uc;
+    if (wgs > 0x7fffffffL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)wgs);
     if (fmt == Split16::BF16)
         hipLaunchKernelGGL(k_contract_split<Split16::BF16>, grid, dim3(512), 0, s, mhi, mlo, mscale,
-                           xhi, xlo, qpart, np, mpad);
+                           xhi, xlo, qpart, np, mpad, nb, uf, uc);
     else
         hipLaunchKernelGGL(k_contract_split<Split16::FP16>, grid, dim3(512), 0, s, mhi, mlo, mscale,
-                           xhi, xlo, qpart, np, mpad);
+                           xhi, xlo, qpart, np, mpad, nb, uf, uc);
     return hipGetLastError();
 }
 
