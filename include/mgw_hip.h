@@ -161,6 +161,19 @@ typedef struct mgw_g0w0_out {
 /* One molecule, all frequencies, on the ERI made resident by mgw_hip_upload_eri. */
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
 
+/* Pade fit and QP fixed point on the device for a given Sigma_c(i w): the
+ * reference's solve_qp_energies (src/gw/qp_solver.cpp:10-47) with
+ * get_pade_coefficients_continued_fraction / evaluate_pade_continued_fraction
+ * (src/gw/pade.cpp:10-101), as run at the end of mgw_hip_g0w0.
+ * sigma_re / sigma_im: nstates x nfreq row-major for `states` (ascending,
+ * unique; NULL = all nmo states); omega: the nfreq imaginary frequencies.
+ * Writes qp[nmo] (NaN for states not listed) and, if not NULL,
+ * iterations[nmo] (0 for states not listed). */
+int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_t* states,
+                    int32_t nfreq, int32_t npar, const double* omega, const double* sigma_re,
+                    const double* sigma_im, const double* sigma_x_diag, const double* mo_energy,
+                    const double* vxc_diag, double fermi_energy, double* qp, int32_t* iterations);
+
 /* ---- request loop (continuous batching; no counterpart in the reference,
  *      which runs one molecule per process) */
 

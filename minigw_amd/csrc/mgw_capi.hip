@@ -913,6 +913,57 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
     });
 }
 
+int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_t* states,
+                    int32_t nfreq, int32_t npar, const double* omega, const double* sigma_re,
+                    const double* sigma_im, const double* sigma_x_diag, const double* mo_energy,
+                    const double* vxc_diag, double fermi_energy, double* qp, int32_t* iterations) {
+    return guarded(ctx, [&] {
+        need(ctx && omega && sigma_re && sigma_im && sigma_x_diag && mo_energy && vxc_diag && qp,
+             "mgw_hip_pade_qp: NULL argument");
+        need(nmo > 0 && nfreq > 0, "mgw_hip_pade_qp: nmo and nfreq must be positive");
+        need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
+        std::vector<int> st;
+        if (nstates <= 0 || !states) {
+            st.resize(nmo);
+            std::iota(st.begin(), st.end(), 0);
+        } else {
+            st.assign(states, states + nstates);
+            for (size_t i = 0; i < st.size(); ++i) {
+                need(st[i] >= 0 && st[i] < nmo, "Selected state is out of range");
+                need(i == 0 || st[i] > st[i - 1], "states must be ascending and unique");
+            }
+        }
+        const int ns = (int)st.size();
+        hipStream_t s = ctx->stream;
+        double* dom = as<double>(ctx->get(ctx->homega, (size_t)nfreq * 8, "omega"));
+        double* sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
+        double* sim = as<double>(ctx->get(ctx->hsig_im, (size_t)ns * nfreq * 8, "sigma im"));
+        double* sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
+        double* dmo = as<double>(ctx->get(ctx->hmo, (size_t)nmo * 8, "mo energies"));
+        double* dvxc = as<double>(ctx->get(ctx->hvxc, (size_t)nmo * 8, "vxc"));
+        int* dst = as<int>(ctx->get(ctx->hstates, (size_t)ns * 4, "states"));
+        double* dqp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
+        int* dit = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
+        ctx->h2d(dom, omega, nfreq);
+        ctx->h2d(sre, sigma_re, (size_t)ns * nfreq);
+        ctx->h2d(sim, sigma_im, (size_t)ns * nfreq);
+        ctx->h2d(sx, sigma_x_diag, nmo);
+        ctx->h2d(dmo, mo_energy, nmo);
+        ctx->h2d(dvxc, vxc_diag, nmo);
+        ctx->h2d(dst, st.data(), ns);
+        std::vector<double> nan(nmo, std::numeric_limits<double>::quiet_NaN());
+        std::vector<int> zero(nmo, 0);
+        ctx->h2d(dqp, nan.data(), nmo);
+        ctx->h2d(dit, zero.data(), nmo);
+        hip_check(mgw::launch_pade_qp(sre, sim, dom, dmo, sx, dvxc, dst, fermi_energy, ns, nfreq,
+                                      npar, dqp, dit, s),
+                  "pade/qp");
+        ctx->d2h(qp, dqp, nmo);
+        if (iterations) ctx->d2h(iterations, dit, nmo);
+        ctx->sync();
+    });
+}
+
 int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, double* c) {
     return guarded(ctx, [&] {
         need(ctx && a && b && c, "mgw_hip_selftest_tile: NULL argument");

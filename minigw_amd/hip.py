@@ -82,6 +82,8 @@ def _load():
         "mgw_hip_build_vph_from_eri": (C.c_int, [vp]),
         "mgw_hip_quadratic_forms": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
+        "mgw_hip_pade_qp": (C.c_int, [vp, C.c_int32, C.c_int32, _ip, C.c_int32, C.c_int32, _dp,
+                                      _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _ip]),
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
         "mgw_hip_set_graphs": (C.c_int, [vp, C.c_int32]),
         "mgw_hip_autotune": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(C.c_int32),
@@ -333,6 +335,27 @@ def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.
     ctx.call("mgw_hip_g0w0", C.byref(job), C.byref(out))
     return HipG0W0Result(om, w, list(range(nmo)) if st is None else st.tolist(), sx,
                          sre + 1j * sim, qp, it, ctx.last_timings())
+
+
+def pade_qp(sigma_c, omegas, sigma_x_diag, mo_energy, vxc_diag, fermi: float, npar: int = 16,
+            states=None, ctx: HipContext | None = None):
+    """Device Pade fit + QP fixed point on a given Sigma_c(i w) (nstates x
+    nfreq complex); mirrors solve_qp_energies (src/gw/qp_solver.cpp:10-47).
+    Returns (qp[nmo], iterations[nmo])."""
+    ctx = ctx or HipContext()
+    sig = np.asarray(sigma_c)
+    mo = _f64(mo_energy)
+    nmo = len(mo)
+    st = None if states is None else np.ascontiguousarray(states, dtype=np.int32)
+    ns = nmo if st is None else len(st)
+    if sig.shape != (ns, len(omegas)):
+        raise ValueError(f"sigma_c shape {sig.shape} != ({ns}, {len(omegas)})")
+    sre, sim = _f64(sig.real), _f64(sig.imag)
+    qp, it = np.zeros(nmo), np.zeros(nmo, dtype=np.int32)
+    ctx.call("mgw_hip_pade_qp", nmo, 0 if st is None else ns, _ptr(st, _ip), len(omegas), npar,
+             _ptr(_f64(omegas)), _ptr(sre), _ptr(sim), _ptr(_f64(sigma_x_diag)), _ptr(mo),
+             _ptr(_f64(vxc_diag)), float(fermi), _ptr(qp), _ptr(it, _ip))
+    return qp, it
 
 
 class HipServer:

@@ -177,6 +177,28 @@ def test_pade_qp_kernel_exact_on_same_sigma(hip, ctx):
     assert np.array_equal(qp, res.qp)
 
 
+def test_pade_qp_entry_point_matches_oracle(hip, ctx):
+    """mgw_hip_pade_qp on a host Sigma_c (oracle's, perturbed, a state
+    subset) equals the oracle's solve_qp bit for bit."""
+    say("case: mgw_hip_pade_qp vs oracle solve_qp, nmo=24, states 0,4,5,6,23")
+    inp = synthetic_input(*CONFIGS["c1"])
+    want = orc.run_g0w0(inp, 48, 12)
+    rng = np.random.default_rng(3)
+    states = [0, 4, 5, 6, 23]
+    sig = want.sigma_c[states] * (1 + 1e-3 * rng.standard_normal((len(states), 48)))
+    qp_o, it_o = orc.solve_qp(inp.mo_energy, inp.fermi_energy, want.sigma_x_diag,
+                              np.diag(inp.vxc), sig, want.omegas, states, 12)
+    qp_d, it_d = hip.pade_qp(sig, want.omegas, want.sigma_x_diag, inp.mo_energy, np.diag(inp.vxc),
+                             inp.fermi_energy, 12, states=states, ctx=ctx)
+    assert np.array_equal(qp_d, qp_o, equal_nan=True)
+    assert np.array_equal(it_d, it_o)
+    assert np.isnan(qp_d[1]) and it_d[1] == 0
+    with pytest.raises(hip.HipError) as e:
+        hip.pade_qp(sig, want.omegas, want.sigma_x_diag, inp.mo_energy, np.diag(inp.vxc),
+                    inp.fermi_energy, 65, states=states, ctx=ctx)
+    assert e.value.code == hip.MGW_ERR_ARG
+
+
 @pytest.mark.parametrize("nmo,nocc,nfreq", [(60, 12, 12), (90, 18, 6)])
 def test_multiblock_vs_oracle(hip, ctx, nmo, nocc, nfreq):
     say(f"case: multi-block factorization nmo={nmo} nocc={nocc} nfreq={nfreq}")
