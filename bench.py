@@ -22,7 +22,6 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-PEAK = {"fp64": 78.6e12, "fp16": 2.5e15, "hbm": 8.0e12}
 METRIC = "G0W0 QP solves/sec + p50/p99 latency, mixed vs FP64 HIP, QP err ≤1 meV"
 
 
@@ -33,9 +32,10 @@ def log(*a):
 def contraction_flops(n: int, m: int, precision: int) -> float:
     """Algorithmic flops of one frequency's contraction Z = (L^-1 S) X, L^-1
     lower triangular: 2 * (n(n+1)/2) * m for the m contracted columns (the
-    unordered (p,k) pairs when q(p,k) = q(k,p)); three products in the split form."""
+    unordered (p,k) pairs when q(p,k) = q(k,p)); three products in the FP16
+    split form, fifteen int8 slice products in the int8 form."""
     f = float(n) * (n + 1) * m
-    return 3.0 * f if precision == 1 else f
+    return {1: 3.0, 2: 15.0}.get(precision, 1.0) * f
 
 
 HA_MEV = 27.211386245988 * 1e3
@@ -90,7 +90,7 @@ def accuracy(hip, ctx, inp, mix, f64, npar: int, trials: int = 3) -> dict:
     }
 
 
-def h2o_check(hip, ctx, npar: int) -> dict:
+def h2o_check(hip, ctx, npar: int, prec: int) -> dict:
     """The same report on H2O (nmo=18, the reference's regression input, kept
     as a fixture in tests/golden), whose QP energies are well conditioned."""
     from minigw_amd.synthetic import GwInput
@@ -99,7 +99,7 @@ def h2o_check(hip, ctx, npar: int) -> dict:
                   fermi_energy=float(d["fermi_energy"]))
     ctx.upload_eri(inp.eri, inp.nocc)
     f64 = hip.run_g0w0(inp, 200, npar, ctx=ctx, upload=False, precision=hip.PREC_FP64)
-    mix = hip.run_g0w0(inp, 200, npar, ctx=ctx, upload=False, precision=hip.PREC_MIXED)
+    mix = hip.run_g0w0(inp, 200, npar, ctx=ctx, upload=False, precision=prec)
     return accuracy(hip, ctx, inp, mix, f64, npar)
 
 
@@ -191,6 +191,33 @@ def run_serve(hip, jobs: int, rate: float, workers: int, npar: int, prec: int) -
             "nmo_range": [min(s[1] for s in specs), max(s[1] for s in specs)]}
 
 
+FORMS = {
+    # name: (precision code, contraction kernel, products per element, vendor peak,
+    #        measured-peak key, dtype label)
+    "mixed": (1, "k_contract_split<FP16>", 3, 2.5e15, "fp16_mfma_32x32", "fp16x3-split+fp64"),
+    "mixed_int8": (2, "k_contract_i8", 15, 5.0e15, "int8_mfma_32x32", "int8x5-slice+fp64"),
+    "fp64": (0, "k_contract_f64", 1, 78.6e12, "fp64_mfma", "fp64"),
+}
+
+
+def roofline(form: str, ph: dict, n: int, m: int, nfreq: int, config: str, peaks: dict) -> dict:
+    """Contraction kernel: algorithmic flops (or int8 ops) per launch over its
+    average launch time (HIP events around the kernel on the context's stream)."""
+    code, kernel, products, peak, key, _ = FORMS[form]
+    launches = max(ph["contract_launches"], 1.0)
+    t_launch = ph["contract"] / launches * 1e-3
+    cols = ph.get("contract_columns", m) or m      # unordered (p,k) pairs when the ERI is symmetric
+    # launches are nb frequencies each except the last: average work per launch
+    work = contraction_flops(n, cols, 0) * products * nfreq / launches
+    achieved = work / t_launch / 1e12
+    return {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 2),
+            "peak": peak / 1e12, "unit": "TOP/s" if form == "mixed_int8" else "TFLOP/s",
+            "frac": round(achieved * 1e12 / peak, 4),
+            "traffic": TRAFFIC.get((kernel, config, nfreq)),
+            "launch_ms": round(t_launch * 1e3, 3), "work_per_launch": work,
+            "measured_peak": round(peaks[key] / 1e12, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +226,9 @@ def main() -> int:
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
     ap.add_argument("--nfreq", type=int, default=200)
     ap.add_argument("--npar", type=int, default=16)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed_int8", choices=list(FORMS))
+    ap.add_argument("--alt-precision", default="mixed", choices=list(FORMS) + ["none"],
+                    help="second mixed form, timed and judged beside the headline one")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-sample-freqs", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -221,7 +250,8 @@ def main() -> int:
     t0 = time.perf_counter()
     inp = synthetic_input(nmo, nocc, seed)
     log(f"bench: generated {args.config} nmo={nmo} nocc={nocc} in {time.perf_counter() - t0:.1f} s")
-    prec = hip.PREC_MIXED if args.precision == "mixed" else hip.PREC_FP64
+    prec = FORMS[args.precision][0]
+    alt = None if args.alt_precision in ("none", args.precision, "fp64") else args.alt_precision
     n, m = inp.nph, nmo * nmo
 
     with hip.HipContext(0) as ctx:
@@ -251,38 +281,40 @@ def main() -> int:
             log(f"bench: step {i + 1}/{args.steps} {lat[-1] * 1e3:.1f} ms")
         elapsed = time.perf_counter() - t_start
 
-        # plain FP64 form: error judge and ratio
-        f64 = None
-        f64_lat = []
-        if prec == hip.PREC_MIXED and args.fp64_steps > 0:
-            for i in range(args.fp64_steps):
+        def timed(p, steps, what):
+            r, ts = None, []
+            for i in range(steps):
                 t1 = time.perf_counter()
-                f64 = step(hip.PREC_FP64)
-                f64_lat.append(time.perf_counter() - t1)
-                log(f"bench: fp64 step {i + 1}/{args.fp64_steps} {f64_lat[-1] * 1e3:.1f} ms")
-        f64_phase = ctx.last_timings() if f64 is not None else None
-        acc = h2o = None
+                r = step(p)
+                ts.append(time.perf_counter() - t1)
+                log(f"bench: {what} step {i + 1}/{steps} {ts[-1] * 1e3:.1f} ms")
+            return r, ts, ctx.last_timings()
+
+        # plain FP64 form: error judge and ratio; the other mixed form beside it
+        f64 = alt_res = None
+        if prec != 0 and args.fp64_steps > 0:
+            f64, f64_lat, f64_phase = timed(0, args.fp64_steps, "fp64")
+            if alt:
+                step(FORMS[alt][0])
+                alt_res, alt_lat, alt_phase = timed(FORMS[alt][0], 1, alt)
+        report = {}
         if f64 is not None:
             log("bench: accuracy report ...")
-            acc = accuracy(hip, ctx, inp, res, f64, args.npar)
-            h2o = h2o_check(hip, ctx, args.npar)
+            report["accuracy_vs_fp64"] = accuracy(hip, ctx, inp, res, f64, args.npar)
+            if alt_res is not None:
+                report["alt_accuracy"] = accuracy(hip, ctx, inp, alt_res, f64, args.npar)
+            report["accuracy_vs_fp64_h2o"] = h2o_check(hip, ctx, args.npar, prec)
+            if alt_res is not None:
+                report["alt_accuracy_h2o"] = h2o_check(hip, ctx, args.npar, FORMS[alt][0])
 
     ms = elapsed / args.steps * 1e3
     value = nmo * args.steps / elapsed
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
-    launches = max(ph["contract_launches"], 1.0)
-    t_launch = ph["contract"] / launches * 1e-3
-    cols = ph.get("contract_columns", m) or m      # unordered (p,k) pairs when the ERI is symmetric
-    # launches are nb frequencies each except the last: average flops per launch
-    flops_launch = contraction_flops(n, cols, prec) * args.nfreq / launches
-    peak = PEAK["fp16"] if prec == hip.PREC_MIXED else PEAK["fp64"]
-    achieved = flops_launch / t_launch / 1e12
-    kernel = "k_contract_split<FP16>" if prec == hip.PREC_MIXED else "k_contract_f64"
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "QP solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp16x3-split+fp64" if prec == hip.PREC_MIXED else "fp64",
+        "dtype": FORMS[args.precision][5],
         "data": "synthetic (seeded SplitMix64 generator)",
         "config": {"workload": f"{args.config}: synthetic nmo={nmo} nocc={nocc} nph={n}, "
                                f"{args.nfreq} GL freqs, {args.npar} Pade, all states, "
@@ -291,22 +323,22 @@ def main() -> int:
         "p50_ms": round(float(np.percentile(np.array(lat) * 1e3, 50)), 2),
         "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 2),
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},
-        "roofline": {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 2),
-                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4),
-                     "traffic": TRAFFIC.get((kernel, args.config, args.nfreq)),
-                     "launch_ms": round(t_launch * 1e3, 3),
-                     "flops_per_launch": flops_launch,
-                     "measured_peak": round((peaks["fp16_mfma_32x32"] if prec == hip.PREC_MIXED
-                                             else peaks["fp64_mfma"]) / 1e12, 1)},
+        "roofline": roofline(args.precision, ph, n, m, args.nfreq, args.config, peaks),
         "measured_peaks_tflops_or_tbps": {k: round(v / 1e12, 2) for k, v in peaks.items()},
     }
     if f64 is not None:
         fv = nmo * len(f64_lat) / sum(f64_lat)
         out["fp64"] = {"value": round(fv, 3), "ms_per_step": round(float(np.mean(f64_lat)) * 1e3, 2),
-                       "phases_ms": {k: round(v, 3) for k, v in f64_phase.items()}}
+                       "phases_ms": {k: round(v, 3) for k, v in f64_phase.items()},
+                       "roofline": roofline("fp64", f64_phase, n, m, args.nfreq, args.config, peaks)}
         out["mixed_over_fp64"] = round(value / fv, 3)
-        out["accuracy_vs_fp64"] = acc
-        out["accuracy_vs_fp64_h2o"] = h2o
+        if alt_res is not None:
+            av = nmo * len(alt_lat) / sum(alt_lat)
+            out["alt"] = {"precision": alt, "value": round(av, 3), "over_fp64": round(av / fv, 3),
+                          "ms_per_step": round(float(np.mean(alt_lat)) * 1e3, 2),
+                          "phases_ms": {k: round(v, 3) for k, v in alt_phase.items()},
+                          "roofline": roofline(alt, alt_phase, n, m, args.nfreq, args.config, peaks)}
+        out.update(report)
     if args.batch_molecules > 0:
         log("bench: uneven-nmo batch ...")
         out["batch"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq, args.npar, prec)

@@ -42,8 +42,12 @@ enum mgw_status {
 
 enum mgw_precision {
     MGW_PREC_FP64 = 0,       /* everything in FP64                                */
-    MGW_PREC_MIXED = 1       /* BF16 hi/lo split contraction on MFMA (FP32 acc),  */
-                             /* FP64 factorization, FP64 Sigma_c/Pade/QP          */
+    MGW_PREC_MIXED = 1,      /* FP16 hi/lo split contraction on MFMA (3 products, */
+                             /* FP32 acc; ~5e-8 relative in Sigma_c), FP64        */
+                             /* factorization and Sigma_c/Pade/QP                 */
+    MGW_PREC_MIXED_INT8 = 2  /* contraction on int8 MFMA: 5 slices per operand,   */
+                             /* exact int32 products, FP64 epilogue (~1e-10       */
+                             /* relative in Sigma_c); FP64 elsewhere              */
 };
 
 /* ---- library / context ------------------------------------------------ */
@@ -249,7 +253,8 @@ int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, do
  * kind 0: v_mfma_f64_16x16x4, a/b double[ntests][64], d double[ntests][64][4];
  * kind 1/2: v_mfma_f32_16x16x32 bf16/f16, a/b uint16[ntests][64][8],
  * d float[ntests][64][4]; kind 3/4: v_mfma_f32_32x32x16 bf16/f16, same
- * inputs, d float[ntests][64][16]. */
+ * inputs, d float[ntests][64][16]; kind 5: v_mfma_i32_32x32x32_i8,
+ * a/b int8[ntests][64][16], d int32[ntests][64][16]. */
 int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const void* a,
                        const void* b, void* d);
 
@@ -257,7 +262,8 @@ int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const voi
  * device -- out[0] FP64 MFMA, out[1] FP16 MFMA, out[2] BF16 MFMA (flop/s,
  * 16x16 tiles, register-resident operands, 32 waves per CU), out[3] HBM copy
  * bandwidth (bytes read + written per second, 2 GiB buffers), out[4] FP16 and
- * out[5] BF16 MFMA with 32x32x16 tiles.  out holds 6 entries. */
+ * out[5] BF16 MFMA with 32x32x16 tiles, out[6] int8 MFMA 32x32x32 and out[7]
+ * int8 16x16x64 (op/s).  out holds 8 entries. */
 int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out);
 
 /* Transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:62-93). */

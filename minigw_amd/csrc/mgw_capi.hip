@@ -279,6 +279,7 @@ size_t batch_bytes(int np, int mpad, int precision) {
                + (size_t)(np / mgw::kCT) * mpad * 8  // qpart
                + 4;                          // info
     if (precision == MGW_PREC_MIXED) b += (size_t)np * np * 4 + (size_t)np * 8;  // M planes
+    if (precision == MGW_PREC_MIXED_INT8) b += (size_t)np * np * mgw::kSlices + (size_t)np * 8;
     return b;
 }
 
@@ -653,7 +654,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         need(out->qp != nullptr, "mgw_hip_g0w0: out->qp is NULL");
         if (job->eta != 0.0)
             fail(MGW_ERR_UNSUPPORTED, "eta != 0 (complex screening) is not supported by the hip backend");
-        need(job->precision == MGW_PREC_FP64 || job->precision == MGW_PREC_MIXED,
+        need(job->precision == MGW_PREC_FP64 || job->precision == MGW_PREC_MIXED ||
+                 job->precision == MGW_PREC_MIXED_INT8,
              "mgw_hip_g0w0: unknown precision %d", job->precision);
         std::vector<int> states;
         if (job->nstates <= 0 || !job->states) {
@@ -744,11 +746,12 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->v_ready = true;
         ctx->wc_ready = false;
         const bool mixed = job->precision == MGW_PREC_MIXED;
+        const bool i8 = job->precision == MGW_PREC_MIXED_INT8;
         const mgw::Split16 fmt = mgw::Split16::FP16;
         uint16_t *xh = nullptr, *xl = nullptr;
         double* xs = nullptr;
         double* xp = nullptr;
-        if (!mixed) {
+        if (!mixed && !i8) {
             xp = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X"));
             hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, n,
                                          np, mpad, st),
@@ -761,6 +764,15 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             hip_check(mgw::launch_split_x(fmt, as<double>(ctx->eri.p), dpk,
                                           as<int>(ctx->colidx.p), xh, xl, xs, n, np, mpad, st),
                       "split X");
+        }
+        if (i8) {
+            need(np <= mgw::kMaxK, "mgw_hip_g0w0: nph %d exceeds the int8 contraction's limit %d", n,
+                 mgw::kMaxK);
+            xh = as<uint16_t>(ctx->get(ctx->xhi, (size_t)mgw::kSlices * mpad * np, "X slices"));
+            xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
+            hip_check(mgw::launch_slice_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p),
+                                          reinterpret_cast<int8_t*>(xh), xs, n, np, mpad, st),
+                      "slice X");
         }
         hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
 
@@ -789,6 +801,10 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             ml = as<uint16_t>(ctx->get(ctx->mlo, (size_t)nb * np * np * 2, "M lo"));
             ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
         }
+        if (i8) {
+            mh = as<uint16_t>(ctx->get(ctx->mhi, (size_t)nb * mgw::kSlices * np * np, "M slices"));
+            ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
+        }
         hip_check(hipMemsetAsync(info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
 
         float t_diel = 0, t_fact = 0, t_contr = 0, t_aux = 0, t_graph = 0;
@@ -803,18 +819,27 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
+            if (i8)
+                hip_check(mgw::launch_slice_m(A, S, reinterpret_cast<int8_t*>(mh), ms, np, b, st),
+                          "slice M");
             if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b,
-                                                           ctx->split_uf, st),
+                                                     ctx->split_uf, st),
                           "contract split");
+            else if (i8)
+                hip_check(mgw::launch_contract_i8(reinterpret_cast<const int8_t*>(mh), ms,
+                                                  reinterpret_cast<const int8_t*>(xh), qpart, np,
+                                                  mpad, b, ctx->split_uf, st),
+                          "contract int8");
             else
                 hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");
             if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
-            if (mixed)
+            if (mixed || i8)
                 hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dcolmap, dq,
-                                                      mgw::contract_split_row_tiles(np), m, mpad,
-                                                      f0p, b, st),
+                                                      mixed ? mgw::contract_split_row_tiles(np)
+                                                            : mgw::contract_i8_row_tiles(np),
+                                                      m, mpad, f0p, b, st),
                           "reduce q");
             else
                 hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0p, b, st),
@@ -980,8 +1005,9 @@ int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const voi
                        const void* b, void* d) {
     return guarded(ctx, [&] {
         need(ctx && a && b && d && ntests > 0 && ntests <= 65536, "mgw_hip_probe_mfma: bad argument");
-        need(kind >= 0 && kind <= 4,
-             "mgw_hip_probe_mfma: kind must be 0 (f64), 1/2 (bf16/f16 16x16), 3/4 (bf16/f16 32x32)");
+        need(kind >= 0 && kind <= 5,
+             "mgw_hip_probe_mfma: kind must be 0 (f64), 1/2 (bf16/f16 16x16), 3/4 (bf16/f16 32x32), "
+             "5 (i8 32x32x32)");
         const size_t in_bytes = kind == 0 ? 64 * 8 : 64 * 16;
         const size_t out_bytes = kind == 0 ? 64 * 4 * 8 : kind <= 2 ? 64 * 4 * 4 : 64 * 16 * 4;
         char* buf = as<char>(ctx->get(ctx->wsrc, (size_t)ntests * (2 * in_bytes + out_bytes), "probe"));
@@ -997,10 +1023,13 @@ int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const voi
             hip_check(mgw::launch_probe16(kind == 1 ? mgw::Split16::BF16 : mgw::Split16::FP16,
                                           as<uint16_t>(da), as<uint16_t>(db), as<float>(dd), ntests,
                                           ctx->stream), "probe 16");
-        else
+        else if (kind <= 4)
             hip_check(mgw::launch_probe32(kind == 3 ? mgw::Split16::BF16 : mgw::Split16::FP16,
                                           as<uint16_t>(da), as<uint16_t>(db), as<float>(dd), ntests,
                                           ctx->stream), "probe 32");
+        else
+            hip_check(mgw::launch_probe_i8(as<int8_t>(da), as<int8_t>(db), as<int>(dd), ntests,
+                                           ctx->stream), "probe i8");
         ctx->d2h(static_cast<char*>(d), dd, (size_t)ntests * out_bytes);
         ctx->sync();
     });
@@ -1019,9 +1048,10 @@ int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
-        const double flop[5] = {2048.0 * 8, 16384.0 * 8, 16384.0 * 8, 32768.0 * 4, 32768.0 * 4};
-        const int iters[5] = {2000, 4000, 4000, 4000, 4000};
-        for (int kind = 0; kind < 5; ++kind) {
+        const double flop[7] = {2048.0 * 8,  16384.0 * 8, 16384.0 * 8, 32768.0 * 4,
+                                32768.0 * 4, 65536.0 * 4, 32768.0 * 8};
+        const int iters[7] = {2000, 4000, 4000, 4000, 4000, 4000, 4000};
+        for (int kind = 0; kind < 7; ++kind) {
             const int blocks = cus * 8;
             hip_check(mgw::launch_mfma_peak(kind, blocks, 10, sink, ctx->stream), "mfma peak");
             hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
@@ -1115,8 +1145,9 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
     const size_t eri = (size_t)nmo * nmo * nmo * nmo * 8;
     const size_t v = (size_t)L.np * L.np * 8;
     size_t fixed = eri + v + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
-    fixed += precision == MGW_PREC_MIXED ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8
-                                         : (size_t)mpad * L.np * 8;
+    fixed += precision == MGW_PREC_MIXED        ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8
+             : precision == MGW_PREC_MIXED_INT8 ? (size_t)mpad * L.np * mgw::kSlices + (size_t)mpad * 8
+                                                : (size_t)mpad * L.np * 8;
     const int nb = std::max(1, std::min(nbatch > 0 ? nbatch : nfreq, nfreq));
     return fixed + (size_t)nb * batch_bytes(L.np, mpad, precision);
 }

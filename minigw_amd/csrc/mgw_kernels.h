@@ -13,6 +13,8 @@ constexpr int kNB = 64;    // factorization block (potrf / trtri tiles)
 constexpr int kCT = 128;   // FP64 contraction tile: rows of M and columns of X per workgroup
 constexpr int kPad = 256;  // np and the contraction column count are padded to this
 constexpr int kMaxPade = 64;
+constexpr int kSlices = 5;       // int8 slices per operand of the integer-MFMA contraction
+constexpr int kMaxK = 65536;     // its largest np (keeps the int32 group sums exact)
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -141,5 +143,20 @@ hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const doub
 // streaming copy of `count` doubles (count % 4 == 0).
 hipError_t launch_mfma_peak(int kind, int blocks, int iters, float* sink, hipStream_t s);
 hipError_t launch_stream_copy(const double* src, double* dst, size_t count, hipStream_t s);
+
+// Integer-MFMA form of the contraction (kSlices int8 slices per operand,
+// exact int32 products, FP64 epilogue).  X columns (as launch_split_x) and
+// the rows of M_w S_w are scaled by powers of two (xscale, mscale) and sliced
+// into kSlices planes of 128 x 32 byte blocks; launch_contract_i8 writes the
+// same qpart as the other contractions with np / 128 row tiles (reduce with
+// launch_reduce_q_scaled).  np and mpad multiples of 128, np <= kMaxK.
+hipError_t launch_slice_x(const double* src, const int64_t* pkrow, const int* colidx,
+                          int8_t* planes, double* xscale, int n, int np, int mpad, hipStream_t s);
+hipError_t launch_slice_m(const double* M, const double* S, int8_t* planes, double* mscale, int np,
+                          int nb, hipStream_t s);
+hipError_t launch_contract_i8(const int8_t* mplanes, const double* mscale, const int8_t* xplanes,
+                              double* qpart, int np, int mpad, int nb, int uf, hipStream_t s);
+int contract_i8_row_tiles(int np);
+hipError_t launch_probe_i8(const int8_t* a, const int8_t* b, int* d, int ntests, hipStream_t s);
 
 }  // namespace mgw

@@ -14,7 +14,8 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 
 // kind 0: v_mfma_f64_16x16x4 (2048 flop, 8 accumulators), 1: f16 16x16x32 (16384),
-// 2: bf16 16x16x32 (16384), 3: f16 32x32x16 (32768, 4 accumulators), 4: bf16 32x32x16
+// 2: bf16 16x16x32 (16384), 3: f16 32x32x16 (32768, 4 accumulators), 4: bf16 32x32x16,
+// 5: i8 32x32x32 (65536 op, 4 accumulators), 6: i8 16x16x64 (32768 op, 8 accumulators)
 template <int KIND>
 __global__ __launch_bounds__(256) void k_mfma_peak(int iters, float* sink) {
     const int lane = threadIdx.x & 63;
@@ -28,6 +29,32 @@ __global__ __launch_bounds__(256) void k_mfma_peak(int iters, float* sink) {
         double s = 0;
         for (int i = 0; i < 8; ++i) s += acc[i][0] + acc[i][3];
         if (s == 12345.0) sink[threadIdx.x] = (float)s;
+    } else if constexpr (KIND == 5) {
+        typedef int i16v __attribute__((ext_vector_type(16)));
+        typedef int i4v __attribute__((ext_vector_type(4)));
+        i16v acc[4];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 16; ++j) acc[i][j] = 0;
+        const i4v av = {0x01010101 + lane, 0x01010101, 0x01010101, 0x01010101};
+        const i4v bv = {0x01010101, 0x01010101 + lane, 0x01010101, 0x01010101};
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[i], 0, 0, 0);
+        int s = 0;
+        for (int i = 0; i < 4; ++i) s += acc[i][0] + acc[i][15];
+        if (s == 12345) sink[threadIdx.x] = (float)s;
+    } else if constexpr (KIND == 6) {
+        typedef int i4v __attribute__((ext_vector_type(4)));
+        i4v acc[8];
+        for (int i = 0; i < 8; ++i) acc[i] = i4v{0, 0, 0, 0};
+        const i4v av = {0x01010101 + lane, 0x01010101, 0x01010101, 0x01010101};
+        const i4v bv = {0x01010101, 0x01010101 + lane, 0x01010101, 0x01010101};
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[i], 0, 0, 0);
+        int s = 0;
+        for (int i = 0; i < 8; ++i) s += acc[i][0] + acc[i][3];
+        if (s == 12345) sink[threadIdx.x] = (float)s;
     } else if constexpr (KIND >= 3) {
         typedef float f16v __attribute__((ext_vector_type(16)));
         f16v acc[4];
@@ -81,7 +108,9 @@ hipError_t launch_mfma_peak(int kind, int blocks, int iters, float* sink, hipStr
     else if (kind == 1) hipLaunchKernelGGL(k_mfma_peak<1>, dim3(blocks), dim3(256), 0, s, iters, sink);
     else if (kind == 2) hipLaunchKernelGGL(k_mfma_peak<2>, dim3(blocks), dim3(256), 0, s, iters, sink);
     else if (kind == 3) hipLaunchKernelGGL(k_mfma_peak<3>, dim3(blocks), dim3(256), 0, s, iters, sink);
-    else hipLaunchKernelGGL(k_mfma_peak<4>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 4) hipLaunchKernelGGL(k_mfma_peak<4>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 5) hipLaunchKernelGGL(k_mfma_peak<5>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else hipLaunchKernelGGL(k_mfma_peak<6>, dim3(blocks), dim3(256), 0, s, iters, sink);
     return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libm
 
 MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_MEMORY, \
     MGW_ERR_STATE = range(7)
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_INT8 = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -217,10 +217,15 @@ class HipContext:
 
     def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         """Raw single-MFMA results: kind 0 f64 (a, b: [t, 64]); 1 bf16 / 2 f16 16x16x32,
-        3 bf16 / 4 f16 32x32x16 (a, b: [t, 64, 8] uint16)."""
+        3 bf16 / 4 f16 32x32x16 (a, b: [t, 64, 8] uint16); 5 i8 32x32x32
+        (a, b: [t, 64, 16] int8, result int32)."""
         if kind == 0:
             a, b = _f64(a), _f64(b)
             d = np.zeros((a.shape[0], 64, 4))
+        elif kind == 5:
+            a = np.ascontiguousarray(a, dtype=np.int8)
+            b = np.ascontiguousarray(b, dtype=np.int8)
+            d = np.zeros((a.shape[0], 64, 16), dtype=np.int32)
         else:
             a = np.ascontiguousarray(a, dtype=np.uint16)
             b = np.ascontiguousarray(b, dtype=np.uint16)
@@ -230,10 +235,11 @@ class HipContext:
 
     def measure_peaks(self) -> dict:
         """Measured FP64 / FP16 / BF16 MFMA flop/s and HBM copy bytes/s of this device."""
-        o = np.zeros(6)
+        o = np.zeros(8)
         self.call("mgw_hip_measure_peaks", _ptr(o))
         return {"fp64_mfma": o[0], "fp16_mfma": o[1], "bf16_mfma": o[2], "hbm_copy": o[3],
-                "fp16_mfma_32x32": o[4], "bf16_mfma_32x32": o[5]}
+                "fp16_mfma_32x32": o[4], "bf16_mfma_32x32": o[5], "int8_mfma_32x32": o[6],
+                "int8_mfma_16x16": o[7]}
 
     def selftest_tile(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         a, b = _f64(a), _f64(b)

@@ -250,3 +250,31 @@ def test_mixed_vs_fp64(hip, ctx, case):
         f"median {np.median(mev):.3g} meV; noise envelope median {np.median(env) * 27211.4:.3g} meV")
     assert d < 1e-6
     assert np.all(dq <= np.maximum(1.0 / 27211.386, 10 * env)), (mev, env)
+
+
+@pytest.mark.parametrize("case", ["h2o", "syn24", "syn60", "syn90"])
+def test_mixed_int8_vs_fp64(hip, ctx, case):
+    """int8-slice form against the FP64 form: Sigma_c(i w) to 1e-9 relative
+    (5 slices truncate at ~2^-35 of a row's largest element, the products are
+    exact); H2O QP energies all within 1 meV; the synthetic molecules' QP
+    energies within 1 meV or inside their noise envelope."""
+    say(f"case: int8 mixed vs FP64 {case}")
+    if case == "h2o":
+        inp = h2o_input()
+    else:
+        nmo = int(case[3:])
+        inp = synthetic_input(nmo, nmo // 5, 31)
+    nf = 200 if case == "h2o" else 48
+    f64 = hip.run_g0w0(inp, nf, 16, ctx=ctx)
+    mix = hip.run_g0w0(inp, nf, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+    d = np.abs(mix.sigma_c - f64.sigma_c).max() / np.abs(f64.sigma_c).max()
+    mev = np.abs(mix.qp - f64.qp) * orc.K_HARTREE_TO_EV * 1000
+    say(f"   rel |dSigma_c| = {d:.2e}; QP within 1 meV: {np.sum(mev <= 1.0)}/{inp.nmo}; "
+        f"max {np.max(mev):.3g} meV")
+    assert d < 1e-9
+    if case == "h2o":
+        assert np.all(mev <= 1.0), mev
+    else:
+        env = qp_envelope(inp, f64.omegas, f64.sigma_x_diag, f64.sigma_c, list(range(inp.nmo)),
+                          16, max(d, 1e-15))
+        assert np.all(mev <= np.maximum(1.0, 10 * env * orc.K_HARTREE_TO_EV * 1000)), (mev, env)

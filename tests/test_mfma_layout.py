@@ -114,3 +114,29 @@ def test_32x32_layout(ctx, kind):
             assert nz == want, (kind, la, e, nz[:4])
             vals = set(float(out[l, r]) for l, r in nz)
             assert vals == {1.0 + e + 8 * (la // 32)}, (kind, la, e, vals)
+
+
+def test_i8_32x32x32_layout(ctx):
+    """v_mfma_i32_32x32x32_i8: a byte e of lane l = A[l % 32][k(l / 32, e)],
+    b likewise with columns and the same k map, d[r] as the 32x32x16 forms."""
+    n = 64 * 16
+    a = np.zeros((n, 64, 16), dtype=np.int8)
+    for la in range(64):
+        for e in range(16):
+            a[la * 16 + e, la, e] = 1
+    bv = np.zeros((64, 16), dtype=np.int8)
+    for lb in range(64):
+        for e in range(16):
+            bv[lb, e] = 1 + e + 16 * (lb // 32)
+    b = np.broadcast_to(bv, (n, 64, 16))
+    d = ctx.probe_mfma(5, a, b)
+    _dump("probe_i8.npz", d=d)
+    for la in range(64):
+        for e in range(16):
+            out = d[la * 16 + e]
+            nz = sorted(tuple(x) for x in np.argwhere(out != 0))
+            i = la % 32
+            want = sorted((j + 32 * ((i % 8) // 4), 4 * (i // 8) + i % 4) for j in range(32))
+            assert nz == want, (la, e, nz[:4])
+            vals = set(int(out[l, r]) for l, r in nz)
+            assert vals == {1 + e + 16 * (la // 32)}, (la, e, vals)
