@@ -29,7 +29,13 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kS = kSlices;
 constexpr int kIBM = 128, kIBN = 128, kIBK = 32;   // workgroup tile and K step
-constexpr int kILd = kIBK + 16;                    // LDS row stride (bytes)
+
+// LDS byte offset of the 16 B half `kh` (0 or 16) of row `row` in a 128 x 32
+// slice block: rows of 32 B, halves swapped in rows 8-15 of every 16, so that
+// the 16 lanes served together by a 16 B access (8 rows x 2 halves when a
+// block is stored, 16 rows x 1 half when a fragment is read) hit 16 distinct
+// groups of 4 of the 64 banks (measured: SQ_LDS_BANK_CONFLICT).
+__device__ inline int lds_off(int row, int kh) { return row * 32 + (kh ^ (((row >> 3) & 1) << 4)); }
 
 // Plane layout: blocks of 128 rows x 32 k (4096 bytes, row-major inside),
 // ordered by row block then k block; one K step of a 128-row tile is one
@@ -148,14 +154,13 @@ __device__ inline I8Tile i8_tile(int L, int nrow, int ncol, int nb, int uf, int 
 // block.  Per K step of 32 the 2 x kS slice blocks (40 KB) are loaded into
 // registers two steps ahead (5 x 16 B per thread) and stored to the LDS
 // buffer the next step reads while the current one is computed from the
-// other (two buffers, one barrier per step; row stride 48 B: the 16 B
-// fragment reads of 8 consecutive rows cover all banks).  M is lower
-// triangular, so the K loop stops at r0 + 128.
+// other (two buffers, one barrier per step; rows swizzled by lds_off).  M is
+// lower triangular, so the K loop stops at r0 + 128.
 __global__ __launch_bounds__(512) void k_contract_i8(
     const int8_t* __restrict__ mplanes, const double* __restrict__ mscale,
     const int8_t* __restrict__ xplanes, double* __restrict__ qpart, int np, int mpad, int nb,
     int uf, int uc) {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][kS][kIBM][kILd];
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][kS][kIBM * kIBK];
     const I8Tile tile = i8_tile(blockIdx.x, np / kIBM, mpad / kIBN, nb, uf, uc);
     if (!tile.valid) return;                      // uniform over the workgroup
     const int w = tile.w, rt = tile.rt;
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(512) void k_contract_i8(
     // k half (g % 2) * 16
     const size_t mplane = (size_t)np * np, xplane = (size_t)mpad * np;
     const int8_t* src[5];
-    int lop[5], lsl[5], lrow[5], lkh[5];
+    int lop[5], lsl[5], lrow[5], lkh[5], lofs[5];
 #pragma unroll
     for (int u = 0; u < 5; ++u) {
         const int g = t + 512 * u;
@@ -176,6 +181,7 @@ __global__ __launch_bounds__(512) void k_contract_i8(
         lsl[u] = (g % 1280) / 256;
         lrow[u] = (g % 256) >> 1;
         lkh[u] = (g & 1) * 16;
+        lofs[u] = lds_off(lrow[u], lkh[u]);
         src[u] = lop[u] == 0
                      ? mplanes + ((size_t)w * kS + lsl[u]) * mplane + slice_index(r0 + lrow[u], lkh[u], np)
                      : xplanes + (size_t)lsl[u] * xplane + slice_index(c0 + lrow[u], lkh[u], np);
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(512) void k_contract_i8(
     for (int u = 0; u < 5; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u]);
 #pragma unroll
     for (int u = 0; u < 5; ++u)
-        *reinterpret_cast<u32x4*>(&lds[0][lop[u]][lsl[u]][lrow[u]][lkh[u]]) = pre[u];
+        *reinterpret_cast<u32x4*>(&lds[0][lop[u]][lsl[u]][lofs[u]]) = pre[u];
 #pragma unroll
     for (int u = 0; u < 5; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u] + 4096);
     __syncthreads();
@@ -201,16 +207,18 @@ __global__ __launch_bounds__(512) void k_contract_i8(
             for (int r = 0; r < 16; ++r) acc[g][x][r] = 0;
 
     const int l32 = lane & 31, kg = (lane >> 5) * 16;
+    const int oa = lds_off(wr + l32, kg), ob0 = lds_off(wc + l32, kg),
+              ob1 = lds_off(wc + 32 + l32, kg);
     for (int st = 0; st < nsteps; ++st) {
         const int cur = st & 1;
         u32x4 a[kS];
 #pragma unroll
         for (int i = 0; i < kS; ++i)
-            a[i] = *reinterpret_cast<const u32x4*>(&lds[cur][0][i][wr + l32][kg]);
+            a[i] = *reinterpret_cast<const u32x4*>(&lds[cur][0][i][oa]);
 #pragma unroll
         for (int j = 0; j < kS; ++j) {
-            const u32x4 b0 = *reinterpret_cast<const u32x4*>(&lds[cur][1][j][wc + l32][kg]);
-            const u32x4 b1 = *reinterpret_cast<const u32x4*>(&lds[cur][1][j][wc + 32 + l32][kg]);
+            const u32x4 b0 = *reinterpret_cast<const u32x4*>(&lds[cur][1][j][ob0]);
+            const u32x4 b1 = *reinterpret_cast<const u32x4*>(&lds[cur][1][j][ob1]);
 #pragma unroll
             for (int i = 0; i + j < kS; ++i) {
                 acc[i + j][0] = mfma_i8(a[i], b0, acc[i + j][0]);
@@ -222,7 +230,7 @@ __global__ __launch_bounds__(512) void k_contract_i8(
         // the final stage, in bounds)
 #pragma unroll
         for (int u = 0; u < 5; ++u)
-            *reinterpret_cast<u32x4*>(&lds[cur ^ 1][lop[u]][lsl[u]][lrow[u]][lkh[u]]) = pre[u];
+            *reinterpret_cast<u32x4*>(&lds[cur ^ 1][lop[u]][lsl[u]][lofs[u]]) = pre[u];
         const int kn = min(st + 2, nsteps - 1);
 #pragma unroll
         for (int u = 0; u < 5; ++u)
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(512) void k_contract_i8(
     // epilogue: combine the groups (Horner in 2^-7), undo the row scale, sum
     // the squares of each column over the tile's rows
     __syncthreads();
-    double* red = reinterpret_cast<double*>(&lds[0][0][0][0][0]);   // [4][128] doubles
+    double* red = reinterpret_cast<double*>(&lds[0][0][0][0]);      // [4][128] doubles
     const double* ms = mscale + (size_t)w * np + r0 + wr;
     double colsum[2] = {0.0, 0.0};
 #pragma unroll
