@@ -136,12 +136,14 @@ struct mgw_hip_ctx {
     Buf hA, hS, hdinv, htbuf, hinfo, hpkrow, hqpart, hq, homega, hweight, hmo, hsig_re, hsig_im,
         hsx, hvxc, hstates, hqp, hiters;
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
+    Buf hqim, hY, wY, wW;             // complex path (eta != 0)
+    bool wc_complex = false;
 
     mgw_hip_ctx() {
         all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
-               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale};
+               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -270,8 +272,11 @@ PhLayout ph_layout(int nmo, int nocc) {
     return L;
 }
 
-size_t batch_bytes(int np, int mpad, int precision) {
+size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
     const size_t nblk = np / mgw::kNB;
+    if (cplx)                                // E/L, S, dinv, Y/Z complex; qpart unused
+        return (size_t)np * np * 16 + (size_t)np * 16 + nblk * 64 * 64 * 16 +
+               (size_t)np * mpad * 16 + 4;
     size_t b = (size_t)np * np * 8           // A
                + (size_t)np * 8              // S
                + nblk * 64 * 64 * 8          // dinv
@@ -444,15 +449,36 @@ int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0
         need(ctx && pi0_re, "mgw_hip_compute_wc: NULL argument");
         if (!ctx->v_ready) fail(MGW_ERR_STATE, "compute_w_c before V_ph was uploaded");
         need(n == ctx->n, "DeviceScreeningWorkspace::compute_w_c: inconsistent pi0 dimension");
+        bool cplx = false;
         for (int i = 0; i < n; ++i) {
-            if (pi0_im && pi0_im[i] != 0.0)
-                fail(MGW_ERR_UNSUPPORTED,
-                     "compute_w_c: complex Pi0 (eta != 0) is not supported by the hip backend");
+            if (pi0_im && pi0_im[i] != 0.0) cplx = true;
             if (!(pi0_re[i] <= 0.0))
                 fail(MGW_ERR_UNSUPPORTED,
                      "compute_w_c: Pi0[%d] = %g > 0 (non-positive particle-hole gap)", i, pi0_re[i]);
         }
         const int np = ctx->np;
+        ctx->wc_ready = false;
+        if (cplx) {
+            // complex symmetric E = I + S V S, S = sqrt(-Pi0): L L^T and the
+            // diagonal-block inverses stay resident for the quadratic forms
+            double* A = as<double>(ctx->get(ctx->wA, (size_t)np * np * 16, "dielectric"));
+            double* S = as<double>(ctx->get(ctx->wS, (size_t)np * 16 + (size_t)np * 16, "scale"));
+            double* dinv = as<double>(ctx->get(ctx->wdinv, (size_t)np * 64 * 16, "dinv"));
+            int* info = as<int>(ctx->get(ctx->winfo, sizeof(int), "info"));
+            double* pre = S + 2 * np;
+            double* pim = pre + np;
+            ctx->h2d(pre, pi0_re, n);
+            ctx->h2d(pim, pi0_im, n);
+            hip_check(hipMemsetAsync(info, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
+            hip_check(mgw::launch_scale_pi0_c(pre, pim, S, n, np, ctx->stream), "scale (complex)");
+            hip_check(mgw::launch_dielectric_c(as<double>(ctx->v.p), S, A, np, 1, ctx->stream),
+                      "dielectric (complex)");
+            hip_check(mgw::launch_potrf_c(A, dinv, info, np, 1, ctx->stream), "potrf (complex)");
+            check_info(ctx, info, 1, "compute_w_c");
+            ctx->wc_complex = true;
+            ctx->wc_ready = true;
+            return;
+        }
         double* A = as<double>(ctx->get(ctx->wA, (size_t)np * np * 8, "dielectric"));
         double* S = as<double>(ctx->get(ctx->wS, (size_t)np * 8 * 2, "scale"));
         double* dinv = as<double>(ctx->get(ctx->wdinv, (size_t)np * 64 * 8, "dinv"));
@@ -466,6 +492,7 @@ int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0
                   "dielectric");
         factor_batch(ctx, A, dinv, info, tbuf, np, 1);
         check_info(ctx, info, 1, "compute_w_c");
+        ctx->wc_complex = false;
         ctx->wc_ready = true;
     });
 }
@@ -475,6 +502,22 @@ int mgw_hip_download_wc(mgw_hip_ctx* ctx, double* wc_re, double* wc_im, int32_t 
         need(ctx && wc_re, "mgw_hip_download_wc: NULL argument");
         if (!ctx->wc_ready) fail(MGW_ERR_STATE, "W_c requested before compute_w_c");
         need(n == ctx->n, "mgw_hip_download_wc: n = %d, workspace has %d", n, ctx->n);
+        if (ctx->wc_complex) {
+            const int np = ctx->np;
+            double* wc = as<double>(ctx->get(ctx->wsrc, (size_t)n * n * 16, "W_c staging"));
+            double* Y = as<double>(ctx->get(ctx->wY, (size_t)np * np * 16, "complex scratch"));
+            double* W = as<double>(ctx->get(ctx->wW, (size_t)np * np * 16, "complex W_c"));
+            hip_check(mgw::launch_wc_explicit_c(as<double>(ctx->wA.p), as<double>(ctx->wdinv.p),
+                                                as<double>(ctx->wS.p), Y, W, wc, wc + (size_t)n * n, n,
+                                                np, ctx->stream),
+                      "W_c (complex)");
+            ctx->d2h(wc_re, wc, (size_t)n * n);
+            if (wc_im) ctx->d2h(wc_im, wc + (size_t)n * n, (size_t)n * n);
+            ctx->sync();
+            ctx->release(ctx->wY);
+            ctx->release(ctx->wW);
+            return;
+        }
         double* wc = as<double>(ctx->get(ctx->wsrc, (size_t)n * n * 8, "W_c staging"));
         hip_check(mgw::launch_wc_explicit(as<double>(ctx->wA.p), as<double>(ctx->wS.p), wc, n,
                                           ctx->np, ctx->stream),
@@ -502,11 +545,24 @@ void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow,
     ctx->h2d(dcol, hcolidx, np);
     double* xp = as<double>(ctx->get(ctx->wxp, (size_t)mpad * np * 8, "packed panel"));
     hip_check(mgw::launch_pack_x(src, dpk, dcol, xp, ctx->n, np, mpad, ctx->stream), "pack X");
+    int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
+    hip_check(mgw::launch_set_int(f0p, 0, ctx->stream), "set offset");
+    if (ctx->wc_complex) {
+        double* Y = as<double>(ctx->get(ctx->wY, (size_t)np * mpad * 16, "complex panel"));
+        double* qi = as<double>(ctx->get(ctx->wW, (size_t)mpad * 8, "q imag"));
+        hip_check(mgw::launch_make_y(as<double>(ctx->wS.p), xp, Y, np, mpad, 1, ctx->stream), "S X");
+        hip_check(mgw::launch_fsub_c(as<double>(ctx->wA.p), as<double>(ctx->wdinv.p), Y, np, mpad, 1,
+                                     ctx->stream),
+                  "L^-1 S X");
+        hip_check(mgw::launch_sumsq_c(Y, nullptr, q, qi, np, m, mpad, f0p, 1, ctx->stream), "q");
+        ctx->d2h(q_re, q, m);
+        if (q_im) ctx->d2h(q_im, qi, m);
+        ctx->sync();
+        return;
+    }
     hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), xp, qpart, np,
                                        mpad, 1, ctx->stream),
               "contract");
-    int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
-    hip_check(mgw::launch_set_int(f0p, 0, ctx->stream), "set offset");
     hip_check(mgw::launch_reduce_q(qpart, nullptr, q, np / mgw::kCT, m, mpad, f0p, 1, ctx->stream),
               "reduce q");
     ctx->d2h(q_re, q, m);
@@ -652,8 +708,10 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
         need(job->mo_energy && job->vxc_diag, "mgw_hip_g0w0: mo_energy / vxc_diag are NULL");
         need(out->qp != nullptr, "mgw_hip_g0w0: out->qp is NULL");
-        if (job->eta != 0.0)
-            fail(MGW_ERR_UNSUPPORTED, "eta != 0 (complex screening) is not supported by the hip backend");
+        const bool cplx = job->eta != 0.0;
+        if (cplx && job->precision != MGW_PREC_FP64)
+            fail(MGW_ERR_UNSUPPORTED,
+                 "eta != 0 (complex screening) runs in MGW_PREC_FP64 only on the hip backend");
         need(job->precision == MGW_PREC_FP64 || job->precision == MGW_PREC_MIXED ||
                  job->precision == MGW_PREC_MIXED_INT8,
              "mgw_hip_g0w0: unknown precision %d", job->precision);
@@ -729,6 +787,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         int* dstates = as<int>(ctx->get(ctx->hstates, (size_t)ns * 4, "states"));
         int64_t* dpk = as<int64_t>(ctx->get(ctx->hpkrow, (size_t)mpad * 8, "pkrow"));
         double* dq = as<double>(ctx->get(ctx->hq, (size_t)nfreq * m * 8, "q"));
+        double* dqi = cplx ? as<double>(ctx->get(ctx->hqim, (size_t)nfreq * m * 8, "q imag")) : nullptr;
         ctx->h2d(dom, omega.data(), nfreq);
         ctx->h2d(dwt, weight.data(), nfreq);
         ctx->h2d(dmo, job->mo_energy, nmo);
@@ -781,14 +840,17 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         size_t room = free_b > (size_t)(2ull << 30) ? free_b - (size_t)(2ull << 30) : free_b / 2;
         if (ctx->budget) room = std::min(room, ctx->budget > ctx->held ? ctx->budget - ctx->held : 0);
-        const size_t per = batch_bytes(np, mpad, job->precision);
+        const size_t per = batch_bytes(np, mpad, job->precision, cplx);
         int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
         nb = std::min(nb, 64);
         const int pick = ctx->forced_nb ? ctx->forced_nb : tuned_batch(nmo, nocc, ns, job->precision);
         if (pick > 0) nb = std::min(nb, pick);
-        double* A = as<double>(ctx->get(ctx->hA, (size_t)nb * np * np * 8, "dielectric batch"));
-        double* S = as<double>(ctx->get(ctx->hS, (size_t)nb * np * 8, "scale batch"));
-        double* dinv = as<double>(ctx->get(ctx->hdinv, (size_t)nb * nblk * 64 * 64 * 8, "dinv"));
+        const size_t zf = cplx ? 2 : 1;    // complex entries are two doubles
+        double* A = as<double>(ctx->get(ctx->hA, zf * nb * np * np * 8, "dielectric batch"));
+        double* S = as<double>(ctx->get(ctx->hS, zf * nb * np * 8, "scale batch"));
+        double* dinv = as<double>(ctx->get(ctx->hdinv, zf * nb * nblk * 64 * 64 * 8, "dinv"));
+        double* Y = cplx ? as<double>(ctx->get(ctx->hY, (size_t)nb * np * mpad * 16, "complex panel"))
+                         : nullptr;
         double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * np * 64 * 8, "trtri scratch"));
         int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
         int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
@@ -811,6 +873,23 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         int n_contr = 0;
         // One frequency batch: kernels only, so that it can also be captured.
         auto enqueue_batch = [&](int b, bool timed) {
+            if (cplx) {
+                if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+                hip_check(mgw::launch_scale_grid_c(ddelta, dom, job->eta, S, n, np, f0p, b, st),
+                          "scale (complex)");
+                hip_check(mgw::launch_dielectric_c(dV, S, A, np, b, st), "dielectric (complex)");
+                if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+                hip_check(mgw::launch_potrf_c(A, dinv, info, np, b, st), "potrf (complex)");
+                if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
+                hip_check(mgw::launch_make_y(S, xp, Y, np, mpad, b, st), "S X (complex)");
+                if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
+                hip_check(mgw::launch_fsub_c(A, dinv, Y, np, mpad, b, st), "L^-1 S X (complex)");
+                if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
+                hip_check(mgw::launch_sumsq_c(Y, dcolmap, dq, dqi, np, m, mpad, f0p, b, st),
+                          "q (complex)");
+                if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
+                return;
+            }
             if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
             hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
             hip_check(mgw::launch_dielectric(dV, S, A, np, b, st), "dielectric");
@@ -849,7 +928,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         for (int f0 = 0; f0 < nfreq; f0 += nb) {
             const int b = std::min(nb, nfreq - f0);
             hip_check(mgw::launch_set_int(f0p, f0, st), "set offset");
-            if (ctx->use_graphs) {
+            if (ctx->use_graphs && !cplx) {
                 // Replay a graph of this batch when one with the same shape and
                 // buffers was captured before (the offset is read on the device).
                 const GraphKey key{np, mpad, m, b, job->precision,
@@ -900,7 +979,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         double* sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
         double* qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
         int* iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
-        hip_check(mgw::launch_sigma_c(dq, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, sre, sim,
+        hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, sre, sim,
                                       st),
                   "sigma_c");
         hip_check(hipEventRecord(ev[3], st), "hipEventRecord");

@@ -125,9 +125,9 @@ hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int 
                               hipStream_t s);
 
 // sigma[s][n] = (1/pi) sum_f sum_k -(g0(k, n, f) q[f][s*nmo + k]) w_f
-hipError_t launch_sigma_c(const double* q, const double* mo_energy, const double* omega,
-                          const double* weight, double fermi, int nmo, int nstates, int nfreq,
-                          double* sig_re, double* sig_im, hipStream_t s);
+hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_energy,
+                          const double* omega, const double* weight, double fermi, int nmo,
+                          int nstates, int nfreq, double* sig_re, double* sig_im, hipStream_t s);
 
 // sx[p] = -sum_{i < nocc} eri[p, i, p, i]
 hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s);
@@ -158,5 +158,33 @@ hipError_t launch_contract_i8(const int8_t* mplanes, const double* mscale, const
                               double* qpart, int np, int mpad, int nb, int uf, hipStream_t s);
 int contract_i8_row_tiles(int np);
 hipError_t launch_probe_i8(const int8_t* a, const int8_t* b, int* d, int ntests, hipStream_t s);
+
+// Complex path for eta != 0 (mgw_complex.hip).  Complex arrays are
+// interleaved (re, im) doubles; matrices row-major np x np, panels np x mpad.
+// S_w = sqrt(-Pi0(omega[*f0p + w], delta, eta)), 0 in the padding.
+hipError_t launch_scale_grid_c(const double* delta, const double* omega, double eta, double* S,
+                               int n, int np, const int* f0p, int nb, hipStream_t s);
+// S = sqrt(-pi0) for a host-given complex Pi0 (pim may be NULL).
+hipError_t launch_scale_pi0_c(const double* pre, const double* pim, double* S, int n, int np,
+                              hipStream_t s);
+// Lower triangle of E_w = I + S_w V S_w.
+hipError_t launch_dielectric_c(const double* V, const double* S, double* E, int np, int nb,
+                               hipStream_t s);
+// In place E_w = L_w L_w^T (no conjugate, lower); dinv: inverses of the 64 x 64
+// diagonal blocks (nb x np/64 x 64 x 64); info[w] = first bad pivot (1-based).
+hipError_t launch_potrf_c(double* E, double* dinv, int* info, int np, int nb, hipStream_t s);
+// Y_w = diag(S_w) X with X packed per column (xp[c * np + r], real).
+hipError_t launch_make_y(const double* S, const double* xp, double* Y, int np, int mpad, int nb,
+                         hipStream_t s);
+// In place Y_w <- L_w^-1 Y_w (blocked forward substitution).
+hipError_t launch_fsub_c(const double* L, const double* dinv, double* Y, int np, int mpad, int nb,
+                         hipStream_t s);
+// q[*f0p + w][c] = -sum_r Z_w[r][colmap[c]]^2 into (qre, qim).
+hipError_t launch_sumsq_c(const double* Z, const int* colmap, double* qre, double* qim, int np,
+                          int m, int mpad, const int* f0p, int nb, hipStream_t s);
+// W_c = -(L^-1 S)^T (L^-1 S), first n rows / columns into re / im (n x n);
+// Y and W: np x np complex scratch.
+hipError_t launch_wc_explicit_c(const double* L, const double* dinv, const double* S, double* Y,
+                                double* W, double* re, double* im, int n, int np, hipStream_t s);
 
 }  // namespace mgw

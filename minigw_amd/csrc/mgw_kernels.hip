@@ -492,7 +492,8 @@ __device__ inline cplx cdiv(cplx a, cplx b) {
 }
 __device__ inline double cabs_(cplx a) { return hypot(a.re, a.im); }
 
-__global__ void k_sigma_c(const double* __restrict__ q, const double* __restrict__ mo,
+__global__ void k_sigma_c(const double* __restrict__ q, const double* __restrict__ qim,
+                          const double* __restrict__ mo,
                           const double* __restrict__ omega, const double* __restrict__ weight,
                           double fermi, int nmo, int nstates, int nfreq, double* __restrict__ sre,
                           double* __restrict__ sim) {
@@ -511,8 +512,14 @@ __global__ void k_sigma_c(const double* __restrict__ q, const double* __restrict
             const cplx den{dd.re + wp * wp, dd.im};
             const cplx g = cdiv(d, den);
             const double qk = qf[k];
-            are -= (g.re * qk) * wf;
-            aim -= (g.im * qk) * wf;
+            if (qim) {      // complex q (eta != 0): (g q) w
+                const double qi = qim[(size_t)f * m + (size_t)s * nmo + k];
+                are -= (g.re * qk - g.im * qi) * wf;
+                aim -= (g.re * qi + g.im * qk) * wf;
+            } else {
+                are -= (g.re * qk) * wf;
+                aim -= (g.im * qk) * wf;
+            }
         }
     }
     const double pi = 3.141592653589793238462643383279502884;
@@ -778,12 +785,12 @@ hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int 
     return hipGetLastError();
 }
 
-hipError_t launch_sigma_c(const double* q, const double* mo_energy, const double* omega,
-                          const double* weight, double fermi, int nmo, int nstates, int nfreq,
-                          double* sig_re, double* sig_im, hipStream_t s) {
+hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_energy,
+                          const double* omega, const double* weight, double fermi, int nmo,
+                          int nstates, int nfreq, double* sig_re, double* sig_im, hipStream_t s) {
     const int total = nstates * nfreq;
-    hipLaunchKernelGGL(k_sigma_c, dim3((total + 127) / 128), dim3(128), 0, s, q, mo_energy, omega,
-                       weight, fermi, nmo, nstates, nfreq, sig_re, sig_im);
+    hipLaunchKernelGGL(k_sigma_c, dim3((total + 127) / 128), dim3(128), 0, s, q, q_im, mo_energy,
+                       omega, weight, fermi, nmo, nstates, nfreq, sig_re, sig_im);
     return hipGetLastError();
 }
 

@@ -274,16 +274,18 @@ class HipScreeningWorkspace:
         self.ctx.call("mgw_hip_compute_wc", _ptr(re), _ptr(im), re.shape[0])
 
     def w_c(self) -> np.ndarray:
-        out = np.zeros((self._n, self._n))
-        self.ctx.call("mgw_hip_download_wc", _ptr(out), None, self._n)
-        return out
+        """W_c (n x n, complex128; imaginary part zero at eta = 0)."""
+        re = np.zeros((self._n, self._n))
+        im = np.zeros((self._n, self._n))
+        self.ctx.call("mgw_hip_download_wc", _ptr(re), _ptr(im), self._n)
+        return re + 1j * im
 
     def quadratic_forms_panel(self, x_panel: np.ndarray) -> np.ndarray:
         x = _f64(x_panel)
         n, nvec = x.shape
-        q = np.zeros(nvec)
-        self.ctx.call("mgw_hip_quadratic_forms_host", _ptr(x), n, nvec, _ptr(q), None)
-        return q.astype(np.complex128)
+        q, qi = np.zeros(nvec), np.zeros(nvec)
+        self.ctx.call("mgw_hip_quadratic_forms_host", _ptr(x), n, nvec, _ptr(q), _ptr(qi))
+        return q + 1j * qi
 
 
 class HipPqPhPanelView:
@@ -303,9 +305,9 @@ class HipPqPhPanelView:
         self.ctx.call("mgw_hip_build_vph_from_eri")
 
     def quadratic_forms(self, p: int, k0: int, width: int) -> np.ndarray:
-        q = np.zeros(max(width, 0))
-        self.ctx.call("mgw_hip_quadratic_forms", p, k0, width, _ptr(q), None)
-        return q.astype(np.complex128)
+        q, qi = np.zeros(max(width, 0)), np.zeros(max(width, 0))
+        self.ctx.call("mgw_hip_quadratic_forms", p, k0, width, _ptr(q), _ptr(qi))
+        return q + 1j * qi
 
 
 @dataclass

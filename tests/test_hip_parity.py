@@ -87,6 +87,38 @@ def test_compute_wc_matches_oracle(hip, ctx, nmo, nocc, seed):
         assert np.all(q.imag == 0)
 
 
+@pytest.mark.parametrize("eta", [0.01, 0.2])
+def test_compute_wc_complex_matches_oracle(hip, ctx, eta):
+    """eta != 0: complex Pi0, W_c = eps^-1 diag(Pi0) against the oracle's
+    complex getrf/getrs, and the complex quadratic forms (no conjugate)."""
+    say(f"case: compute_w_c complex, eta={eta}, nmo=24 and 40")
+    for nmo, nocc, seed in [(24, 5, 20240601), (40, 9, 3)]:
+        inp = synthetic_input(nmo, nocc, seed)
+        ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+        v = orc.v_ph_matrix(inp.eri, ph)
+        ws = hip.HipScreeningWorkspace(v, ctx)
+        for w in [1e-12, 0.37, 25.0]:
+            pi0 = orc.pi0_diag(w, ph.delta, eta)
+            ws.compute_w_c(pi0)
+            want = orc.compute_w_c(v, pi0)
+            got = ws.w_c()
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < 1e-12, (nmo, w, err)
+            rng = np.random.default_rng(2)
+            x = rng.standard_normal((ph.size, 70))
+            q = ws.quadratic_forms_panel(x)
+            qw = orc.quadratic_forms_panel(want, x)
+            assert np.abs(q - qw).max() <= 1e-11 * np.abs(qw).max(), (nmo, w)
+        view = hip.HipPqPhPanelView(inp.eri, inp.nocc, ctx)
+        pi0 = orc.pi0_diag(0.8, ph.delta, eta)
+        ws.compute_w_c(pi0)
+        wc = orc.compute_w_c(v, pi0)
+        for p, k0, width in [(0, 0, nmo), (3, 2, 5)]:
+            q = view.quadratic_forms(p, k0, width)
+            want = orc.quadratic_forms_panel(wc, orc.pq_panel(inp.eri, ph, p, k0, width))
+            assert np.abs(q - want).max() <= 1e-11 * np.abs(want).max(), (p, k0, width)
+
+
 def test_eri_panels_match_oracle(hip, ctx):
     say("case: ERI-resident panels nmo=24")
     inp = synthetic_input(24, 5, 11)
@@ -117,9 +149,9 @@ def test_boundary_errors(hip, ctx):
     with pytest.raises(hip.HipError) as e:
         ws.compute_w_c(np.full(4, -0.5))
     assert e.value.code == hip.MGW_ERR_ARG
-    with pytest.raises(hip.HipError) as e:
-        ws.compute_w_c(np.full(5, -0.5) + 0.01j)
-    assert e.value.code == hip.MGW_ERR_UNSUPPORTED
+    p0 = -0.5 + 0.01j                          # n = 1 closed form per diagonal entry
+    ws.compute_w_c(np.full(5, p0))
+    assert np.allclose(ws.w_c(), np.eye(5) * (p0 / (1 - 0.3 * p0)), atol=1e-15)
     ws.compute_w_c(np.full(5, -0.5))
     assert np.allclose(ws.w_c(), np.eye(5) * (-0.5 / (1 + 0.5 * 0.3)), atol=1e-15)
 
@@ -141,11 +173,12 @@ def test_h2o_state5_fixture(hip, ctx):
     assert np.isnan(res.qp[0]) and res.qp_iterations[4] > 0
 
 
-@pytest.mark.parametrize("name", ["h2o_all", "syn_c1", "syn_small", "syn_odd"])
+@pytest.mark.parametrize("name", ["h2o_all", "syn_c1", "syn_small", "syn_odd", "h2o_eta",
+                                  "syn_small_eta"])
 def test_all_states_vs_reference_path(hip, ctx, name):
     r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
     say(f"case: all states vs reference path, {name}")
-    if name == "h2o_all":
+    if name in ("h2o_all", "h2o_eta"):
         inp = h2o_input()
     elif name == "syn_c1":
         inp = synthetic_input(*CONFIGS["c1"])
@@ -153,7 +186,7 @@ def test_all_states_vs_reference_path(hip, ctx, name):
         inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
                       fermi_energy=float(r["fermi_energy"]))
     nf, npar = int(r["nfreq"]), int(r["npar"])
-    res = hip.run_g0w0(inp, nf, npar, ctx=ctx)
+    res = hip.run_g0w0(inp, nf, npar, eta=float(r["eta"]), ctx=ctx)
     assert np.array_equal(res.omegas, r["omega"])
     assert np.allclose(res.sigma_x_diag, r["sigma_x_diag"], rtol=1e-14, atol=1e-15)
     dsig = np.abs(res.sigma_c - r["sigma_c"]).max()
@@ -199,12 +232,13 @@ def test_pade_qp_entry_point_matches_oracle(hip, ctx):
     assert e.value.code == hip.MGW_ERR_ARG
 
 
-@pytest.mark.parametrize("nmo,nocc,nfreq", [(60, 12, 12), (90, 18, 6)])
-def test_multiblock_vs_oracle(hip, ctx, nmo, nocc, nfreq):
-    say(f"case: multi-block factorization nmo={nmo} nocc={nocc} nfreq={nfreq}")
+@pytest.mark.parametrize("nmo,nocc,nfreq,eta", [(60, 12, 12, 0.0), (90, 18, 6, 0.0),
+                                                 (60, 12, 8, 0.03)])
+def test_multiblock_vs_oracle(hip, ctx, nmo, nocc, nfreq, eta):
+    say(f"case: multi-block factorization nmo={nmo} nocc={nocc} nfreq={nfreq} eta={eta}")
     inp = synthetic_input(nmo, nocc, 99)
-    res = hip.run_g0w0(inp, nfreq, 6, ctx=ctx)
-    want = orc.run_g0w0(inp, nfreq, 6)
+    res = hip.run_g0w0(inp, nfreq, 6, eta=eta, ctx=ctx)
+    want = orc.run_g0w0(inp, nfreq, 6, eta=eta)
     d = np.abs(res.sigma_c - want.sigma_c).max() / np.abs(want.sigma_c).max()
     say(f"   rel |dSigma_c| = {d:.3e}")
     assert d < 1e-11
@@ -218,7 +252,7 @@ def test_subset_states_and_errors(hip, ctx):
     assert np.allclose(sub.sigma_c, full.sigma_c[[2, 9, 15]], rtol=1e-13, atol=1e-15)
     assert np.isnan(sub.qp[0]) and not np.isnan(sub.qp[9])
     with pytest.raises(hip.HipError) as e:
-        hip.run_g0w0(inp, 24, 8, eta=0.01, ctx=ctx, upload=False)
+        hip.run_g0w0(inp, 24, 8, eta=0.01, ctx=ctx, upload=False, precision=hip.PREC_MIXED)
     assert e.value.code == hip.MGW_ERR_UNSUPPORTED
     with pytest.raises(hip.HipError) as e:
         hip.run_g0w0(inp, 24, 8, states=[3, 3], ctx=ctx, upload=False)
