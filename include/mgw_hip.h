@@ -90,10 +90,12 @@ int mgw_hip_set_memory_budget(mgw_hip_ctx* ctx, size_t bytes);
  * (src/workspace/device_screening_workspace.cu:289-310). */
 int mgw_hip_upload_vph(mgw_hip_ctx* ctx, const double* v_rowmajor, int32_t n);
 
-/* W_c = (I - diag(pi0) V)^-1 diag(pi0), kept resident; replaces
- * DeviceScreeningWorkspace::compute_w_c (src/workspace/device_screening_workspace.cu:337-374).
- * pi0_im may be NULL (all zero).  This backend requires Im(pi0) == 0 and
- * pi0 <= 0 (eta == 0 with positive ph gaps); otherwise MGW_ERR_UNSUPPORTED. */
+/* W_c = (I - diag(pi0) V)^-1 diag(pi0), kept resident (as the factor of
+ * I + S V S, S = sqrt(-pi0)); replaces DeviceScreeningWorkspace::compute_w_c
+ * (src/workspace/device_screening_workspace.cu:337-374).  pi0_im may be NULL
+ * (all zero, eta == 0: real FP64 MFMA path); a non-zero imaginary part
+ * (eta != 0) takes the complex FP64 path.  Re(pi0) > 0 (a non-positive ph
+ * gap) is MGW_ERR_UNSUPPORTED; a zero pivot is MGW_ERR_NUMERIC. */
 int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0_im, int32_t n);
 
 /* Copy the resident W_c (n x n, row-major) to the host.  wc_im may be NULL. */
@@ -148,7 +150,8 @@ typedef struct mgw_g0w0_job {
     const double* mo_energy;    /* nmo                                        */
     const double* vxc_diag;     /* nmo: vxc(p, p)                             */
     double fermi_energy;
-    double eta;                 /* must be 0 (MGW_ERR_UNSUPPORTED otherwise)  */
+    double eta;                 /* broadening; != 0 runs the complex path,    */
+                                /* MGW_PREC_FP64 only (else UNSUPPORTED)      */
     int32_t precision;          /* enum mgw_precision                         */
 } mgw_g0w0_job;
 
