@@ -37,7 +37,8 @@ enum mgw_status {
     MGW_ERR_NUMERIC = 3,     /* factorization breakdown (non-SPD dielectric)      */
     MGW_ERR_UNSUPPORTED = 4, /* valid request this backend does not implement     */
     MGW_ERR_MEMORY = 5,      /* device allocation failed / over the HBM budget    */
-    MGW_ERR_STATE = 6        /* call order (e.g. quadratic forms before compute_wc) */
+    MGW_ERR_STATE = 6,       /* call order (e.g. quadratic forms before compute_wc) */
+    MGW_ERR_IO = 7           /* file missing, unreadable or malformed (mgw_io.h)  */
 };
 
 enum mgw_precision {
@@ -180,6 +181,16 @@ int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_
                     int32_t nfreq, int32_t npar, const double* omega, const double* sigma_re,
                     const double* sigma_im, const double* sigma_x_diag, const double* mo_energy,
                     const double* vxc_diag, double fermi_energy, double* qp, int32_t* iterations);
+
+/* Real-axis continuation of one state's Sigma_c(i w) (sigma_re / sigma_im:
+ * nfreq values on the imaginary frequencies omega): the Thiele Pade of
+ * mgw_hip_pade_qp evaluated at e_i = x_start + i * interval - fermi_energy,
+ * i < npoints, into out_re / out_im; the values of E_c.out
+ * (output_self_energy_after_pade, src/io.cpp:94-108). */
+int mgw_hip_pade_real_axis(mgw_hip_ctx* ctx, const double* sigma_re, const double* sigma_im,
+                           const double* omega, int32_t nfreq, int32_t npar, double fermi_energy,
+                           double x_start, double interval, int32_t npoints, double* out_re,
+                           double* out_im);
 
 /* ---- request loop (continuous batching; no counterpart in the reference,
  *      which runs one molecule per process) */

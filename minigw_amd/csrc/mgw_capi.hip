@@ -1068,6 +1068,31 @@ int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_
     });
 }
 
+int mgw_hip_pade_real_axis(mgw_hip_ctx* ctx, const double* sigma_re, const double* sigma_im,
+                           const double* omega, int32_t nfreq, int32_t npar, double fermi_energy,
+                           double x_start, double interval, int32_t npoints, double* out_re,
+                           double* out_im) {
+    return guarded(ctx, [&] {
+        need(ctx && sigma_re && sigma_im && omega && out_re && out_im,
+             "mgw_hip_pade_real_axis: NULL argument");
+        need(nfreq > 0 && npoints >= 0, "mgw_hip_pade_real_axis: nfreq must be positive");
+        need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
+        if (npoints == 0) return;
+        double* d = as<double>(ctx->get(ctx->wsrc, (size_t)(3 * nfreq + 2 * npoints) * 8, "pade"));
+        ctx->h2d(d, sigma_re, nfreq);
+        ctx->h2d(d + nfreq, sigma_im, nfreq);
+        ctx->h2d(d + 2 * nfreq, omega, nfreq);
+        double* ore = d + 3 * nfreq;
+        double* oim = ore + npoints;
+        hip_check(mgw::launch_pade_real_axis(d, d + nfreq, d + 2 * nfreq, nfreq, npar, fermi_energy,
+                                             x_start, interval, npoints, ore, oim, ctx->stream),
+                  "pade real axis");
+        ctx->d2h(out_re, ore, npoints);
+        ctx->d2h(out_im, oim, npoints);
+        ctx->sync();
+    });
+}
+
 int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, double* c) {
     return guarded(ctx, [&] {
         need(ctx && a && b && c, "mgw_hip_selftest_tile: NULL argument");

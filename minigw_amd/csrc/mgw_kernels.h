@@ -133,6 +133,10 @@ hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_
 hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s);
 
 // Pade fit + QP fixed point per state (FP64, no FMA contraction).
+// Thiele Pade of one state's row evaluated at x_start + i * interval - fermi.
+hipError_t launch_pade_real_axis(const double* sig_re, const double* sig_im, const double* omega,
+                                 int nfreq, int npar, double fermi, double x_start, double interval,
+                                 int npoints, double* out_re, double* out_im, hipStream_t s);
 hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const double* omega,
                           const double* mo_energy, const double* sx, const double* vxc_diag,
                           const int* states, double fermi, int nstates, int nfreq, int npar,

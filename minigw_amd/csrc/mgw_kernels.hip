@@ -537,19 +537,15 @@ __global__ void k_sigma_x(const double* __restrict__ eri, double* __restrict__ s
     sx[p] = v;
 }
 
-// Thiele continued-fraction Pade (src/gw/pade.cpp:10-101) + QP fixed point
-// (src/gw/qp_solver.cpp:10-47), one thread per state.
-__global__ void k_pade_qp(const double* __restrict__ sre, const double* __restrict__ sim,
-                          const double* __restrict__ omega, const double* __restrict__ mo,
-                          const double* __restrict__ sx, const double* __restrict__ vxc,
-                          const int* __restrict__ states, double fermi, int nstates, int nfreq,
-                          int npar, double* __restrict__ qp, int* __restrict__ iters) {
+// Thiele continued-fraction Pade (src/gw/pade.cpp:10-70): coefficients of
+// one state's Sigma_c row from npar sampled points (0, s, 2s, ..., nfreq-1),
+// reciprocal differences with the 1e-20 guard.  Returns the number of points.
+__device__ int pade_fit(const double* __restrict__ row_re, const double* __restrict__ row_im,
+                        const double* __restrict__ omega, int nfreq, int npar, cplx* xs,
+                        cplx* coef) {
 #pragma clang fp contract(off)
-    const int si = blockIdx.x * blockDim.x + threadIdx.x;
-    if (si >= nstates) return;
-    const int state = states[si];
     const int npa = npar < nfreq ? npar : nfreq;
-    cplx xs[kMaxPade], g[kMaxPade], coef[kMaxPade];
+    cplx g[kMaxPade];
     const int step = npa > 1 ? nfreq / (npa - 1) : 0;
     for (int i = 0; i < npa; ++i) {
         int idx;
@@ -557,7 +553,7 @@ __global__ void k_pade_qp(const double* __restrict__ sre, const double* __restri
         else if (step == 0) idx = i;
         else idx = (i < npa - 1) ? i * step : nfreq - 1;
         xs[i] = cplx{0.0, omega[idx]};
-        g[i] = cplx{sre[(size_t)si * nfreq + idx], sim[(size_t)si * nfreq + idx]};
+        g[i] = cplx{row_re[idx], row_im[idx]};
     }
     coef[0] = g[0];
     for (int ip = 1; ip < npa; ++ip) {
@@ -570,24 +566,42 @@ __global__ void k_pade_qp(const double* __restrict__ sre, const double* __restri
         }
         coef[ip] = g[ip];
     }
+    return npa;
+}
+
+// Backward evaluation of the continued fraction at a real z
+// (src/gw/pade.cpp:72-101), 1e-20 guards.
+__device__ cplx pade_eval(const cplx* coef, const cplx* xs, int npa, double z) {
+#pragma clang fp contract(off)
+    if (npa == 1) return coef[0];
+    cplx val{1.0, 0.0};
+    for (int k = npa - 2; k >= 0; --k) {
+        const cplx num = cmul(coef[k + 1], cplx{z - xs[k].re, 0.0 - xs[k].im});
+        if (cabs_(val) < 1e-20) val = cplx{1e-20, 0.0};
+        const cplx qv = cdiv(num, val);
+        val = cplx{1.0 + qv.re, 0.0 + qv.im};
+    }
+    return (cabs_(val) < 1e-20) ? cdiv(coef[0], cplx{1e-20, 0.0}) : cdiv(coef[0], val);
+}
+
+// Pade + QP fixed point (src/gw/qp_solver.cpp:10-47), one thread per state.
+__global__ void k_pade_qp(const double* __restrict__ sre, const double* __restrict__ sim,
+                          const double* __restrict__ omega, const double* __restrict__ mo,
+                          const double* __restrict__ sx, const double* __restrict__ vxc,
+                          const int* __restrict__ states, double fermi, int nstates, int nfreq,
+                          int npar, double* __restrict__ qp, int* __restrict__ iters) {
+#pragma clang fp contract(off)
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= nstates) return;
+    const int state = states[si];
+    cplx xs[kMaxPade], coef[kMaxPade];
+    const int npa = pade_fit(sre + (size_t)si * nfreq, sim + (size_t)si * nfreq, omega, nfreq, npar,
+                             xs, coef);
     const double e0 = mo[state];
     double cur = e0;
     int used = 200;
     for (int it = 0; it < 200; ++it) {
-        const double z = cur - fermi;
-        cplx val{1.0, 0.0};
-        cplx res;
-        if (npa == 1) {
-            res = coef[0];
-        } else {
-            for (int k = npa - 2; k >= 0; --k) {
-                const cplx num = cmul(coef[k + 1], cplx{z - xs[k].re, 0.0 - xs[k].im});
-                if (cabs_(val) < 1e-20) val = cplx{1e-20, 0.0};
-                const cplx qv = cdiv(num, val);
-                val = cplx{1.0 + qv.re, 0.0 + qv.im};
-            }
-            res = (cabs_(val) < 1e-20) ? cdiv(coef[0], cplx{1e-20, 0.0}) : cdiv(coef[0], val);
-        }
+        const cplx res = pade_eval(coef, xs, npa, cur - fermi);
         const double nq = e0 + res.re + sx[state] - vxc[state];
         if (fabs(nq - cur) < 1e-6) {
             cur = nq;
@@ -598,6 +612,23 @@ __global__ void k_pade_qp(const double* __restrict__ sre, const double* __restri
     }
     qp[state] = cur;
     iters[state] = used;
+}
+
+// Real-axis continuation of one state's Sigma_c (the values of E_c.out,
+// src/io.cpp:94-108): point i at e_i = x_start + i * interval - fermi.
+__global__ void k_pade_real_axis(const double* __restrict__ sre, const double* __restrict__ sim,
+                                 const double* __restrict__ omega, int nfreq, int npar,
+                                 double fermi, double x_start, double interval, int npoints,
+                                 double* __restrict__ out_re, double* __restrict__ out_im) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoints) return;
+    cplx xs[kMaxPade], coef[kMaxPade];
+    const int npa = pade_fit(sre, sim, omega, nfreq, npar, xs, coef);
+    const double e = x_start + static_cast<double>(i) * interval - fermi;
+    const cplx v = pade_eval(coef, xs, npa, e);
+    out_re[i] = v.re;
+    out_im[i] = v.im;
 }
 
 // ERI from its factor: eri[P][R] = sum_K B[P][K] B[R][K] for P, R < N = nmo^2,
@@ -796,6 +827,14 @@ hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_
 
 hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s) {
     hipLaunchKernelGGL(k_sigma_x, dim3((nmo + 63) / 64), dim3(64), 0, s, eri, sx, nmo, nocc);
+    return hipGetLastError();
+}
+
+hipError_t launch_pade_real_axis(const double* sig_re, const double* sig_im, const double* omega,
+                                 int nfreq, int npar, double fermi, double x_start, double interval,
+                                 int npoints, double* out_re, double* out_im, hipStream_t s) {
+    hipLaunchKernelGGL(k_pade_real_axis, dim3((npoints + 127) / 128), dim3(128), 0, s, sig_re, sig_im,
+                       omega, nfreq, npar, fermi, x_start, interval, npoints, out_re, out_im);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmgw_hip.so")
 
 MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_MEMORY, \
-    MGW_ERR_STATE = range(7)
+    MGW_ERR_STATE, MGW_ERR_IO = range(8)
 PREC_FP64, PREC_MIXED, PREC_MIXED_INT8 = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
