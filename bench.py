@@ -43,8 +43,10 @@ HA_MEV = 27.211386245988 * 1e3
 # HBM bytes per contraction launch (FETCH_SIZE + WRITE_SIZE, each scaled by
 # what the counter reads for a 2 GiB copy), averaged over the molecule's
 # launches, from two rocprofv3 --pmc runs of this bench; the commands and
-# per-dispatch figures are in profiles/kernels_c3_nmo200_mixed_fp64.md.
-TRAFFIC = {("k_contract_split<FP16>", "c3", 200): (130.6e9 + 0.19e9),
+# per-dispatch figures are in profiles/kernels_c3_nmo200_mixed_fp64.md
+# (FP16 split and FP64) and profiles/kernels_c3_nmo200_int8_fp64.md (int8).
+TRAFFIC = {("k_contract_i8", "c3", 200): (309.0e9 + 0.39e9),          # ..._int8_fp64.md
+           ("k_contract_split<FP16>", "c3", 200): (130.6e9 + 0.19e9),
            ("k_contract_f64", "c3", 200): (1421.2e9 + 0.39e9)}
 
 
