@@ -49,7 +49,7 @@ void hip_check(hipError_t e, const char* what) {
 // ------------------------------------------------------------------ context
 struct GraphKey {
     int np, mpad, m, b, precision;
-    std::array<const void*, 19> ptr;
+    std::array<const void*, 21> ptr;
     bool operator==(const GraphKey& o) const {
         return np == o.np && mpad == o.mpad && m == o.m && b == o.b && precision == o.precision &&
                ptr == o.ptr;
@@ -137,13 +137,15 @@ struct mgw_hip_ctx {
         hsx, hvxc, hstates, hqp, hiters;
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
     Buf hqim, hY, wY, wW;             // complex path (eta != 0)
+    Buf hfr, hfc;                     // factor block slices (mixed forms)
+    bool int8_factor = true;          // mixed forms: block products on int8 MFMA (MGW_INT8_FACTOR=0: FP64)
     bool wc_complex = false;
 
     mgw_hip_ctx() {
         all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
-               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW};
+               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -285,12 +287,27 @@ size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
                + 4;                          // info
     if (precision == MGW_PREC_MIXED) b += (size_t)np * np * 4 + (size_t)np * 8;  // M planes
     if (precision == MGW_PREC_MIXED_INT8) b += (size_t)np * np * mgw::kSlices + (size_t)np * 8;
+    if (precision != MGW_PREC_FP64) b += 2 * mgw::fact_slice_bytes(np);  // factor block slices
     return b;
 }
 
 // Factor + invert nb symmetrised dielectric matrices already in A (lower).
+// With slice buffers (mixed forms) the block products run on int8 MFMA.
 void factor_batch(mgw_hip_ctx* ctx, double* A, double* dinv, int* info, double* tbuf, int np,
-                  int nb) {
+                  int nb, char* rslices = nullptr, char* cslices = nullptr) {
+    if (rslices && cslices) {
+        const size_t nblk = (size_t)(np / mgw::kNB);
+        const size_t planes = (size_t)nb * nblk * nblk * mgw::kSlices * 4096;
+        int8_t* RP = reinterpret_cast<int8_t*>(rslices);
+        int8_t* CP = reinterpret_cast<int8_t*>(cslices);
+        double* RS = reinterpret_cast<double*>(rslices + planes);
+        double* CS = reinterpret_cast<double*>(cslices + planes);
+        hip_check(mgw::launch_potrf_i8(A, dinv, info, RP, RS, CP, CS, np, nb, ctx->stream),
+                  "potrf (int8 products) launch");
+        hip_check(mgw::launch_trtri_i8(A, dinv, tbuf, RP, RS, CP, CS, np, nb, ctx->stream),
+                  "trtri (int8 products) launch");
+        return;
+    }
     hip_check(mgw::launch_potrf(A, dinv, info, np, nb, ctx->stream), "potrf launch");
     hip_check(mgw::launch_trtri(A, dinv, tbuf, np, nb, ctx->stream), "trtri launch");
 }
@@ -357,6 +374,7 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         hip_check(hipSetDevice(device), "hipSetDevice");
         auto* ctx = new mgw_hip_ctx();
         ctx->device = device;
+        if (const char* env = std::getenv("MGW_INT8_FACTOR")) ctx->int8_factor = env[0] != '0';
         if (const char* env = std::getenv("MGW_SPLIT_UNIT_FREQS")) {
             const int uf = std::atoi(env);
             if (uf == 1 || uf == 2 || uf == 4 || uf == 8 || uf == 16 || uf == 32) ctx->split_uf = uf;
@@ -851,6 +869,11 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         double* dinv = as<double>(ctx->get(ctx->hdinv, zf * nb * nblk * 64 * 64 * 8, "dinv"));
         double* Y = cplx ? as<double>(ctx->get(ctx->hY, (size_t)nb * np * mpad * 16, "complex panel"))
                          : nullptr;
+        char *fr = nullptr, *fc = nullptr;
+        if (job->precision != MGW_PREC_FP64 && ctx->int8_factor) {
+            fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np), "factor row slices"));
+            fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np), "factor column slices"));
+        }
         double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * np * 64 * 8, "trtri scratch"));
         int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
         int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
@@ -894,7 +917,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
             hip_check(mgw::launch_dielectric(dV, S, A, np, b, st), "dielectric");
             if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-            factor_batch(ctx, A, dinv, info, tbuf, np, b);
+            factor_batch(ctx, A, dinv, info, tbuf, np, b, fr, fc);
             if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
@@ -933,7 +956,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 // buffers was captured before (the offset is read on the device).
                 const GraphKey key{np, mpad, m, b, job->precision,
                                    {A, S, dinv, tbuf, info, qpart, dq, xp, xh, xl, xs, mh, ml, ms, dV,
-                                    ddelta, dom, dcolmap, f0p}};
+                                    ddelta, dom, dcolmap, f0p, fr, fc}};
                 hipGraphExec_t exec = ctx->find_graph(key);
                 if (!exec) {
                     hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),

@@ -57,6 +57,27 @@ hipError_t launch_dielectric(const double* v, const double* S, double* A, int np
 // info[w] = 1-based global row of the first non-positive pivot, 0 if none.
 hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipStream_t s);
 
+// Single steps of the blocked factorization (block column k / j): the
+// diagonal block's factor and inverse, the panel solve below it, and the
+// inverse factor's column j from T (tbuf) -- shared by the FP64 and the
+// int8-update forms.
+hipError_t launch_potf2_block(double* A, double* dinv, int* info, int np, int k, int nb,
+                              hipStream_t s);
+hipError_t launch_trsm_block(double* A, const double* dinv, int np, int k, int nb, hipStream_t s);
+hipError_t launch_trtri_block(double* A, const double* dinv, const double* tbuf, int np, int j,
+                              int nb, hipStream_t s);
+
+// Factorization of the mixed forms (mgw_ozfact.hip): the same blocked
+// Cholesky and inverse sweep with the block products on int8 MFMA.  RP/RS and
+// CP/CS hold the row- and column-scaled slices of every 64 x 64 block
+// (nb x fact_slice_bytes(np) bytes each: planes first, then nb x (np/64)^2 x
+// 64 scales as doubles at the byte offset nb x (np/64)^2 x kSlices x 4096).
+size_t fact_slice_bytes(int np);
+hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
+                           double* CS, int np, int nb, hipStream_t s);
+hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
+                           const int8_t* CP, const double* CS, int np, int nb, hipStream_t s);
+
 // In-place inverse of the lower factor: A_w <- L_w^{-1} (lower, zeros above the
 // diagonal inside diagonal blocks).  tbuf: nb x np x 64 scratch.
 hipError_t launch_trtri(double* A, const double* dinv, double* tbuf, int np, int nb,

@@ -769,6 +769,25 @@ hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipS
     return hipGetLastError();
 }
 
+hipError_t launch_potf2_block(double* A, double* dinv, int* info, int np, int k, int nb,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_trsm_block(double* A, const double* dinv, int np, int k, int nb, hipStream_t s) {
+    const int t = np / kNB - k - 1;
+    if (t > 0) hipLaunchKernelGGL(k_trsm_panel, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_trtri_block(double* A, const double* dinv, const double* tbuf, int np, int j,
+                              int nb, hipStream_t s) {
+    const int t = np / kNB - 1 - j;
+    hipLaunchKernelGGL(k_trtri_m, dim3(t + 1, nb), dim3(256), 0, s, A, dinv, tbuf, np, j);
+    return hipGetLastError();
+}
+
 hipError_t launch_trtri(double* A, const double* dinv, double* tbuf, int np, int nb,
                         hipStream_t s) {
     const int nblk = np / kNB;

@@ -17,15 +17,13 @@
 // Bounds: |t u| <= 2^12, at most kSlices pairs per group and K <= kMaxK
 // rows, so a group sum stays below 5 * 2^12 * 2^16 < 2^31.
 #include "mgw_kernels.h"
+#include "mgw_slices.h"
 
 #include <hip/hip_runtime.h>
 
 namespace mgw {
 namespace {
 
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-typedef int i32x16 __attribute__((ext_vector_type(16)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kS = kSlices;
 constexpr int kIBM = 128, kIBN = 128, kIBK = 32;   // workgroup tile and K step
@@ -44,31 +42,12 @@ __device__ inline size_t slice_index(int row, int k, int np) {
     return ((size_t)(row >> 7) * (np >> 5) + (k >> 5)) * 4096 + (size_t)(row & 127) * 32 + (k & 31);
 }
 
-// scale 2^-e with max|v| < 2^e (1 for an all-zero vector)
-__device__ inline double slice_scale(double mx) {
-    if (!(mx > 0.0)) return 1.0;
-    int e;
-    frexp(mx, &e);                  // mx = f 2^e, f in [0.5, 1)
-    return ldexp(1.0, -e);
-}
 
 __device__ inline double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
     return v;
 }
 
-// r in (-1, 1) -> kS slices; every operation is exact in FP64
-__device__ inline void slice_value(double r, int8_t* t) {
-    double x = r * 64.0;
-    double q = rint(x);
-    t[0] = (int8_t)q;
-#pragma unroll
-    for (int i = 1; i < kS; ++i) {
-        x = (x - q) * 128.0;
-        q = rint(x);
-        t[i] = (int8_t)q;
-    }
-}
 
 // One wave per column c of X: gather, scale, slice.
 __global__ __launch_bounds__(256) void k_slice_x(const double* __restrict__ src,
@@ -121,12 +100,6 @@ __global__ __launch_bounds__(256) void k_slice_m(const double* __restrict__ M,
     if (lane == 0) mscale[(size_t)w * np + r] = scale;
 }
 
-__device__ inline i32x16 mfma_i8(u32x4 a, u32x4 b, i32x16 c) {
-    return __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a),
-                                                  __builtin_bit_cast(i32x4, b), c, 0, 0, 0);
-}
-// result d[r] of the 32x32 forms is D[row32(lane, r)][lane % 32]
-__device__ inline int row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
 // Workgroup -> tile map: units of uf frequencies x uc column tiles of one row
 // panel land on one XCD (workgroup L runs on XCD L % 8) and share its L2;

@@ -1,0 +1,244 @@
+// Factorization GEMMs of the mixed forms on int8 MFMA.
+//
+// The blocked Cholesky's trailing updates (A_ik -= sum_{j<k} L_ij L_kj^T) and
+// the inverse factor's sweeps (T_i = sum_{k=j+1}^{i} M_ik L_kj) are products
+// of 64 x 64 blocks.  Each finished block is cut once into kSlices int8
+// slices (mgw_slices.h) with a power-of-two scale per row (per column for the
+// L_kj operand of the sweeps, stored transposed); a product over one 64-wide
+// K block runs as the slice products with i + j < kSlices on
+// v_mfma_i32_32x32x32_i8 (exact in int32), and the K blocks, whose scales
+// differ, are combined in FP64.  The diagonal blocks (factor and inverse in
+// LDS), the panel solves and M_ij = -T_i L_jj^-1 stay FP64 (mgw_kernels.hip).
+// The truncation error per block product is ~2^-35 of the largest element of
+// each row segment; the contraction that follows carries the same order.
+#include "mgw_kernels.h"
+#include "mgw_slices.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mgw {
+namespace {
+
+constexpr int kS = kSlices;
+static_assert(kSlices == 5, "the int32 group combination of k_gemm_i8blk assumes 5 slices");
+constexpr size_t kBlkBytes = (size_t)kS * 4096;     // slices of one 64 x 64 block
+
+__device__ inline size_t blk_index(int w, int nblk, int bi, int bj) {
+    return ((size_t)w * nblk + bi) * nblk + bj;
+}
+
+// One workgroup per 64 x 64 block (bi = b0 + blockIdx.x, bj) of A_w: row-scaled
+// slices into RP/RS (when RP != NULL) and, transposed, column-scaled slices
+// into CP/CS (when CP != NULL), both at block (bi, bj).  RS / CS hold the
+// inverse scales 2^e (the factor that restores a row from its slices).
+__global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__ A, int np, int b0,
+                                                      int bj, int8_t* __restrict__ RP,
+                                                      double* __restrict__ RS,
+                                                      int8_t* __restrict__ CP,
+                                                      double* __restrict__ CS) {
+    __shared__ double blk[64][65];
+    __shared__ double rsc[64], csc[64];
+    const int w = blockIdx.y, t = threadIdx.x, nblk = np / 64, bi = b0 + blockIdx.x;
+    const double* src = A + (size_t)w * np * np + (size_t)bi * 64 * np + (size_t)bj * 64;
+    for (int e = t; e < 4096; e += 256) blk[e >> 6][e & 63] = src[(size_t)(e >> 6) * np + (e & 63)];
+    __syncthreads();
+    if (t < 64) {
+        double m = 0.0;
+        for (int c = 0; c < 64; ++c) m = fmax(m, fabs(blk[t][c]));
+        rsc[t] = slice_scale(m);
+    } else if (t < 128) {
+        double m = 0.0;
+        for (int r = 0; r < 64; ++r) m = fmax(m, fabs(blk[r][t - 64]));
+        csc[t - 64] = slice_scale(m);
+    }
+    __syncthreads();
+    const size_t bix = blk_index(w, nblk, bi, bj);
+    if (RP) {
+        int8_t* out = RP + bix * kBlkBytes;
+        for (int e = t; e < 4096; e += 256) {
+            const int r = e >> 6, c = e & 63;
+            int8_t s[kS];
+            slice_value(blk[r][c] * rsc[r], s);
+#pragma unroll
+            for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = s[i];
+        }
+        if (t < 64) RS[bix * 64 + t] = 1.0 / rsc[t];     // 2^e, exact
+    }
+    if (CP) {
+        int8_t* out = CP + bix * kBlkBytes;
+        for (int e = t; e < 4096; e += 256) {
+            const int c = e >> 6, r = e & 63;       // transposed: row c of the output
+            int8_t s[kS];
+            slice_value(blk[r][c] * csc[c], s);
+#pragma unroll
+            for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = s[i];
+        }
+        if (t < 64) CS[bix * 64 + t] = 1.0 / csc[t];
+    }
+}
+
+// LDS byte offset of 16 B chunk `ch` (0..3) of row `row` in a 64 x 64 byte
+// slice block: chunks rotated by row / 4, so the 16 lanes of a 16 B access
+// (4 rows x 4 chunks stored, 16 rows x 1 chunk read) hit distinct bank groups.
+__device__ inline int blk_off(int row, int ch) { return row * 64 + ((ch ^ ((row >> 2) & 3)) << 4); }
+
+// One 64 x 64 output tile per workgroup, 256 threads = 4 waves of 32 x 32.
+// MODE 0 (Cholesky update of block column k = kj): tile i = k + blockIdx.x,
+//   K blocks kb < k, A = RP(i, kb), B = RP(k, kb); A_w(i, k) -= product.
+// MODE 1 (inverse sweep, column j = kj): tile i = j + 1 + blockIdx.x,
+//   K blocks j < kb <= i, A = RP(i, kb) (M rows), B = CP(kb, j) (L columns);
+//   tbuf_w(i) = product (64 x 64 row-major).
+// Per K block the 2 x kS slice blocks (40 KB) go through LDS (two buffers,
+// register prefetch one block ahead), 2 k chunks x 15 slice products per
+// wave, then the int32 groups are combined and scaled into FP64 accumulators.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ RP,
+                                                    const double* __restrict__ RS,
+                                                    const int8_t* __restrict__ CP,
+                                                    const double* __restrict__ CS,
+                                                    double* __restrict__ Amat,
+                                                    double* __restrict__ tbuf, int np, int kj) {
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][kS][4096];
+    __shared__ double sc[2][2][64];                   // inverse scales of the staged blocks
+    const int w = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int nblk = np / 64;
+    const int i = MODE == 0 ? kj + blockIdx.x : kj + 1 + blockIdx.x;
+    const int kb0 = MODE == 0 ? 0 : kj + 1, kb1 = MODE == 0 ? kj : i + 1;   // [kb0, kb1)
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int8_t* Bbase = MODE == 0 ? RP : CP;
+    const double* Bsc = MODE == 0 ? RS : CS;
+    auto a_blk = [&](int kb) { return blk_index(w, nblk, i, kb); };
+    auto b_blk = [&](int kb) { return MODE == 0 ? blk_index(w, nblk, kj, kb) : blk_index(w, nblk, kb, kj); };
+
+    // loader: chunk u of thread t -> operand u / 5, slice u % 5, row t / 4, chunk t % 4
+    const int lrow = t >> 2, lch = t & 3;
+    const int goff = lrow * 64 + lch * 16, loff = blk_off(lrow, lch);
+    u32x4 pre[10];
+    double psc = 0.0;                                 // thread t < 128: one inverse scale
+    auto fetch = [&](int kb) {
+        const int8_t* a = RP + a_blk(kb) * kBlkBytes + goff;
+        const int8_t* b = Bbase + b_blk(kb) * kBlkBytes + goff;
+#pragma unroll
+        for (int s = 0; s < kS; ++s) {
+            pre[s] = *reinterpret_cast<const u32x4*>(a + (size_t)s * 4096);
+            pre[kS + s] = *reinterpret_cast<const u32x4*>(b + (size_t)s * 4096);
+        }
+        if (t < 64) psc = RS[a_blk(kb) * 64 + t];
+        else if (t < 128) psc = Bsc[b_blk(kb) * 64 + t - 64];
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < kS; ++s) {
+            *reinterpret_cast<u32x4*>(&lds[buf][0][s][loff]) = pre[s];
+            *reinterpret_cast<u32x4*>(&lds[buf][1][s][loff]) = pre[kS + s];
+        }
+        if (t < 128) sc[buf][t >> 6][t & 63] = psc;
+    };
+
+    double acc64[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+    const int l32 = lane & 31, kgl = lane >> 5;
+    if (kb0 < kb1) {
+        fetch(kb0);
+        stash(0);
+        if (kb0 + 1 < kb1) fetch(kb0 + 1);
+        __syncthreads();
+        for (int kb = kb0; kb < kb1; ++kb) {
+            const int cur = (kb - kb0) & 1;
+            i32x16 acc[kS];
+#pragma unroll
+            for (int g = 0; g < kS; ++g)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[g][r] = 0;
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc) {
+                const int ch = 2 * kc + kgl;
+                const int oa = blk_off(wr + l32, ch), ob = blk_off(wc + l32, ch);
+                u32x4 a[kS];
+#pragma unroll
+                for (int s = 0; s < kS; ++s) a[s] = *reinterpret_cast<const u32x4*>(&lds[cur][0][s][oa]);
+#pragma unroll
+                for (int j = 0; j < kS; ++j) {
+                    const u32x4 b = *reinterpret_cast<const u32x4*>(&lds[cur][1][j][ob]);
+#pragma unroll
+                    for (int s = 0; s + j < kS; ++s) acc[s + j] = mfma_i8(a[s], b, acc[s + j]);
+                }
+            }
+            // next block into the other buffer (its readers passed the last
+            // barrier), fetch the one after
+            if (kb + 1 < kb1) {
+                stash(cur ^ 1);
+                if (kb + 2 < kb1) fetch(kb + 2);
+            }
+            // combine this K block's groups with its row / column scales:
+            // sum_g G_g 2^-(12 + 7g) = (G0 2^28 + G1 2^21 + G2 2^14 + G3 2^7 + G4) 2^-40
+            // and for K = 64 (|G_g| <= 5 * 2^18) hi = G0 * 128 + G1 and
+            // lo = G3 * 128 + G4 fit in int32, z = hi 2^21 + G2 2^14 + lo < 2^47
+            // is exact in FP64
+            const double inv_b = sc[cur][1][wc + l32] * 0x1p-40;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int hi = acc[0][r] * 128 + acc[1][r];
+                const int lo = acc[3][r] * 128 + acc[4][r];
+                const double z = fma((double)hi, 0x1p21, fma((double)acc[2][r], 0x1p14, (double)lo));
+                acc64[r] = fma(z * inv_b, sc[cur][0][wr + row32(lane, r)], acc64[r]);
+            }
+            __syncthreads();
+        }
+    }
+    if (MODE == 0) {
+        double* out = Amat + (size_t)w * np * np + (size_t)i * 64 * np + (size_t)kj * 64;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[(size_t)(wr + row32(lane, r)) * np + wc + l32] -= acc64[r];
+    } else {
+        double* out = tbuf + (size_t)w * np * 64 + (size_t)i * 64 * 64;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[(wr + row32(lane, r)) * 64 + wc + l32] = acc64[r];
+    }
+}
+
+}  // namespace
+
+size_t fact_slice_bytes(int np) {
+    const size_t nblk = (size_t)(np / 64);
+    return nblk * nblk * (kBlkBytes + 64 * 8);
+}
+
+hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
+                           double* CS, int np, int nb, hipStream_t s) {
+    if (np % 64 != 0) return hipErrorInvalidValue;
+    const int nblk = np / 64;
+    for (int k = 0; k < nblk; ++k) {
+        if (k > 0)
+            hipLaunchKernelGGL(k_gemm_i8blk<0>, dim3(nblk - k, nb), dim3(256), 0, s, RP, RS, CP, CS, A,
+                               nullptr, np, k);
+        if (hipError_t e = launch_potf2_block(A, dinv, info, np, k, nb, s); e != hipSuccess) return e;
+        const int t = nblk - k - 1;
+        if (t > 0) {
+            if (hipError_t e = launch_trsm_block(A, dinv, np, k, nb, s); e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_slice_blocks, dim3(t, nb), dim3(256), 0, s, A, np, k + 1, k, RP, RS,
+                               CP, CS);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
+                           const int8_t* CP, const double* CS, int np, int nb, hipStream_t s) {
+    if (np % 64 != 0) return hipErrorInvalidValue;
+    const int nblk = np / 64;
+    for (int j = nblk - 1; j >= 0; --j) {
+        const int t = nblk - 1 - j;
+        if (t > 0)
+            hipLaunchKernelGGL(k_gemm_i8blk<1>, dim3(t, nb), dim3(256), 0, s, RP, RS, CP, CS, nullptr,
+                               tbuf, np, j);
+        if (hipError_t e = launch_trtri_block(A, dinv, tbuf, np, j, nb, s); e != hipSuccess) return e;
+        if (j > 0)      // row slices of the new inverse blocks (i, j), i >= j
+            hipLaunchKernelGGL(k_slice_blocks, dim3(t + 1, nb), dim3(256), 0, s, A, np, j, j, RP, RS,
+                               nullptr, nullptr);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mgw

@@ -81,7 +81,8 @@ def test_serve_admission_and_rejection(hip):
         for t in tickets:
             srv.wait(t)
         assert srv.stats()["peak_running"] >= 1
-        big = synthetic_input(40, 8, 5)
+        big = synthetic_input(64, 12, 5)
+        assert hip.estimate_bytes(64, 12, 40, 0, 16, hip.PREC_FP64) > budget
         t = srv.submit(big, nfreq=40, npar=12)
         with pytest.raises(hip.HipError) as e:
             srv.wait(t)
