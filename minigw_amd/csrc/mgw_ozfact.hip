@@ -91,15 +91,15 @@ __device__ inline int blk_off(int row, int ch) { return row * 64 + ((ch ^ ((row 
 // Per K block the 2 x kS slice blocks (40 KB) go through LDS (two buffers,
 // register prefetch one block ahead), 2 k chunks x 15 slice products per
 // wave, then the int32 groups are combined and scaled into FP64 accumulators.
+// Two waves per SIMD (two workgroups per CU: 80 KB of LDS each, <= 256
+// registers per wave) so that one wave's group combination overlaps the
+// other's MFMAs.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ RP,
-                                                    const double* __restrict__ RS,
-                                                    const int8_t* __restrict__ CP,
-                                                    const double* __restrict__ CS,
-                                                    double* __restrict__ Amat,
-                                                    double* __restrict__ tbuf, int np, int kj) {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][kS][4096];
-    __shared__ double sc[2][2][64];                   // inverse scales of the staged blocks
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_i8blk(
+    const int8_t* __restrict__ RP, const double* __restrict__ RS, const int8_t* __restrict__ CP,
+    const double* __restrict__ CS, double* __restrict__ Amat, double* __restrict__ tbuf, int np,
+    int kj) {
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][kS][4096];   // 80 KB: two per CU
     const int w = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nblk = np / 64;
     const int i = MODE == 0 ? kj + blockIdx.x : kj + 1 + blockIdx.x;
@@ -114,7 +114,6 @@ __global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ R
     const int lrow = t >> 2, lch = t & 3;
     const int goff = lrow * 64 + lch * 16, loff = blk_off(lrow, lch);
     u32x4 pre[10];
-    double psc = 0.0;                                 // thread t < 128: one inverse scale
     auto fetch = [&](int kb) {
         const int8_t* a = RP + a_blk(kb) * kBlkBytes + goff;
         const int8_t* b = Bbase + b_blk(kb) * kBlkBytes + goff;
@@ -123,8 +122,6 @@ __global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ R
             pre[s] = *reinterpret_cast<const u32x4*>(a + (size_t)s * 4096);
             pre[kS + s] = *reinterpret_cast<const u32x4*>(b + (size_t)s * 4096);
         }
-        if (t < 64) psc = RS[a_blk(kb) * 64 + t];
-        else if (t < 128) psc = Bsc[b_blk(kb) * 64 + t - 64];
     };
     auto stash = [&](int buf) {
 #pragma unroll
@@ -132,7 +129,6 @@ __global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ R
             *reinterpret_cast<u32x4*>(&lds[buf][0][s][loff]) = pre[s];
             *reinterpret_cast<u32x4*>(&lds[buf][1][s][loff]) = pre[kS + s];
         }
-        if (t < 128) sc[buf][t >> 6][t & 63] = psc;
     };
 
     double acc64[16];
@@ -146,6 +142,13 @@ __global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ R
         __syncthreads();
         for (int kb = kb0; kb < kb1; ++kb) {
             const int cur = (kb - kb0) & 1;
+            // this K block's inverse scales (powers of two, kept as exponents),
+            // loaded now and used after the MFMAs
+            int ea[16];
+            const double* sa = RS + a_blk(kb) * 64 + wr;
+            const int eb = ilogb(Bsc[b_blk(kb) * 64 + wc + l32]) - 40;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) ea[r] = ilogb(sa[row32(lane, r)]) + eb;
             i32x16 acc[kS];
 #pragma unroll
             for (int g = 0; g < kS; ++g)
@@ -176,13 +179,12 @@ __global__ __launch_bounds__(256) void k_gemm_i8blk(const int8_t* __restrict__ R
             // and for K = 64 (|G_g| <= 5 * 2^18) hi = G0 * 128 + G1 and
             // lo = G3 * 128 + G4 fit in int32, z = hi 2^21 + G2 2^14 + lo < 2^47
             // is exact in FP64
-            const double inv_b = sc[cur][1][wc + l32] * 0x1p-40;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int hi = acc[0][r] * 128 + acc[1][r];
                 const int lo = acc[3][r] * 128 + acc[4][r];
                 const double z = fma((double)hi, 0x1p21, fma((double)acc[2][r], 0x1p14, (double)lo));
-                acc64[r] = fma(z * inv_b, sc[cur][0][wr + row32(lane, r)], acc64[r]);
+                acc64[r] += ldexp(z, ea[r]);
             }
             __syncthreads();
         }
