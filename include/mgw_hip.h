@@ -244,12 +244,19 @@ int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* ru
  * entries the caller provides (<= 12). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 
-/* Autotuner: times mgw_hip_g0w0 on the resident ERI with 8, 16, 32 and 64
- * frequencies factored per batch (two batches each), keeps the fastest per
- * (nmo, nocc, nstates, precision) for every later mgw_hip_g0w0 call of the
- * process, and returns it with its time per frequency. */
+/* Autotuner (no counterpart in the reference): times mgw_hip_g0w0 on the
+ * resident ERI with 8, 16, 32 and 64 frequencies factored per batch (two
+ * batches each) and, for the mixed forms, the contraction's tile unit shape
+ * (2, 4 or 8 frequencies per 32-tile unit sharing an XCD's L2) at that batch;
+ * keeps the fastest per (nmo, nocc, nstates, precision) for every later
+ * mgw_hip_g0w0 call of the process and returns the batch with its time per
+ * frequency.  MGW_SPLIT_UNIT_FREQS, when set, overrides the unit shape. */
 int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_batch,
                      double* best_ms_per_freq);
+
+/* The choice the autotuner keeps for this job's shape: batch and unit
+ * frequencies (0 = not tuned / default). */
+int mgw_hip_autotune_result(const mgw_g0w0_job* job, int32_t* batch, int32_t* unit_freqs);
 
 /* Run each frequency batch of mgw_hip_g0w0 as a captured hipGraph, replayed
  * for later batches and molecules of the same shape on the same buffers

@@ -48,11 +48,11 @@ void hip_check(hipError_t e, const char* what) {
 
 // ------------------------------------------------------------------ context
 struct GraphKey {
-    int np, mpad, m, b, precision;
+    int np, mpad, m, b, precision, uf;
     std::array<const void*, 21> ptr;
     bool operator==(const GraphKey& o) const {
         return np == o.np && mpad == o.mpad && m == o.m && b == o.b && precision == o.precision &&
-               ptr == o.ptr;
+               uf == o.uf && ptr == o.ptr;
     }
 };
 
@@ -108,8 +108,10 @@ struct mgw_hip_ctx {
 
     // frequency batch forced by the autotuner while it times candidates (0 = auto)
     int forced_nb = 0;
-    // frequencies per 32-tile unit of the split contraction (MGW_SPLIT_UNIT_FREQS overrides)
-    int split_uf = 4;
+    // frequencies per 32-tile unit of the mixed contractions: MGW_SPLIT_UNIT_FREQS
+    // (0 = the autotuned value, else 4); forced_uf while the autotuner times one
+    int split_uf = 0;
+    int forced_uf = 0;
 
     // captured frequency-batch graphs (hipGraph), keyed by shape and buffers
     bool use_graphs = false;
@@ -328,12 +330,17 @@ void check_info(mgw_hip_ctx* ctx, const int* dinfo, int count, const char* what)
 // Frequency-batch sizes picked by mgw_hip_autotune, per (nmo, nocc, nstates,
 // precision), shared by all contexts of the process.
 std::mutex g_tune_mu;
-std::map<std::array<int, 4>, int> g_tune;
+// autotuned frequency batch and (mixed forms) contraction unit shape
+struct Tuned {
+    int nb = 0;     // 0: not tuned
+    int uf = 0;     // frequencies per 32-tile unit of the mixed contractions, 0: default
+};
+std::map<std::array<int, 4>, Tuned> g_tune;
 
-int tuned_batch(int nmo, int nocc, int ns, int precision) {
+Tuned tuned(int nmo, int nocc, int ns, int precision) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find({nmo, nocc, ns, precision});
-    return it == g_tune.end() ? 0 : it->second;
+    return it == g_tune.end() ? Tuned{} : it->second;
 }
 
 // ================================================================ C ABI
@@ -861,8 +868,13 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const size_t per = batch_bytes(np, mpad, job->precision, cplx);
         int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
         nb = std::min(nb, 64);
-        const int pick = ctx->forced_nb ? ctx->forced_nb : tuned_batch(nmo, nocc, ns, job->precision);
+        const Tuned tn = tuned(nmo, nocc, ns, job->precision);
+        const int pick = ctx->forced_nb ? ctx->forced_nb : tn.nb;
         if (pick > 0) nb = std::min(nb, pick);
+        const int uf = ctx->forced_uf ? ctx->forced_uf
+                       : ctx->split_uf ? ctx->split_uf
+                       : tn.uf ? tn.uf
+                               : 4;
         const size_t zf = cplx ? 2 : 1;    // complex entries are two doubles
         double* A = as<double>(ctx->get(ctx->hA, zf * nb * np * np * 8, "dielectric batch"));
         double* S = as<double>(ctx->get(ctx->hS, zf * nb * np * 8, "scale batch"));
@@ -927,12 +939,12 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b,
-                                                     ctx->split_uf, st),
+                                                     uf, st),
                           "contract split");
             else if (i8)
                 hip_check(mgw::launch_contract_i8(reinterpret_cast<const int8_t*>(mh), ms,
                                                   reinterpret_cast<const int8_t*>(xh), qpart, np,
-                                                  mpad, b, ctx->split_uf, st),
+                                                  mpad, b, uf, st),
                           "contract int8");
             else
                 hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");
@@ -954,7 +966,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             if (ctx->use_graphs && !cplx) {
                 // Replay a graph of this batch when one with the same shape and
                 // buffers was captured before (the offset is read on the device).
-                const GraphKey key{np, mpad, m, b, job->precision,
+                const GraphKey key{np, mpad, m, b, job->precision, uf,
                                    {A, S, dinv, tbuf, info, qpart, dq, xp, xh, xl, xs, mh, ml, ms, dV,
                                     ddelta, dom, dcolmap, f0p, fr, fc}};
                 hipGraphExec_t exec = ctx->find_graph(key);
@@ -1208,6 +1220,15 @@ int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
     });
 }
 
+int mgw_hip_autotune_result(const mgw_g0w0_job* job, int32_t* batch, int32_t* unit_freqs) {
+    if (!job || !batch || !unit_freqs) return MGW_ERR_ARG;
+    const int ns = job->nstates > 0 && job->states ? job->nstates : job->nmo;
+    const Tuned tn = tuned(job->nmo, job->nocc, ns, job->precision);
+    *batch = tn.nb;
+    *unit_freqs = tn.uf;
+    return MGW_OK;
+}
+
 int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_batch,
                      double* best_ms_per_freq) {
     return guarded(ctx, [&] {
@@ -1244,10 +1265,40 @@ int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_ba
                 best_t = t;
             }
         }
+        // mixed forms: the contraction's unit shape (frequencies per 32-tile
+        // unit sharing an XCD's L2) at the chosen batch
+        int best_uf = 0;
+        if (job->precision != MGW_PREC_FP64 && best > 0) {
+            double best_c = 0.0;
+            for (int uf : {4, 2, 8}) {
+                mgw_g0w0_job j = *job;
+                j.nfreq = std::min(job->nfreq, 2 * best);
+                sre.resize((size_t)ns * j.nfreq);
+                sim.resize((size_t)ns * j.nfreq);
+                mgw_g0w0_out o{};
+                o.qp = qp.data();
+                o.sigma_c_re = sre.data();
+                o.sigma_c_im = sim.data();
+                ctx->forced_nb = best;
+                ctx->forced_uf = uf;
+                const int rc = mgw_hip_g0w0(ctx, &j, &o);
+                ctx->forced_nb = 0;
+                ctx->forced_uf = 0;
+                if (rc != MGW_OK) {
+                    ctx->use_graphs = graphs;
+                    throw MgwError(rc, ctx->error);
+                }
+                const double tc = ctx->timings[3] / j.nfreq;
+                if (best_uf == 0 || tc < best_c) {
+                    best_uf = uf;
+                    best_c = tc;
+                }
+            }
+        }
         ctx->use_graphs = graphs;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            g_tune[{nmo, job->nocc, ns, job->precision}] = best;
+            g_tune[{nmo, job->nocc, ns, job->precision}] = Tuned{best, best_uf};
         }
         *best_batch = best;
         if (best_ms_per_freq) *best_ms_per_freq = best_t;

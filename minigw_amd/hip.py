@@ -88,6 +88,8 @@ def _load():
         "mgw_hip_set_graphs": (C.c_int, [vp, C.c_int32]),
         "mgw_hip_autotune": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_double)]),
+        "mgw_hip_autotune_result": (C.c_int, [C.POINTER(G0W0Job), C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int32)]),
         "mgw_hip_selftest_tile": (C.c_int, [vp, _dp, _dp, _dp]),
         "mgw_hip_probe_mfma": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, vp]),
         "mgw_hip_measure_peaks": (C.c_int, [vp, _dp]),
@@ -445,8 +447,9 @@ class HipServer:
 
 def autotune(inp, nfreq: int = 200, npar: int = 16, precision: int = PREC_FP64,
              ctx: HipContext | None = None):
-    """Pick the frequency batch size for this molecule's shape on the resident
-    ERI; returns (batch, ms per frequency).  Later run_g0w0 calls use it."""
+    """Pick the frequency batch size (and, for the mixed forms, the contraction's
+    unit shape) for this molecule's shape on the resident ERI; returns (batch,
+    ms per frequency).  Later run_g0w0 calls use them."""
     ctx = ctx or HipContext()
     mo = _f64(inp.mo_energy)
     vxc = _f64(np.diag(inp.vxc))
@@ -455,3 +458,14 @@ def autotune(inp, nfreq: int = 200, npar: int = 16, precision: int = PREC_FP64,
     b, t = C.c_int32(0), C.c_double(0)
     ctx.call("mgw_hip_autotune", C.byref(job), C.byref(b), C.byref(t))
     return b.value, t.value
+
+
+def autotune_result(inp, nfreq: int = 200, npar: int = 16, precision: int = PREC_FP64):
+    """(batch, unit frequencies) the autotuner keeps for this shape (0 = none)."""
+    mo = _f64(inp.mo_energy)
+    vxc = _f64(np.diag(inp.vxc))
+    job = G0W0Job(inp.nmo, inp.nocc, nfreq, npar, 0, None, _ptr(mo), _ptr(vxc),
+                  float(inp.fermi_energy), 0.0, precision)
+    b, u = C.c_int32(0), C.c_int32(0)
+    _check("mgw_hip_autotune_result", lib.mgw_hip_autotune_result(C.byref(job), C.byref(b), C.byref(u)))
+    return b.value, u.value

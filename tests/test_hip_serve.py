@@ -114,13 +114,22 @@ def test_graph_replays_match_eager(hip):
 
 
 def test_autotune_picks_a_batch_and_keeps_results(hip):
-    say("case: frequency-batch autotuner, nmo=30")
+    say("case: frequency-batch and unit-shape autotuner, nmo=30")
     inp = synthetic_input(30, 6, 43)
     with hip.HipContext(0) as ctx:
         ctx.upload_eri(inp.eri, inp.nocc)
         before = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False)
         nb, ms = hip.autotune(inp, 100, 16, ctx=ctx)
         assert nb in (8, 16, 32, 64) and ms > 0
+        assert hip.autotune_result(inp, 100, 16) == (nb, 0)      # FP64: no unit shape
         after = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False)
         assert after.timings["freq_batch"] == min(nb, 100)
+        assert np.array_equal(after.sigma_c, before.sigma_c)
+        # mixed form: the unit shape changes the tile order only, not the sums
+        p = hip.PREC_MIXED_INT8
+        before = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=p)
+        nb, ms = hip.autotune(inp, 100, 16, p, ctx=ctx)
+        tb, uf = hip.autotune_result(inp, 100, 16, p)
+        assert tb == nb and uf in (2, 4, 8)
+        after = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=p)
         assert np.array_equal(after.sigma_c, before.sigma_c)
