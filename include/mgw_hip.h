@@ -44,11 +44,21 @@ enum mgw_status {
 enum mgw_precision {
     MGW_PREC_FP64 = 0,       /* everything in FP64                                */
     MGW_PREC_MIXED = 1,      /* FP16 hi/lo split contraction on MFMA (3 products, */
-                             /* FP32 acc; ~5e-8 relative in Sigma_c), FP64        */
-                             /* factorization and Sigma_c/Pade/QP                 */
-    MGW_PREC_MIXED_INT8 = 2  /* contraction on int8 MFMA: 5 slices per operand,   */
+                             /* FP32 acc; ~5e-8 relative in Sigma_c); the         */
+                             /* factorization's block products on int8 MFMA       */
+                             /* (MGW_INT8_FACTOR=0: FP64); Sigma_c/Pade/QP FP64   */
+    MGW_PREC_MIXED_INT8 = 2, /* contraction on int8 MFMA: 5 slices per operand,   */
                              /* exact int32 products, FP64 epilogue (~1e-10       */
-                             /* relative in Sigma_c); FP64 elsewhere              */
+                             /* relative in Sigma_c); factorization as MIXED      */
+    MGW_PREC_FP64_CD = 3     /* FP64 throughout, with the ERI decomposed on the   */
+                             /* device (pivoted Cholesky over unique pairs, to    */
+                             /* mgw_hip_set_cd_tolerance): the dielectric is      */
+                             /* factored in the rank-r pair space (r x r instead  */
+                             /* of nph x nph, Woodbury identity).  Needs an ERI   */
+                             /* with (pq|rs) = (qp|rs) = (rs|pq) (else            */
+                             /* UNSUPPORTED) and eta == 0; no counterpart in the  */
+                             /* C++ reference (its PySCF companion's auxiliary-   */
+                             /* basis form, pyscf_prep/gw_ac.py:169-219)          */
 };
 
 /* ---- library / context ------------------------------------------------ */
@@ -235,13 +245,15 @@ int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* ru
                          int32_t* peak_running, size_t* reserved);
 
 /* Phase times of the last mgw_hip_g0w0 call in milliseconds (HIP events on the
- * context's stream): [0] V build (+ X split), [1] dielectric build,
- * [2] factorization (potrf + trtri), [3] contraction kernel, [4] Sigma_c
- * quadrature, [5] Pade + QP, [6] total, [7] frequencies per batch,
- * [8] contraction launches, [9] M split + q reduction, [10] contraction
- * columns (unordered (p,k) pairs when the ERI is exactly pk-symmetric), [11] time
- * in replayed batch graphs (phases 1-3 and 9 are then 0).  n is the number of
- * entries the caller provides (<= 12). */
+ * context's stream): [0] V build (+ X split; with MGW_PREC_FP64_CD the ERI
+ * decomposition), [1] dielectric build, [2] factorization (potrf + trtri),
+ * [3] contraction kernel, [4] Sigma_c quadrature, [5] Pade + QP, [6] total,
+ * [7] frequencies per batch, [8] contraction launches, [9] M split + q
+ * reduction, [10] contraction columns (unordered (p,k) pairs when the ERI is
+ * exactly pk-symmetric), [11] time in replayed batch graphs (phases 1-3 and 9
+ * are then 0), [12] rank of the ERI decomposition (MGW_PREC_FP64_CD, else 0),
+ * [13] its host wall time.  n is the number of entries the caller provides
+ * (<= 14). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 
 /* Autotuner (no counterpart in the reference): times mgw_hip_g0w0 on the
@@ -263,6 +275,12 @@ int mgw_hip_autotune_result(const mgw_g0w0_job* job, int32_t* batch, int32_t* un
  * (enable = 1), or launch kernels directly (0, the default, which also gives
  * the per-phase timings). */
 int mgw_hip_set_graphs(mgw_hip_ctx* ctx, int32_t enable);
+
+/* Stopping tolerance of the MGW_PREC_FP64_CD decomposition: pivots stop once
+ * the largest residual diagonal of the pair matrix is <= tol times the first
+ * (largest) pivot; every residual entry is then bounded by that value.
+ * 0 <= tol < 1, default 1e-12.  No counterpart in the reference. */
+int mgw_hip_set_cd_tolerance(mgw_hip_ctx* ctx, double tol);
 
 /* Diagnostic with no counterpart in the reference: C = A B^T for 64 x 64
  * row-major A, B through the factorization kernels' FP64 MFMA tile (checks

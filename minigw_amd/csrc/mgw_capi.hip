@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -62,7 +63,7 @@ struct mgw_hip_ctx {
     std::string error;
     size_t budget = 0;     // 0 = no cap
     size_t held = 0;
-    double timings[12] = {0};
+    double timings[14] = {0};
 
     // grow-only device buffer with budget accounting
     struct Buf {
@@ -140,6 +141,12 @@ struct mgw_hip_ctx {
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
     Buf hqim, hY, wY, wW;             // complex path (eta != 0)
     Buf hfr, hfc;                     // factor block slices (mixed forms)
+    // Cholesky-decomposed ERI form: pair tables, decomposition workspace,
+    // factor columns, and B^T / pair vectors / their norms
+    Buf cdidx, cdwork, cdl, cdb, cdx, cdn;
+    int eri_pair_sym = -1;            // symmetric to 1e-10 for the decomposed form: -1 unchecked
+    double eri_asym = 0.0, eri_absmax = 0.0;
+    double cd_tol = 1e-12;            // stop when the residual diagonal <= cd_tol * first pivot
     bool int8_factor = true;          // mixed forms: block products on int8 MFMA (MGW_INT8_FACTOR=0: FP64)
     bool wc_complex = false;
 
@@ -147,7 +154,8 @@ struct mgw_hip_ctx {
         all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
-               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc};
+               &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
+               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -325,6 +333,128 @@ void check_info(mgw_hip_ctx* ctx, const int* dinfo, int count, const char* what)
                  info[i]);
 }
 
+struct CdResult {
+    int rank = 0, rpad = 0;
+    double *bt = nullptr, *lx = nullptr, *nrm = nullptr;
+};
+
+// Pivoted Cholesky decomposition of the resident ERI over unique pairs
+// (M[u][v] = (p_u q_u | p_v q_v), u = (p <= q)) on the device, stopped at
+// residual diagonal <= cd_tol * first pivot; then B^T (rpad x np: the rows of
+// the particle-hole pairs) and the pair vectors of the contraction columns
+// (mpad x rpad) with their squared norms.  The factor grows by doubling when
+// the rank passes its capacity; the host polls the rank every 64 pivots.
+CdResult cd_decompose(mgw_hip_ctx* ctx, const PhLayout& L,
+                      const std::vector<std::pair<int, int>>& cols, int mpad) {
+    const int nmo = L.nmo, nocc = L.nocc, np = L.np, mu = (int)cols.size();
+    const int N = nmo * (nmo + 1) / 2;
+    hipStream_t st = ctx->stream;
+    std::vector<int> uidx((size_t)nmo * nmo);
+    std::vector<int> idx((size_t)N + np + mpad, 0);       // pq | phu | colu
+    for (int p = 0, u = 0; p < nmo; ++p)
+        for (int q = p; q < nmo; ++q, ++u) {
+            idx[u] = p * nmo + q;
+            uidx[(size_t)p * nmo + q] = uidx[(size_t)q * nmo + p] = u;
+        }
+    for (int i = 0; i < nocc; ++i)
+        for (int a = 0; a < L.nvir; ++a)
+            idx[(size_t)N + i * L.nvir + a] = uidx[(size_t)i * nmo + nocc + a];
+    for (int c = 0; c < mu; ++c)
+        idx[(size_t)N + np + c] = uidx[(size_t)cols[c].first * nmo + cols[c].second];
+    int* didx = as<int>(ctx->get(ctx->cdidx, idx.size() * 4, "pair tables"));
+    ctx->h2d(didx, idx.data(), idx.size());
+    const int* dpq = didx;
+    const double* eri = as<double>(ctx->eri.p);
+
+    // The form uses one ordering of every pair: the ERI must be symmetric
+    // under pq <-> rs and p <-> k (on the (pk|ia) columns) to 1e-10 of its
+    // largest element (checked once per resident ERI).
+    if (ctx->eri_pair_sym < 0) {
+        unsigned long long* dout =
+            as<unsigned long long>(ctx->get(ctx->cdwork, 3 * sizeof(unsigned long long), "flag"));
+        hip_check(hipMemsetAsync(dout, 0, 3 * sizeof(unsigned long long), st), "hipMemsetAsync");
+        hip_check(mgw::launch_cd_asymmetry(eri, dpq, nmo, nocc, N, dout, st), "symmetry check");
+        double a[3];
+        ctx->d2h(a, dout, 3);
+        ctx->sync();
+        ctx->eri_asym = std::max(a[0], a[1]);
+        ctx->eri_pair_sym = ctx->eri_asym <= 1e-10 * a[2] ? 1 : 0;
+        ctx->eri_absmax = a[2];
+    }
+    if (!ctx->eri_pair_sym)
+        fail(MGW_ERR_UNSUPPORTED,
+             "MGW_PREC_FP64_CD: the resident ERI is not symmetric under pq <-> rs / p <-> k "
+             "(largest difference %g, largest element %g); use MGW_PREC_FP64",
+             ctx->eri_asym, ctx->eri_absmax);
+
+    const int nblk = mgw::cd_blocks(N), Npad = nblk * 64;
+    const size_t wbytes = (size_t)Npad * 8 + (size_t)2 * nblk * 16 + (size_t)N * 8 + 16 +
+                          (size_t)N * 4 + 16;
+    char* wk = as<char>(ctx->get(ctx->cdwork, wbytes, "decomposition workspace"));
+    double* dD = reinterpret_cast<double*>(wk);
+    double2* dbm = reinterpret_cast<double2*>(wk + (size_t)Npad * 8);
+    double* dpval = reinterpret_cast<double*>(wk + (size_t)Npad * 8 + (size_t)2 * nblk * 16);
+    double* dres = dpval + N;
+    int* dpiv = reinterpret_cast<int*>(dres + 2);
+    int* drank = dpiv + N;
+
+    const size_t col_bytes = (size_t)Npad * 8;
+    int rcap = std::min(N, mgw::round_up(std::max(256, 4 * nmo), 64));
+    rcap = std::min(N, std::max<int>(rcap, (int)std::min<size_t>(ctx->cdl.bytes / col_bytes, N)));
+    double* Lc = as<double>(ctx->get(ctx->cdl, (size_t)rcap * col_bytes, "decomposition factor"));
+    hip_check(mgw::launch_cd_init(eri, dpq, nmo, N, dD, dbm, drank, st), "decomposition init");
+    int k = 0, rank = -1;
+    for (;;) {
+        const int k1 = std::min(rcap, k + 64);
+        hip_check(mgw::launch_cd_steps(eri, dpq, nmo, N, Lc, dD, dbm, dpiv, dpval, drank, k, k1,
+                                       ctx->cd_tol, st),
+                  "decomposition steps");
+        ctx->d2h(&rank, drank, 1);
+        ctx->sync();
+        k = k1;
+        if (rank >= 0) break;
+        if (k >= N) {               // every pair is a pivot: full rank
+            rank = N;
+            break;
+        }
+        if (k >= rcap) {            // grow the factor, keeping its columns
+            const int ncap = std::min(N, 2 * rcap);
+            mgw_hip_ctx::Buf grown;
+            double* Ln = as<double>(ctx->get(grown, (size_t)ncap * col_bytes, "decomposition factor"));
+            std::swap(grown, ctx->cdl);
+            hip_check(hipMemcpyAsync(Ln, Lc, (size_t)rcap * col_bytes, hipMemcpyDeviceToDevice, st),
+                      "hipMemcpyAsync D2D");
+            ctx->sync();
+            ctx->release(grown);
+            Lc = Ln;
+            rcap = ncap;
+        }
+    }
+    hip_check(mgw::launch_cd_residual(dD, N, dres, st), "decomposition residual");
+    double res[2] = {0.0, 0.0}, p0 = 0.0;
+    ctx->d2h(res, dres, 2);
+    if (rank > 0) ctx->d2h(&p0, dpval, 1);
+    ctx->sync();
+    if (res[1] != 0.0)
+        fail(MGW_ERR_NUMERIC, "ERI decomposition: %g non-finite residual diagonal entries", res[1]);
+    if (res[0] < -1e-8 * p0)
+        fail(MGW_ERR_UNSUPPORTED,
+             "ERI decomposition: the pair matrix is not positive semidefinite (residual diagonal "
+             "%g, first pivot %g); use MGW_PREC_FP64",
+             res[0], p0);
+
+    CdResult r;
+    r.rank = rank;
+    r.rpad = mgw::round_up(std::max(rank, 1), mgw::kCT);
+    r.bt = as<double>(ctx->get(ctx->cdb, (size_t)r.rpad * np * 8, "B^T"));
+    r.lx = as<double>(ctx->get(ctx->cdx, (size_t)mpad * r.rpad * 8, "pair vectors"));
+    r.nrm = as<double>(ctx->get(ctx->cdn, (size_t)mpad * 8, "pair vector norms"));
+    hip_check(mgw::launch_cd_gather(Lc, N, didx + N, didx + N + np, rank, L.n, mu, r.bt, r.lx,
+                                    r.nrm, r.rpad, np, mpad, st),
+              "gather pair vectors");
+    return r;
+}
+
 }  // namespace
 
 // Frequency-batch sizes picked by mgw_hip_autotune, per (nmo, nocc, nstates,
@@ -422,6 +552,14 @@ int mgw_hip_set_graphs(mgw_hip_ctx* ctx, int32_t enable) {
         need(ctx != nullptr, "mgw_hip_set_graphs: ctx is NULL");
         ctx->use_graphs = enable != 0;
         if (!ctx->use_graphs) ctx->drop_graphs();
+    });
+}
+
+int mgw_hip_set_cd_tolerance(mgw_hip_ctx* ctx, double tol) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_cd_tolerance: ctx is NULL");
+        need(tol >= 0.0 && tol < 1.0, "mgw_hip_set_cd_tolerance: tol must be in [0, 1)");
+        ctx->cd_tol = tol;
     });
 }
 
@@ -637,6 +775,7 @@ void finish_eri_upload(mgw_hip_ctx* ctx, int nmo, int nocc) {
     ctx->d2h(&asym, dflag, 1);
     ctx->sync();
     ctx->eri_pk_sym = (asym == 0);
+    ctx->eri_pair_sym = -1;
     ctx->nmo = nmo;
     ctx->nocc = nocc;
 }
@@ -738,8 +877,9 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             fail(MGW_ERR_UNSUPPORTED,
                  "eta != 0 (complex screening) runs in MGW_PREC_FP64 only on the hip backend");
         need(job->precision == MGW_PREC_FP64 || job->precision == MGW_PREC_MIXED ||
-                 job->precision == MGW_PREC_MIXED_INT8,
+                 job->precision == MGW_PREC_MIXED_INT8 || job->precision == MGW_PREC_FP64_CD,
              "mgw_hip_g0w0: unknown precision %d", job->precision);
+        const bool cd = job->precision == MGW_PREC_FP64_CD;
         std::vector<int> states;
         if (job->nstates <= 0 || !job->states) {
             states.resize(nmo);
@@ -753,11 +893,13 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         }
         const int ns = (int)states.size();
         const PhLayout L = ph_layout(nmo, nocc);
-        const int n = L.n, np = L.np, nblk = np / mgw::kNB;
+        const int n = L.n, np = L.np;
         const int m = ns * nmo;
         // Contraction columns.  With (pk|ia) == (kp|ia) exactly, q(p,k) = q(k,p):
         // each unordered pair {p,k} with p or k among the states is contracted once
-        // and colmap sends every (state, k) to its column.
+        // and colmap sends every (state, k) to its column.  The decomposed form
+        // has one vector per unordered pair (its ERI symmetry is checked to a
+        // tolerance) and always contracts unordered pairs.
         std::vector<int> colmap(m);
         std::vector<std::pair<int, int>> cols;
         {
@@ -767,7 +909,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             for (int s = 0; s < ns; ++s)
                 for (int k = 0; k < nmo; ++k) {
                     const int p = states[s];
-                    if (ctx->eri_pk_sym && k < p && slot[k] >= 0) {
+                    if ((ctx->eri_pk_sym || cd) && k < p && slot[k] >= 0) {
                         colmap[(size_t)s * nmo + k] = colmap[(size_t)slot[k] * nmo + p];
                     } else {
                         colmap[(size_t)s * nmo + k] = (int)cols.size();
@@ -802,8 +944,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         } evg{ev};
         hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
 
-        // resident per-molecule data
-        double* dV = as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
+        // resident per-molecule data (the decomposed form needs no V_ph)
+        double* dV = cd ? nullptr : as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
         double* dom = as<double>(ctx->get(ctx->homega, (size_t)nfreq * 8, "omega"));
         double* dwt = as<double>(ctx->get(ctx->hweight, (size_t)nfreq * 8, "weight"));
         double* dmo = as<double>(ctx->get(ctx->hmo, (size_t)(nmo + np) * 8, "mo/delta"));
@@ -822,20 +964,43 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->h2d(dpk, pkrow.data(), mpad);
         int* dcolmap = as<int>(ctx->get(ctx->hcolmap, (size_t)m * 4, "colmap"));
         ctx->h2d(dcolmap, colmap.data(), m);
-        hip_check(mgw::launch_build_v(as<double>(ctx->eri.p), as<int64_t>(ctx->rowbase.p),
-                                      as<int>(ctx->colidx.p), dV, n, np, st),
-                  "build V_ph");
-        ctx->n = n;
-        ctx->np = np;
-        ctx->v_ready = true;
+        if (!cd) {
+            hip_check(mgw::launch_build_v(as<double>(ctx->eri.p), as<int64_t>(ctx->rowbase.p),
+                                          as<int>(ctx->colidx.p), dV, n, np, st),
+                      "build V_ph");
+            ctx->n = n;
+            ctx->np = np;
+            ctx->v_ready = true;
+        }
         ctx->wc_ready = false;
+        // Decomposed form: M = L L^T over unique pairs, then B^T (ph rows of L)
+        // and the pair vectors of the contraction columns; fp = order of the
+        // factored matrices (the rank, padded) instead of np.
+        int fp = np, cd_rank = 0;
+        double cd_ms = 0.0;
+        double *cdBt = nullptr, *cdLx = nullptr, *cdnrm = nullptr;
+        if (cd) {
+            const auto c0 = std::chrono::steady_clock::now();
+            const CdResult r = cd_decompose(ctx, L, cols, mpad);
+            cd_rank = r.rank;
+            fp = r.rpad;
+            cdBt = r.bt;
+            cdLx = r.lx;
+            cdnrm = r.nrm;
+            cd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
+                        .count();
+        }
         const bool mixed = job->precision == MGW_PREC_MIXED;
         const bool i8 = job->precision == MGW_PREC_MIXED_INT8;
         const mgw::Split16 fmt = mgw::Split16::FP16;
         uint16_t *xh = nullptr, *xl = nullptr;
         double* xs = nullptr;
         double* xp = nullptr;
-        if (!mixed && !i8) {
+        if (cd) {
+            xp = cdLx;
+            xs = cdnrm;
+            dV = cdBt;
+        } else if (!mixed && !i8) {
             xp = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X"));
             hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, n,
                                          np, mpad, st),
@@ -865,9 +1030,10 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         size_t room = free_b > (size_t)(2ull << 30) ? free_b - (size_t)(2ull << 30) : free_b / 2;
         if (ctx->budget) room = std::min(room, ctx->budget > ctx->held ? ctx->budget - ctx->held : 0);
-        const size_t per = batch_bytes(np, mpad, job->precision, cplx);
+        const size_t per = cd ? batch_bytes(fp, mpad, MGW_PREC_FP64) + (size_t)np * 8
+                              : batch_bytes(np, mpad, job->precision, cplx);
         int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
-        nb = std::min(nb, 64);
+        nb = std::min(nb, cd ? 256 : 64);
         const Tuned tn = tuned(nmo, nocc, ns, job->precision);
         const int pick = ctx->forced_nb ? ctx->forced_nb : tn.nb;
         if (pick > 0) nb = std::min(nb, pick);
@@ -876,9 +1042,10 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                        : tn.uf ? tn.uf
                                : 4;
         const size_t zf = cplx ? 2 : 1;    // complex entries are two doubles
-        double* A = as<double>(ctx->get(ctx->hA, zf * nb * np * np * 8, "dielectric batch"));
+        const int fnblk = fp / mgw::kNB;
+        double* A = as<double>(ctx->get(ctx->hA, zf * nb * fp * fp * 8, "dielectric batch"));
         double* S = as<double>(ctx->get(ctx->hS, zf * nb * np * 8, "scale batch"));
-        double* dinv = as<double>(ctx->get(ctx->hdinv, zf * nb * nblk * 64 * 64 * 8, "dinv"));
+        double* dinv = as<double>(ctx->get(ctx->hdinv, zf * nb * fnblk * 64 * 64 * 8, "dinv"));
         double* Y = cplx ? as<double>(ctx->get(ctx->hY, (size_t)nb * np * mpad * 16, "complex panel"))
                          : nullptr;
         char *fr = nullptr, *fc = nullptr;
@@ -886,11 +1053,11 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np), "factor row slices"));
             fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np), "factor column slices"));
         }
-        double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * np * 64 * 8, "trtri scratch"));
+        double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * fp * 64 * 8, "trtri scratch"));
         int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
         int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
         double* qpart = as<double>(
-            ctx->get(ctx->hqpart, (size_t)nb * (np / mgw::kCT) * mpad * 8, "qpart"));
+            ctx->get(ctx->hqpart, (size_t)nb * (fp / mgw::kCT) * mpad * 8, "qpart"));
         uint16_t *mh = nullptr, *ml = nullptr;
         double* ms = nullptr;
         if (mixed) {
@@ -922,6 +1089,24 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
                 hip_check(mgw::launch_sumsq_c(Y, dcolmap, dq, dqi, np, m, mpad, f0p, b, st),
                           "q (complex)");
+                if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
+                return;
+            }
+            if (cd) {
+                if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+                hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
+                hip_check(mgw::launch_aux_dielectric(cdBt, S, A, fp, np, b, st),
+                          "pair-space dielectric");
+                if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+                factor_batch(ctx, A, dinv, info, tbuf, fp, b);
+                if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
+                if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
+                hip_check(mgw::launch_contract_f64(A, nullptr, cdLx, qpart, fp, mpad, b, st),
+                          "contract (pair space)");
+                if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
+                hip_check(mgw::launch_reduce_q_cd(qpart, cdnrm, dcolmap, dq, fp / mgw::kCT, m, mpad,
+                                                  f0p, b, st),
+                          "reduce q (pair space)");
                 if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
                 return;
             }
@@ -966,7 +1151,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             if (ctx->use_graphs && !cplx) {
                 // Replay a graph of this batch when one with the same shape and
                 // buffers was captured before (the offset is read on the device).
-                const GraphKey key{np, mpad, m, b, job->precision, uf,
+                const GraphKey key{fp, mpad, m, b, job->precision, uf,
                                    {A, S, dinv, tbuf, info, qpart, dq, xp, xh, xl, xs, mh, ml, ms, dV,
                                     ddelta, dom, dcolmap, f0p, fr, fc}};
                 hipGraphExec_t exec = ctx->find_graph(key);
@@ -1049,6 +1234,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->timings[9] = t_aux;
         ctx->timings[10] = mu;
         ctx->timings[11] = t_graph;
+        ctx->timings[12] = cd_rank;
+        ctx->timings[13] = cd_ms;
     });
 }
 
@@ -1310,7 +1497,7 @@ int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n) {
         g_thread_error = "mgw_hip_last_timings: bad argument";
         return MGW_ERR_ARG;
     }
-    for (int i = 0; i < n && i < 12; ++i) ms[i] = ctx->timings[i];
+    for (int i = 0; i < n && i < 14; ++i) ms[i] = ctx->timings[i];
     return MGW_OK;
 }
 

@@ -89,7 +89,8 @@ hipError_t launch_pack_x(const double* src, const int64_t* pkrow, const int* col
                          int n, int np, int mpad, hipStream_t s);
 
 // qpart[w][rt][c] = sum_{r in row tile rt} (sum_k M_w[r][k] S_w[k] xp[c][k])^2,
-// M_w lower triangular, 128-row tiles (np, mpad multiples of 128).
+// M_w lower triangular, 128-row tiles (np, mpad multiples of 128); S == NULL
+// means S_w = 1.
 hipError_t launch_contract_f64(const double* M, const double* S, const double* xp, double* qpart,
                                int np, int mpad, int nb, hipStream_t s);
 
@@ -183,6 +184,36 @@ hipError_t launch_contract_i8(const int8_t* mplanes, const double* mscale, const
                               double* qpart, int np, int mpad, int nb, int uf, hipStream_t s);
 int contract_i8_row_tiles(int np);
 hipError_t launch_probe_i8(const int8_t* a, const int8_t* b, int* d, int ntests, hipStream_t s);
+
+// Cholesky-decomposed ERI form (mgw_cd.hip).  Pairs u < N are unique orbital
+// pairs, pq[u] = p * nmo + q with p <= q.  L is column-major: column j of
+// all pairs at Lc + j * Npad, Npad = cd_blocks(N) * 64.  bm: 2 * cd_blocks(N)
+// double2 per-block maxima.  launch_cd_steps runs pivots [k0, k1) and stops
+// (rank = the pivot count) once the largest residual diagonal is <= tol times
+// the first pivot; rank stays -1 while it has not converged.
+int cd_blocks(int N);
+hipError_t launch_cd_init(const double* eri, const int* pq, int nmo, int N, double* D, double2* bm,
+                          int* rank, hipStream_t s);
+hipError_t launch_cd_steps(const double* eri, const int* pq, int nmo, int N, double* Lc, double* D,
+                           double2* bm, int* piv, double* pval, int* rank, int k0, int k1,
+                           double tol, hipStream_t s);
+// out (3 x uint64, zeroed by the caller) = bit patterns of max |M[u][v] -
+// M[v][u]|, max |(pk|ia) - (kp|ia)| and max |M[u][v]|.
+hipError_t launch_cd_asymmetry(const double* eri, const int* pq, int nmo, int nocc, int N,
+                               unsigned long long* out, hipStream_t s);
+// out[0] = min residual diagonal, out[1] = count of non-finite entries.
+hipError_t launch_cd_residual(const double* D, int N, double* out, hipStream_t s);
+// Bt (rpad x np): row j = L column j over the ph pairs phu[ph]; Lx (mpad x
+// rpad): row c = L over colu[c] for c < mu, and nrm[c] = ||Lx[c]||^2.
+hipError_t launch_cd_gather(const double* Lc, int N, const int* phu, const int* colu, int r, int n,
+                            int mu, double* Bt, double* Lx, double* nrm, int rpad, int np, int mpad,
+                            hipStream_t s);
+// Lower 128-tiles of C_w = I + Bt diag(S_w^2) Bt^T (rpad x rpad, row-major, per w).
+hipError_t launch_aux_dielectric(const double* Bt, const double* S, double* C, int rpad, int np,
+                                 int nb, hipStream_t s);
+// q[*f0p + w][c] = sum_rt qpart[w][rt][colmap[c]] - nrm[colmap[c]].
+hipError_t launch_reduce_q_cd(const double* qpart, const double* nrm, const int* colmap, double* q,
+                              int nrt, int m, int mpad, const int* f0p, int nb, hipStream_t s);
 
 // Complex path for eta != 0 (mgw_complex.hip).  Complex arrays are
 // interleaved (re, im) doubles; matrices row-major np x np, panels np x mpad.

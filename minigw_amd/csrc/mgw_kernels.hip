@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_contract_f64(
     int lin = blockIdx.x + ncol * blockIdx.y;
     const int rt = nrow - 1 - lin / ncol, r0 = rt * kCT, c0 = (lin % ncol) * kCT;
     const double* Mw = M + (size_t)w * np * np;
-    const double* Sw = S + (size_t)w * np;
+    const double* Sw = S ? S + (size_t)w * np : nullptr;     // NULL: no column scaling
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int li = lane & 15, lk = lane >> 4;
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256) void k_contract_f64(
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int kk = k0 + lkq + u;
-            ra[u] = (kk <= r0 + lrow) ? arow[kk] * Sw[kk] : 0.0;
+            ra[u] = (kk <= r0 + lrow) ? (S ? arow[kk] * Sw[kk] : arow[kk]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 8; u += 2) {

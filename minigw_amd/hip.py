@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libm
 
 MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_MEMORY, \
     MGW_ERR_STATE, MGW_ERR_IO = range(8)
-PREC_FP64, PREC_MIXED, PREC_MIXED_INT8 = 0, 1, 2
+PREC_FP64, PREC_MIXED, PREC_MIXED_INT8, PREC_FP64_CD = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -86,6 +86,7 @@ def _load():
                                       _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _ip]),
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
         "mgw_hip_set_graphs": (C.c_int, [vp, C.c_int32]),
+        "mgw_hip_set_cd_tolerance": (C.c_int, [vp, C.c_double]),
         "mgw_hip_autotune": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_double)]),
         "mgw_hip_autotune_result": (C.c_int, [C.POINTER(G0W0Job), C.POINTER(C.c_int32),
@@ -206,15 +207,20 @@ class HipContext:
         """Capture/replay each frequency batch as a hipGraph (no per-phase timings then)."""
         self.call("mgw_hip_set_graphs", 1 if enable else 0)
 
+    def set_cd_tolerance(self, tol: float):
+        """Stopping tolerance of the PREC_FP64_CD pair-matrix decomposition
+        (relative to its first pivot; default 1e-12)."""
+        self.call("mgw_hip_set_cd_tolerance", float(tol))
+
     def release_buffers(self):
         self.call("mgw_hip_release_buffers")
 
     def last_timings(self) -> dict:
-        t = np.zeros(12)
-        self.call("mgw_hip_last_timings", _ptr(t), 12)
+        t = np.zeros(14)
+        self.call("mgw_hip_last_timings", _ptr(t), 14)
         keys = ["build_v", "dielectric", "factorize", "contract", "sigma_c", "pade_qp", "total",
                 "freq_batch", "contract_launches", "split_reduce", "contract_columns",
-                "graph_batches"]
+                "graph_batches", "cd_rank", "cd_wall_ms"]
         return dict(zip(keys, t.tolist()))
 
     def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:

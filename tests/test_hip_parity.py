@@ -312,3 +312,111 @@ def test_mixed_int8_vs_fp64(hip, ctx, case):
         env = qp_envelope(inp, f64.omegas, f64.sigma_x_diag, f64.sigma_c, list(range(inp.nmo)),
                           16, max(d, 1e-15))
         assert np.all(mev <= np.maximum(1.0, 10 * env * orc.K_HARTREE_TO_EV * 1000)), (mev, env)
+
+
+# ------------------------------------------------------------ decomposed-ERI form
+def _golden_case(name):
+    r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
+    if name == "h2o_all":
+        inp = h2o_input()
+    elif name == "syn_c1":
+        inp = synthetic_input(*CONFIGS["c1"])
+    else:
+        inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
+                      fermi_energy=float(r["fermi_energy"]))
+    return r, inp
+
+
+@pytest.mark.parametrize("name", ["h2o_all", "syn_c1", "syn_small", "syn_odd"])
+def test_fp64_cd_vs_reference_path(hip, ctx, name):
+    """Decomposed-ERI form (device pivoted Cholesky of the pair matrix, pair-space
+    screening) against the reference path's all-state outputs: Sigma_c to
+    1e-10 absolute, QP energies to 1e-8 Ha or inside the noise envelope.  The
+    synthetic ERIs have rank <= 3 nmo, which the decomposition finds."""
+    r, inp = _golden_case(name)
+    say(f"case: FP64 decomposed-ERI form vs reference path, {name}")
+    nf, npar = int(r["nfreq"]), int(r["npar"])
+    res = hip.run_g0w0(inp, nf, npar, ctx=ctx, precision=hip.PREC_FP64_CD)
+    rank = int(ctx.last_timings()["cd_rank"])
+    dsig = np.abs(res.sigma_c - r["sigma_c"]).max()
+    say(f"   rank {rank}; max |dSigma_c| = {dsig:.3e}")
+    npairs = inp.nmo * (inp.nmo + 1) // 2
+    assert 0 < rank <= (npairs if name == "h2o_all" else min(npairs, 3 * inp.nmo))
+    assert dsig < 1e-10
+    env = qp_envelope(inp, r["omega"], r["sigma_x_diag"], r["sigma_c"], list(range(inp.nmo)),
+                      npar, max(dsig / np.abs(r["sigma_c"]).max(), 1e-15))
+    dq = np.abs(res.qp - r["qp"])
+    say(f"   QP: {np.sum(dq < 1e-8)}/{inp.nmo} states within 1e-8 Ha; max {dq.max():.2e}")
+    assert np.all(dq <= np.maximum(1e-8, 10 * env)), (dq, env)
+
+
+def test_fp64_cd_h2o_state5_fixture(hip, ctx):
+    """The reference's h2o_serial E_c_before_Pade.out through the decomposed form,
+    with compare_reference.py's 1e-8 component check."""
+    say("case: H2O state 5 fixture, decomposed-ERI form")
+    inp = h2o_input()
+    fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
+    res = hip.run_g0w0(inp, 200, 16, states=[4], ctx=ctx, precision=hip.PREC_FP64_CD)
+    sc, ref = res.sigma_c[0], fx["e_c_before_pade"]
+    for x, y in zip(sc, ref):
+        assert math.isclose(x.real, y.real, rel_tol=1e-8, abs_tol=1e-8)
+        assert math.isclose(x.imag, y.imag, rel_tol=1e-8, abs_tol=1e-8)
+    say(f"   max |dSigma_c| = {np.abs(sc - ref).max():.3e}")
+    assert np.abs(sc - ref).max() < 1e-10
+    row = fx["summary_row_orbital5_ev"]
+    assert math.isclose(res.qp[4] * orc.K_HARTREE_TO_EV, row[4], rel_tol=1e-5, abs_tol=1e-5)
+
+
+def test_fp64_cd_grows_factor_and_matches_fp64(hip, ctx):
+    """A pair matrix of rank 300 (> the decomposition's first capacity of 256
+    columns at nmo = 40) exercises the factor's growth; the result matches
+    the n x n FP64 form."""
+    from minigw_amd.synthetic import eri_from_factor, synthetic_factor
+    say("case: decomposed form, rank 300 at nmo 40 (factor growth) vs FP64 form")
+    nmo, nocc, k = 40, 8, 300
+    mo, fermi, _, vxc = synthetic_factor(nmo, nocc, 11)
+    rng = np.random.default_rng(4)
+    b = rng.normal(0, 0.05, (nmo, nmo, k))
+    b = 0.5 * (b + b.transpose(1, 0, 2))
+    inp = GwInput(mo_energy=mo, eri=eri_from_factor(b.reshape(nmo * nmo, k), nmo), vxc=vxc,
+                  nocc=nocc, fermi_energy=fermi)
+    f64 = hip.run_g0w0(inp, 24, 16, ctx=ctx)
+    cdr = hip.run_g0w0(inp, 24, 16, ctx=ctx, upload=False, precision=hip.PREC_FP64_CD)
+    rank = int(ctx.last_timings()["cd_rank"])
+    d = np.abs(cdr.sigma_c - f64.sigma_c).max()
+    say(f"   rank {rank}; max |dSigma_c| vs FP64 form = {d:.3e}")
+    assert rank == k
+    assert d < 1e-11
+
+
+def test_fp64_cd_tolerance_and_errors(hip, ctx):
+    """A loose stopping tolerance lowers the rank; eta != 0, an ERI without
+    pair symmetry and a tolerance outside [0, 1) are refused."""
+    say("case: decomposed form: tolerance, unsupported inputs, argument errors")
+    inp = synthetic_input(16, 4, 5)
+    tight = hip.run_g0w0(inp, 16, 8, ctx=ctx, precision=hip.PREC_FP64_CD)
+    r_tight = int(ctx.last_timings()["cd_rank"])
+    try:
+        ctx.set_cd_tolerance(0.2)
+        loose = hip.run_g0w0(inp, 16, 8, ctx=ctx, upload=False, precision=hip.PREC_FP64_CD)
+        r_loose = int(ctx.last_timings()["cd_rank"])
+    finally:
+        ctx.set_cd_tolerance(1e-12)
+    say(f"   rank {r_tight} at 1e-12, {r_loose} at 0.2")
+    assert r_loose < r_tight
+    assert np.all(np.isfinite(loose.sigma_c))
+    assert np.abs(loose.sigma_c - tight.sigma_c).max() > 1e-9
+    with pytest.raises(hip.HipError) as e:
+        ctx.set_cd_tolerance(1.0)
+    assert e.value.code == hip.MGW_ERR_ARG
+    with pytest.raises(hip.HipError) as e:
+        hip.run_g0w0(inp, 16, 8, eta=0.01, ctx=ctx, upload=False, precision=hip.PREC_FP64_CD)
+    assert e.value.code == hip.MGW_ERR_UNSUPPORTED
+    bad = GwInput(mo_energy=inp.mo_energy, eri=inp.eri.copy(), vxc=inp.vxc, nocc=inp.nocc,
+                  fermi_energy=inp.fermi_energy)
+    bad.eri[0, 0, 1, 2] += 1e-3          # (00|12) != (12|00); (pk|ia) block untouched
+    with pytest.raises(hip.HipError) as e:
+        hip.run_g0w0(bad, 16, 8, ctx=ctx, precision=hip.PREC_FP64_CD)
+    assert e.value.code == hip.MGW_ERR_UNSUPPORTED
+    ok = hip.run_g0w0(inp, 16, 8, ctx=ctx, precision=hip.PREC_FP64_CD)
+    assert np.abs(ok.sigma_c - tight.sigma_c).max() == 0.0
