@@ -4,7 +4,10 @@ A step is one molecule's G0W0 on the device: all imaginary frequencies of the
 transformed Gauss-Legendre grid, every state's Sigma_c(i w), Pade fit and QP
 fixed point, on the ERI already resident in HBM.  value = QP solves per second
 (one QP solve = one state's QP energy) of the mixed-precision form; the plain
-FP64 form runs beside it as the error judge and for the mixed/FP64 ratio.
+FP64 form runs beside it as the error judge and for the mixed/FP64 ratio, and
+the decomposed-ERI FP64 form (pivoted Cholesky of the pair matrix on the
+device, screening in the rank-r pair space) is reported as
+"decomposed_fp64" with its own accuracy against the plain form.
 
 Prints one JSON line.  Default workload: the synthetic nmo=200, nocc=40
 molecule (nph=6400, ERI 12.8 GB), 200 frequencies, 16 Pade parameters.
@@ -44,10 +47,12 @@ HA_MEV = 27.211386245988 * 1e3
 # what the counter reads for a 2 GiB copy), averaged over the molecule's
 # launches, from two rocprofv3 --pmc runs of this bench; the commands and
 # per-dispatch figures are in profiles/kernels_c3_nmo200_mixed_fp64.md
-# (FP16 split and FP64) and profiles/kernels_c3_nmo200_int8_fp64.md (int8).
-TRAFFIC = {("k_contract_i8", "c3", 200): (309.0e9 + 0.39e9),          # ..._int8_fp64.md
-           ("k_contract_split<FP16>", "c3", 200): (130.6e9 + 0.19e9),
-           ("k_contract_f64", "c3", 200): (1421.2e9 + 0.39e9)}
+# (FP16 split and FP64), profiles/kernels_c3_nmo200_int8_fp64.md (int8) and
+# profiles/kernels_c3_nmo200_fp64_cd.md (decomposed-ERI form).
+TRAFFIC = {("mixed_int8", "c3", 200): (309.0e9 + 0.39e9),          # ..._int8_fp64.md
+           ("mixed", "c3", 200): (130.6e9 + 0.19e9),
+           ("fp64", "c3", 200): (1421.2e9 + 0.39e9),
+           ("fp64_cd", "c3", 200): (66.45e9 + 0.16e9)}       # ..._fp64_cd.md
 
 
 def accuracy(hip, ctx, inp, mix, f64, npar: int, trials: int = 3) -> dict:
@@ -199,6 +204,8 @@ FORMS = {
     "mixed": (1, "k_contract_split<FP16>", 3, 2.5e15, "fp16_mfma_32x32", "fp16x3-split+fp64"),
     "mixed_int8": (2, "k_contract_i8", 15, 5.0e15, "int8_mfma_32x32", "int8x5-slice+fp64"),
     "fp64": (0, "k_contract_f64", 1, 78.6e12, "fp64_mfma", "fp64"),
+    # FP64 with the ERI decomposed on the device (pair-space screening, rank r)
+    "fp64_cd": (3, "k_contract_f64", 1, 78.6e12, "fp64_mfma", "fp64"),
 }
 
 
@@ -209,13 +216,15 @@ def roofline(form: str, ph: dict, n: int, m: int, nfreq: int, config: str, peaks
     launches = max(ph["contract_launches"], 1.0)
     t_launch = ph["contract"] / launches * 1e-3
     cols = ph.get("contract_columns", m) or m      # unordered (p,k) pairs when the ERI is symmetric
-    # launches are nb frequencies each except the last: average work per launch
-    work = contraction_flops(n, cols, 0) * products * nfreq / launches
+    # launches are nb frequencies each except the last: average work per launch;
+    # the decomposed form contracts over the rank r of the ERI's pair matrix
+    k = int(ph["cd_rank"]) if form == "fp64_cd" else n
+    work = contraction_flops(k, cols, 0) * products * nfreq / launches
     achieved = work / t_launch / 1e12
     return {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 2),
             "peak": peak / 1e12, "unit": "TOP/s" if form == "mixed_int8" else "TFLOP/s",
             "frac": round(achieved * 1e12 / peak, 4),
-            "traffic": TRAFFIC.get((kernel, config, nfreq)),
+            "traffic": TRAFFIC.get((form, config, nfreq)),
             "launch_ms": round(t_launch * 1e3, 3), "work_per_launch": work,
             "measured_peak": round(peaks[key] / 1e12, 1)}
 
@@ -232,6 +241,8 @@ def main() -> int:
     ap.add_argument("--alt-precision", default="mixed", choices=list(FORMS) + ["none"],
                     help="second mixed form, timed and judged beside the headline one")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cd-steps", type=int, default=3,
+                    help="steps of the decomposed-ERI FP64 form reported beside (0: skip)")
     ap.add_argument("--cpu-sample-freqs", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-molecules", type=int, default=32)
@@ -299,6 +310,10 @@ def main() -> int:
             if alt:
                 step(FORMS[alt][0])
                 alt_res, alt_lat, alt_phase = timed(FORMS[alt][0], 1, alt)
+        cd_res = None
+        if prec != 3 and args.cd_steps > 0 and f64 is not None:
+            step(3)
+            cd_res, cd_lat, cd_phase = timed(3, args.cd_steps, "fp64_cd")
         report = {}
         if f64 is not None:
             log("bench: accuracy report ...")
@@ -308,6 +323,9 @@ def main() -> int:
             report["accuracy_vs_fp64_h2o"] = h2o_check(hip, ctx, args.npar, prec)
             if alt_res is not None:
                 report["alt_accuracy_h2o"] = h2o_check(hip, ctx, args.npar, FORMS[alt][0])
+            if cd_res is not None:
+                report["cd_accuracy"] = accuracy(hip, ctx, inp, cd_res, f64, args.npar)
+                report["cd_accuracy_h2o"] = h2o_check(hip, ctx, args.npar, 3)
 
     ms = elapsed / args.steps * 1e3
     value = nmo * args.steps / elapsed
@@ -340,6 +358,15 @@ def main() -> int:
                           "ms_per_step": round(float(np.mean(alt_lat)) * 1e3, 2),
                           "phases_ms": {k: round(v, 3) for k, v in alt_phase.items()},
                           "roofline": roofline(alt, alt_phase, n, m, args.nfreq, args.config, peaks)}
+        if cd_res is not None:
+            cv = nmo * len(cd_lat) / sum(cd_lat)
+            out["decomposed_fp64"] = {
+                "precision": "fp64_cd", "value": round(cv, 3), "over_fp64": round(cv / fv, 3),
+                "ms_per_step": round(float(np.mean(cd_lat)) * 1e3, 2),
+                "p50_ms": round(float(np.percentile(np.array(cd_lat) * 1e3, 50)), 2),
+                "rank": int(cd_phase["cd_rank"]),
+                "phases_ms": {k: round(v, 3) for k, v in cd_phase.items()},
+                "roofline": roofline("fp64_cd", cd_phase, n, m, args.nfreq, args.config, peaks)}
         out.update(report)
     if args.batch_molecules > 0:
         log("bench: uneven-nmo batch ...")

@@ -310,7 +310,9 @@ int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight);
 
 /* HBM bytes mgw_hip_g0w0 needs for one molecule with nbatch frequencies
  * resident at once (ERI + V + dielectric batch + contraction scratch);
- * the admission estimate of the request loop. */
+ * the admission estimate of the request loop.  For MGW_PREC_FP64_CD it
+ * assumes a decomposition rank of at most 4 nmo (ERI + factor + pair-space
+ * batch). */
 size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t nstates,
                               int32_t nbatch, int32_t precision);
 

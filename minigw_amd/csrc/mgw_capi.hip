@@ -1508,6 +1508,18 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
     const int ns = nstates > 0 ? nstates : nmo;
     const int m = ns * nmo, mpad = mgw::round_up(m, mgw::kPad);
     const size_t eri = (size_t)nmo * nmo * nmo * nmo * 8;
+    if (precision == MGW_PREC_FP64_CD) {
+        // decomposed form, sized for a rank up to 4 nmo (the factor's first
+        // capacity; a larger rank grows it, which the budget then checks)
+        const int N = nmo * (nmo + 1) / 2, Npad = mgw::round_up(N, 64);
+        const int cap = std::min(N, mgw::round_up(std::max(256, 4 * nmo), 64));
+        const int rpad = mgw::round_up(std::min(N, 4 * nmo), mgw::kCT);
+        const size_t fixed = eri + (size_t)nfreq * m * 8 + (size_t)cap * Npad * 8 +
+                             (size_t)Npad * 24 + (size_t)rpad * L.np * 8 +
+                             (size_t)mpad * (rpad + 1) * 8 + ((size_t)N + L.np + mpad) * 4;
+        const int nb = std::max(1, std::min({nbatch > 0 ? nbatch : nfreq, nfreq, 256}));
+        return fixed + (size_t)nb * (batch_bytes(rpad, mpad, MGW_PREC_FP64) + (size_t)L.np * 8);
+    }
     const size_t v = (size_t)L.np * L.np * 8;
     size_t fixed = eri + v + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
     fixed += precision == MGW_PREC_MIXED        ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8

@@ -2,7 +2,7 @@
 
     python -m minigw_amd.run --input-dir DIR [--output-dir DIR] [--freq-points 200]
         [--pade-params 16] [--state 5 | --all-states] [--eta 0.0]
-        [--precision fp64|mixed|mixed_int8]
+        [--precision fp64|mixed|mixed_int8|fp64_cd] [--cd-tolerance 1e-12]
 
 The options, defaults and outputs follow miniGW's CLI (src/cli.cpp,
 include/cli.hpp:10-22; src/main.cpp:38-133): DIR/gw_input.h5 (or the raw form
@@ -30,7 +30,9 @@ def main(argv=None) -> int:
     ap.add_argument("--state", type=int, default=5)
     ap.add_argument("--all-states", action="store_true")
     ap.add_argument("--eta", type=float, default=0.0)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed_int8"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed_int8", "fp64_cd"])
+    ap.add_argument("--cd-tolerance", type=float, default=1e-12,
+                    help="fp64_cd: ERI decomposition stops at this fraction of the first pivot")
     ap.add_argument("--linalg-backend", default="hip", choices=["hip"])
     args = ap.parse_args(argv)
     if args.freq_points <= 0:
@@ -61,8 +63,9 @@ def main(argv=None) -> int:
     if not all(0 <= s < nmo for s in states):
         raise SystemExit("Selected state is out of range")
     prec = {"fp64": hip.PREC_FP64, "mixed": hip.PREC_MIXED,
-            "mixed_int8": hip.PREC_MIXED_INT8}[args.precision]
+            "mixed_int8": hip.PREC_MIXED_INT8, "fp64_cd": hip.PREC_FP64_CD}[args.precision]
     with hip.HipContext(hip.device_for_local_rank(0, 1)) as ctx:
+        ctx.set_cd_tolerance(args.cd_tolerance)
         t1 = time.perf_counter()
         res = hip.run_g0w0(inp, args.freq_points, args.pade_params, states=states, eta=args.eta,
                            precision=prec, ctx=ctx)
