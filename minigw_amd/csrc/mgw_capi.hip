@@ -144,6 +144,7 @@ struct mgw_hip_ctx {
     // Cholesky-decomposed ERI form: pair tables, decomposition workspace,
     // factor columns, and B^T / pair vectors / their norms
     Buf cdidx, cdwork, cdl, cdb, cdx, cdn;
+    Buf hsg;                          // Sigma_c quadrature factors g(n, f, k)
     int eri_pair_sym = -1;            // symmetric to 1e-10 for the decomposed form: -1 unchecked
     double eri_asym = 0.0, eri_absmax = 0.0;
     double cd_tol = 1e-12;            // stop when the residual diagonal <= cd_tol * first pivot
@@ -155,7 +156,7 @@ struct mgw_hip_ctx {
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
-               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn};
+               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -297,7 +298,8 @@ size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
                + 4;                          // info
     if (precision == MGW_PREC_MIXED) b += (size_t)np * np * 4 + (size_t)np * 8;  // M planes
     if (precision == MGW_PREC_MIXED_INT8) b += (size_t)np * np * mgw::kSlices + (size_t)np * 8;
-    if (precision != MGW_PREC_FP64) b += 2 * mgw::fact_slice_bytes(np);  // factor block slices
+    if (precision != MGW_PREC_FP64)     // factor block slices, second trtri slot
+        b += 2 * mgw::fact_slice_bytes(np) + (size_t)np * 64 * 8;
     return b;
 }
 
@@ -1025,6 +1027,17 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         }
         hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
 
+        // per-molecule outputs before the batch, which takes the room that is left
+        double* sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
+        double* sim = as<double>(ctx->get(ctx->hsig_im, (size_t)ns * nfreq * 8, "sigma im"));
+        double* sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
+        double* qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
+        int* iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
+        double2* gq = as<double2>(
+            ctx->get(ctx->hsg, (size_t)nfreq * nfreq * nmo * sizeof(double2), "quadrature factors"));
+        int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
+        int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
+
         // frequency batch size from the device budget
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
@@ -1053,9 +1066,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np), "factor row slices"));
             fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np), "factor column slices"));
         }
-        double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)nb * fp * 64 * 8, "trtri scratch"));
-        int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
-        int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
+        double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)(fr ? 2 : 1) * nb * fp * 64 * 8,
+                                           "trtri scratch"));
         double* qpart = as<double>(
             ctx->get(ctx->hqpart, (size_t)nb * (fp / mgw::kCT) * mpad * 8, "qpart"));
         uint16_t *mh = nullptr, *ml = nullptr;
@@ -1194,13 +1206,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->wc_ready = false;
 
         hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-        double* sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
-        double* sim = as<double>(ctx->get(ctx->hsig_im, (size_t)ns * nfreq * 8, "sigma im"));
-        double* sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
-        double* qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
-        int* iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
-        hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, sre, sim,
-                                      st),
+        hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, gq,
+                                      sre, sim, st),
                   "sigma_c");
         hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
         hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), sx, nmo, nocc, st), "sigma_x");
@@ -1514,14 +1521,16 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
         const int N = nmo * (nmo + 1) / 2, Npad = mgw::round_up(N, 64);
         const int cap = std::min(N, mgw::round_up(std::max(256, 4 * nmo), 64));
         const int rpad = mgw::round_up(std::min(N, 4 * nmo), mgw::kCT);
-        const size_t fixed = eri + (size_t)nfreq * m * 8 + (size_t)cap * Npad * 8 +
+        const size_t fixed = eri + (size_t)nfreq * nfreq * nmo * 16 + (size_t)nfreq * m * 8 +
+                             (size_t)cap * Npad * 8 +
                              (size_t)Npad * 24 + (size_t)rpad * L.np * 8 +
                              (size_t)mpad * (rpad + 1) * 8 + ((size_t)N + L.np + mpad) * 4;
         const int nb = std::max(1, std::min({nbatch > 0 ? nbatch : nfreq, nfreq, 256}));
         return fixed + (size_t)nb * (batch_bytes(rpad, mpad, MGW_PREC_FP64) + (size_t)L.np * 8);
     }
     const size_t v = (size_t)L.np * L.np * 8;
-    size_t fixed = eri + v + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
+    const size_t sg = (size_t)nfreq * nfreq * nmo * 16;     // quadrature factors
+    size_t fixed = eri + v + sg + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
     fixed += precision == MGW_PREC_MIXED        ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8
              : precision == MGW_PREC_MIXED_INT8 ? (size_t)mpad * L.np * mgw::kSlices + (size_t)mpad * 8
                                                 : (size_t)mpad * L.np * 8;

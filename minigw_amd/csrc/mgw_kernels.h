@@ -72,6 +72,8 @@ hipError_t launch_trtri_block(double* A, const double* dinv, const double* tbuf,
 // CP/CS hold the row- and column-scaled slices of every 64 x 64 block
 // (nb x fact_slice_bytes(np) bytes each: planes first, then nb x (np/64)^2 x
 // 64 scales as doubles at the byte offset nb x (np/64)^2 x kSlices x 4096).
+// Both sweeps update two block columns per launch; launch_trtri_i8's tbuf
+// holds two slots of nb x np x 64 doubles.
 size_t fact_slice_bytes(int np);
 hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
                            double* CS, int np, int nb, hipStream_t s);
@@ -146,10 +148,12 @@ int contract_split_row_tiles(int np);
 hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int n, int np,
                               hipStream_t s);
 
-// sigma[s][n] = (1/pi) sum_f sum_k -(g0(k, n, f) q[f][s*nmo + k]) w_f
+// sigma[s][n] = (1/pi) sum_f sum_k -(g0(k, n, f) q[f][s*nmo + k]) w_f; gbuf:
+// nfreq^2 * nmo complex scratch for the state-independent g0 factors.
 hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_energy,
                           const double* omega, const double* weight, double fermi, int nmo,
-                          int nstates, int nfreq, double* sig_re, double* sig_im, hipStream_t s);
+                          int nstates, int nfreq, double2* gbuf, double* sig_re, double* sig_im,
+                          hipStream_t s);
 
 // sx[p] = -sum_{i < nocc} eri[p, i, p, i]
 hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s);

@@ -191,53 +191,74 @@ __device__ inline void gemm64(const double* __restrict__ a, size_t lda,
 }
 
 // ------------------------------------------------------------------ potrf pieces
-// Diagonal block k of every matrix: Cholesky in LDS, then the inverse of the
-// 64 x 64 factor by column-wise forward substitution.
+// Diagonal block k of every matrix: Cholesky and the inverse of its factor,
+// held in registers: thread t owns row i = t / 4, columns c0 = 16 (t % 4) ..
+// c0 + 15 of the working block T (lower part of A_kk) and of B (starts as I,
+// ends as L^-1 up to row scaling).  Step j: the owners publish column j of T
+// and row j of B to LDS (double-buffered, one barrier per step); every row
+// i > j then applies the rank-1 update with the unscaled column and pivot
+// p_j = T[j][j]:  T[i][l] -= T[i][j] T[l][j] / p_j (j < l <= i),
+// B[i][l] -= T[i][j] B[j][l] / p_j (l <= j).  Finally L[i][c] = T[i][c] / sqrt(p_c)
+// (c <= i) and L^-1[i][c] = B[i][c] / sqrt(p_i).
 __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* __restrict__ dinv,
                                                int* __restrict__ info, int np, int k) {
-    __shared__ double L[64][65];
-    __shared__ double X[64][65];
+    __shared__ double col[2][64], row[2][64];
+    __shared__ double rs[64];                    // 1 / sqrt(p_j)
     const int w = blockIdx.x, t = threadIdx.x;
+    const int i = t >> 2, c0 = (t & 3) * 16;
     const int nblk = np / kNB;
     double* Aw = A + (size_t)w * np * np;
     const size_t off = (size_t)k * kNB * np + (size_t)k * kNB;
-    for (int e = t; e < 64 * 64; e += 256) {
-        const int i = e >> 6, j = e & 63;
-        L[i][j] = (j <= i) ? Aw[off + (size_t)i * np + j] : 0.0;
+    double T[16], B[16];
+    {
+        const double* src = Aw + off + (size_t)i * np + c0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            T[c] = (c0 + c <= i) ? src[c] : 0.0;
+            B[c] = (c0 + c == i) ? 1.0 : 0.0;
+        }
     }
-    bool bad = false;
+    int bad = 0;
     for (int j = 0; j < 64; ++j) {
-        __syncthreads();
-        const double piv = L[j][j];
-        if (!(piv > 0.0)) bad = true;
-        const double djj = sqrt(piv > 0.0 ? piv : 1.0);
-        __syncthreads();
-        if (t == j) L[j][j] = djj;
-        else if (t > j && t < 64) L[t][j] /= djj;
-        __syncthreads();
-        for (int e = t; e < 64 * 64; e += 256) {
-            const int i = e >> 6, l = e & 63;
-            if (l > j && l <= i) L[i][l] -= L[i][j] * L[l][j];
+        const int buf = j & 1;
+        if ((t & 3) == (j >> 4)) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c == (j & 15)) col[buf][i] = T[c];
         }
-        if (bad && t == 0 && info[w] == 0) info[w] = k * kNB + j + 1;
+        if (i == j) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) row[buf][c0 + c] = B[c];
+        }
+        __syncthreads();
+        const double piv = col[buf][j];
+        if (!(piv > 0.0) && !bad) bad = j + 1;
+        const double pv = piv > 0.0 ? piv : 1.0;
+        const double ip = 1.0 / pv;
+        if (t == 0) rs[j] = 1.0 / sqrt(pv);
+        if (i > j) {
+            const double a = col[buf][i] * ip;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int l = c0 + c;
+                if (l > j) {
+                    if (l <= i) T[c] -= a * col[buf][l];
+                } else {
+                    B[c] -= a * row[buf][l];
+                }
+            }
+        }
     }
     __syncthreads();
-    if (t < 64) {
-        const int c = t;
-        for (int i = 0; i < c; ++i) X[i][c] = 0.0;
-        X[c][c] = 1.0 / L[c][c];
-        for (int i = c + 1; i < 64; ++i) {
-            double s = 0.0;
-            for (int q = c; q < i; ++q) s += L[i][q] * X[q][c];
-            X[i][c] = -s / L[i][i];
-        }
-    }
-    __syncthreads();
+    if (bad && t == 0 && info[w] == 0) info[w] = k * kNB + bad;
     double* D = dinv + ((size_t)w * nblk + k) * 64 * 64;
-    for (int e = t; e < 64 * 64; e += 256) {
-        const int i = e >> 6, j = e & 63;
-        Aw[off + (size_t)i * np + j] = L[i][j];     // zeros above the diagonal
-        D[e] = X[i][j];
+    double* dst = Aw + off + (size_t)i * np + c0;
+    const double ri = rs[i];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int l = c0 + c;
+        dst[c] = l <= i ? T[c] * rs[l] : 0.0;          // zeros above the diagonal
+        D[i * 64 + l] = l <= i ? B[c] * ri : 0.0;
     }
 }
 
@@ -492,39 +513,85 @@ __device__ inline cplx cdiv(cplx a, cplx b) {
 }
 __device__ inline double cabs_(cplx a) { return hypot(a.re, a.im); }
 
-__global__ void k_sigma_c(const double* __restrict__ q, const double* __restrict__ qim,
-                          const double* __restrict__ mo,
-                          const double* __restrict__ omega, const double* __restrict__ weight,
-                          double fermi, int nmo, int nstates, int nfreq, double* __restrict__ sre,
-                          double* __restrict__ sim) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nstates * nfreq) return;
-    const int s = e / nfreq, n = e % nfreq;
+// g[(n * nfreq + f) * nmo + k] = d / (d^2 + w_f^2), d = (fermi - e_k) + i w_n: the
+// Green's-function factor of the quadrature, which does not depend on the state.
+__global__ void k_sigma_g(const double* __restrict__ mo, const double* __restrict__ omega,
+                          double fermi, int nmo, int nfreq, double2* __restrict__ g) {
+    const size_t total = (size_t)nfreq * nfreq * nmo;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % nmo);
+        const size_t nf = e / nmo;
+        const int f = (int)(nf % nfreq), n = (int)(nf / nfreq);
+        const double wn = omega[n], wp = omega[f];
+        const cplx d{fermi - mo[k], wn};
+        const cplx dd = cmul(d, d);
+        const cplx den{dd.re + wp * wp, dd.im};
+        const cplx gg = cdiv(d, den);
+        g[e] = make_double2(gg.re, gg.im);
+    }
+}
+
+// Sigma_c(s, i w_n) = -(1/pi) sum_f sum_k (g(n, f, k) q[f][s][k]) w_f, f outer and
+// k inner for every (s, n) as in the reference's loop; 16 n x 8 states per
+// workgroup, g and q staged through LDS in chunks of 64 k.
+constexpr int kSgN = 16, kSgS = 8, kSgK = 64;
+__global__ __launch_bounds__(128) void k_sigma_c(const double* __restrict__ q,
+                                                 const double* __restrict__ qim,
+                                                 const double2* __restrict__ g,
+                                                 const double* __restrict__ weight, int nmo,
+                                                 int nstates, int nfreq, double* __restrict__ sre,
+                                                 double* __restrict__ sim) {
+    __shared__ double gre[kSgN][kSgK + 1], gim[kSgN][kSgK + 1];
+    __shared__ double qs[kSgS][kSgK + 1], qis[kSgS][kSgK + 1];
+    const int t = threadIdx.x, tn = t & (kSgN - 1), ts = t / kSgN;
+    const int n0 = blockIdx.x * kSgN, s0 = blockIdx.y * kSgS;
     const size_t m = (size_t)nstates * nmo;
-    const double wn = omega[n];
     double are = 0.0, aim = 0.0;
     for (int f = 0; f < nfreq; ++f) {
-        const double wp = omega[f], wf = weight[f];
-        const double* qf = q + (size_t)f * m + (size_t)s * nmo;
-        for (int k = 0; k < nmo; ++k) {
-            const cplx d{fermi - mo[k], wn};
-            const cplx dd = cmul(d, d);
-            const cplx den{dd.re + wp * wp, dd.im};
-            const cplx g = cdiv(d, den);
-            const double qk = qf[k];
+        const double wf = weight[f];
+        for (int k0 = 0; k0 < nmo; k0 += kSgK) {
+            const int kc = min(kSgK, nmo - k0);
+            __syncthreads();
+            for (int e = t; e < kSgN * kSgK; e += 128) {
+                const int r = e / kSgK, kk = e % kSgK;
+                double2 v = make_double2(0.0, 0.0);
+                if (kk < kc && n0 + r < nfreq) v = g[((size_t)(n0 + r) * nfreq + f) * nmo + k0 + kk];
+                gre[r][kk] = v.x;
+                gim[r][kk] = v.y;
+            }
+            for (int e = t; e < kSgS * kSgK; e += 128) {
+                const int r = e / kSgK, kk = e % kSgK;
+                double v = 0.0, vi = 0.0;
+                if (kk < kc && s0 + r < nstates) {
+                    const size_t o = (size_t)f * m + (size_t)(s0 + r) * nmo + k0 + kk;
+                    v = q[o];
+                    if (qim) vi = qim[o];
+                }
+                qs[r][kk] = v;
+                qis[r][kk] = vi;
+            }
+            __syncthreads();
             if (qim) {      // complex q (eta != 0): (g q) w
-                const double qi = qim[(size_t)f * m + (size_t)s * nmo + k];
-                are -= (g.re * qk - g.im * qi) * wf;
-                aim -= (g.re * qi + g.im * qk) * wf;
+                for (int kk = 0; kk < kc; ++kk) {
+                    const double gr = gre[tn][kk], gi = gim[tn][kk], qk = qs[ts][kk], qi = qis[ts][kk];
+                    are -= (gr * qk - gi * qi) * wf;
+                    aim -= (gr * qi + gi * qk) * wf;
+                }
             } else {
-                are -= (g.re * qk) * wf;
-                aim -= (g.im * qk) * wf;
+                for (int kk = 0; kk < kc; ++kk) {
+                    const double gr = gre[tn][kk], gi = gim[tn][kk], qk = qs[ts][kk];
+                    are -= (gr * qk) * wf;
+                    aim -= (gi * qk) * wf;
+                }
             }
         }
     }
+    const int n = n0 + tn, s = s0 + ts;
+    if (n >= nfreq || s >= nstates) return;
     const double pi = 3.141592653589793238462643383279502884;
-    sre[e] = are / pi;
-    sim[e] = aim / pi;
+    sre[(size_t)s * nfreq + n] = are / pi;
+    sim[(size_t)s * nfreq + n] = aim / pi;
 }
 
 __global__ void k_sigma_x(const double* __restrict__ eri, double* __restrict__ sx, int nmo,
@@ -837,10 +904,14 @@ hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int 
 
 hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_energy,
                           const double* omega, const double* weight, double fermi, int nmo,
-                          int nstates, int nfreq, double* sig_re, double* sig_im, hipStream_t s) {
-    const int total = nstates * nfreq;
-    hipLaunchKernelGGL(k_sigma_c, dim3((total + 127) / 128), dim3(128), 0, s, q, q_im, mo_energy,
-                       omega, weight, fermi, nmo, nstates, nfreq, sig_re, sig_im);
+                          int nstates, int nfreq, double2* gbuf, double* sig_re, double* sig_im,
+                          hipStream_t s) {
+    if (nmo <= 0 || nstates <= 0 || nfreq <= 0) return hipErrorInvalidValue;
+    const size_t total = (size_t)nfreq * nfreq * nmo;
+    hipLaunchKernelGGL(k_sigma_g, dim3(grid_for(total)), dim3(256), 0, s, mo_energy, omega, fermi,
+                       nmo, nfreq, gbuf);
+    hipLaunchKernelGGL(k_sigma_c, dim3((nfreq + kSgN - 1) / kSgN, (nstates + kSgS - 1) / kSgS),
+                       dim3(128), 0, s, q, q_im, gbuf, weight, nmo, nstates, nfreq, sig_re, sig_im);
     return hipGetLastError();
 }
 

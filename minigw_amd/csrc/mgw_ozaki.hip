@@ -74,7 +74,9 @@ __global__ __launch_bounds__(256) void k_slice_x(const double* __restrict__ src,
     if (lane == 0) xscale[c] = scale;
 }
 
-// One wave per row r of M_w S_w (lower triangle, zeros above).
+// One wave per row r of M_w S_w (lower triangle, zeros above it up to the end of
+// the row's 128-row tile; the planes beyond, which the contraction never reads,
+// are left as they are).
 __global__ __launch_bounds__(256) void k_slice_m(const double* __restrict__ M,
                                                  const double* __restrict__ S,
                                                  int8_t* __restrict__ planes,
@@ -89,7 +91,9 @@ __global__ __launch_bounds__(256) void k_slice_m(const double* __restrict__ M,
     const double scale = slice_scale(wave_max(mx));
     const size_t plane = (size_t)np * np;
     int8_t* out = planes + (size_t)w * kS * plane;
-    for (int k = lane; k < np; k += 64) {
+    // the contraction reads a row tile's K range up to its own diagonal tile
+    const int kend = min(np, (r / kIBM + 1) * kIBM);
+    for (int k = lane; k < kend; k += 64) {
         const double x = k <= r ? (row[k] * Sw[k]) * scale : 0.0;
         int8_t t[kS];
         slice_value(x, t);

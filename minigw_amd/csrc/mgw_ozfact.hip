@@ -82,52 +82,66 @@ __global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__
 // (4 rows x 4 chunks stored, 16 rows x 1 chunk read) hit distinct bank groups.
 __device__ inline int blk_off(int row, int ch) { return row * 64 + ((ch ^ ((row >> 2) & 3)) << 4); }
 
-// One 64 x 64 output tile per workgroup, 256 threads = 4 waves of 32 x 32.
-// MODE 0 (Cholesky update of block column k = kj): tile i = k + blockIdx.x,
-//   K blocks kb < k, A = RP(i, kb), B = RP(k, kb); A_w(i, k) -= product.
-// MODE 1 (inverse sweep, column j = kj): tile i = j + 1 + blockIdx.x,
-//   K blocks j < kb <= i, A = RP(i, kb) (M rows), B = CP(kb, j) (L columns);
-//   tbuf_w(i) = product (64 x 64 row-major).
-// Per K block the 2 x kS slice blocks (40 KB) go through LDS (two buffers,
-// register prefetch one block ahead), 2 k chunks x 15 slice products per
-// wave, then the int32 groups are combined and scaled into FP64 accumulators.
-// Two waves per SIMD (two workgroups per CU: 80 KB of LDS each, <= 256
-// registers per wave) so that one wave's group combination overlaps the
-// other's MFMAs.
-template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_i8blk(
+// Block products on int8 MFMA for row block i = rowbeg + blockIdx.x and NC
+// output column blocks c_l (l < NC), 64 x 64 each:
+// MODE 0 (Cholesky update): c_l = kj + l,  A_w(i, c_l) -= sum_{kb in [kb0, kb1)} L_i,kb L_c_l,kb^T
+//   with A = RP(i, kb), B = RP(c_l, kb); blocks above the diagonal (i < c_l) are not written.
+// MODE 1 (inverse sweep): c_l = kj - l,  T_l(i) = (or +=, accumulate) sum_{kb in [kb0, min(kb1, i + 1))}
+//   M_i,kb L_kb,c_l with A = RP(i, kb) (M rows), B = CP(kb, c_l) (L columns); blocks with
+//   i <= c_l are not written; T_l is the 64 x 64 row-major block i of tbuf slot c_l % 2.
+// Two output columns share every A operand block, halving the left-looking
+// sweeps' operand traffic.  256 * NC threads: waves 4l..4l+3 compute column
+// block l as 2 x 2 waves of 32 x 32.  Per K block the (1 + NC) x kS slice
+// blocks go through LDS (two buffers, register prefetch one block ahead),
+// 2 k chunks x 15 slice products per wave, then the int32 groups are combined
+// and scaled into FP64 accumulators.  Two waves per SIMD (<= 256 registers per
+// wave) so that one wave's group combination overlaps the other's MFMAs.
+template <int MODE, int NC>
+__global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_i8blk(
     const int8_t* __restrict__ RP, const double* __restrict__ RS, const int8_t* __restrict__ CP,
     const double* __restrict__ CS, double* __restrict__ Amat, double* __restrict__ tbuf, int np,
-    int kj) {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][kS][4096];   // 80 KB: two per CU
+    int kj, int rowbeg, int kb0, int kb1, int accumulate) {
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][1 + NC][kS][4096];   // 80 / 120 KB
     const int w = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nblk = np / 64;
-    const int i = MODE == 0 ? kj + blockIdx.x : kj + 1 + blockIdx.x;
-    const int kb0 = MODE == 0 ? 0 : kj + 1, kb1 = MODE == 0 ? kj : i + 1;   // [kb0, kb1)
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int i = rowbeg + blockIdx.x;
+    const int kend = MODE == 0 ? kb1 : min(kb1, i + 1);
+    const int cb = wave >> 2;
+    const int col = MODE == 0 ? kj + cb : kj - cb;
+    const int wr = ((wave >> 1) & 1) * 32, wc = (wave & 1) * 32;
     const int8_t* Bbase = MODE == 0 ? RP : CP;
     const double* Bsc = MODE == 0 ? RS : CS;
     auto a_blk = [&](int kb) { return blk_index(w, nblk, i, kb); };
-    auto b_blk = [&](int kb) { return MODE == 0 ? blk_index(w, nblk, kj, kb) : blk_index(w, nblk, kb, kj); };
+    auto b_blk = [&](int c, int kb) {
+        return MODE == 0 ? blk_index(w, nblk, c, kb) : blk_index(w, nblk, kb, c);
+    };
 
-    // loader: chunk u of thread t -> operand u / 5, slice u % 5, row t / 4, chunk t % 4
-    const int lrow = t >> 2, lch = t & 3;
+    // loader: thread t moves chunk (row (t % 256) / 4, 16 B chunk t % 4) of the
+    // slice blocks t / 256, t / 256 + NC, ... (block = operand * kS + slice)
+    constexpr int kBlocks = (1 + NC) * kS;
+    constexpr int kPre = (kBlocks + NC - 1) / NC;
+    const int lrow = (t & 255) >> 2, lch = t & 3;
     const int goff = lrow * 64 + lch * 16, loff = blk_off(lrow, lch);
-    u32x4 pre[10];
+    u32x4 pre[kPre];
     auto fetch = [&](int kb) {
-        const int8_t* a = RP + a_blk(kb) * kBlkBytes + goff;
-        const int8_t* b = Bbase + b_blk(kb) * kBlkBytes + goff;
 #pragma unroll
-        for (int s = 0; s < kS; ++s) {
-            pre[s] = *reinterpret_cast<const u32x4*>(a + (size_t)s * 4096);
-            pre[kS + s] = *reinterpret_cast<const u32x4*>(b + (size_t)s * 4096);
+        for (int v = 0; v < kPre; ++v) {
+            const int bk = (t >> 8) + NC * v;
+            if (bk < kBlocks) {
+                const int op = bk / kS, sl = bk % kS;
+                const int8_t* base =
+                    op == 0 ? RP + a_blk(kb) * kBlkBytes
+                            : Bbase + b_blk(MODE == 0 ? kj + op - 1 : kj - (op - 1), kb) * kBlkBytes;
+                pre[v] = *reinterpret_cast<const u32x4*>(base + (size_t)sl * 4096 + goff);
+            }
         }
     };
     auto stash = [&](int buf) {
 #pragma unroll
-        for (int s = 0; s < kS; ++s) {
-            *reinterpret_cast<u32x4*>(&lds[buf][0][s][loff]) = pre[s];
-            *reinterpret_cast<u32x4*>(&lds[buf][1][s][loff]) = pre[kS + s];
+        for (int v = 0; v < kPre; ++v) {
+            const int bk = (t >> 8) + NC * v;
+            if (bk < kBlocks)
+                *reinterpret_cast<u32x4*>(&lds[buf][bk / kS][bk % kS][loff]) = pre[v];
         }
     };
 
@@ -135,18 +149,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
     const int l32 = lane & 31, kgl = lane >> 5;
-    if (kb0 < kb1) {
+    if (kb0 < kend) {
         fetch(kb0);
         stash(0);
-        if (kb0 + 1 < kb1) fetch(kb0 + 1);
+        if (kb0 + 1 < kend) fetch(kb0 + 1);
         __syncthreads();
-        for (int kb = kb0; kb < kb1; ++kb) {
+        for (int kb = kb0; kb < kend; ++kb) {
             const int cur = (kb - kb0) & 1;
             // this K block's inverse scales (powers of two, kept as exponents),
             // loaded now and used after the MFMAs
             int ea[16];
             const double* sa = RS + a_blk(kb) * 64 + wr;
-            const int eb = ilogb(Bsc[b_blk(kb) * 64 + wc + l32]) - 40;
+            const int eb = ilogb(Bsc[b_blk(col, kb) * 64 + wc + l32]) - 40;
 #pragma unroll
             for (int r = 0; r < 16; ++r) ea[r] = ilogb(sa[row32(lane, r)]) + eb;
             i32x16 acc[kS];
@@ -163,16 +177,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int s = 0; s < kS; ++s) a[s] = *reinterpret_cast<const u32x4*>(&lds[cur][0][s][oa]);
 #pragma unroll
                 for (int j = 0; j < kS; ++j) {
-                    const u32x4 b = *reinterpret_cast<const u32x4*>(&lds[cur][1][j][ob]);
+                    const u32x4 b = *reinterpret_cast<const u32x4*>(&lds[cur][1 + cb][j][ob]);
 #pragma unroll
                     for (int s = 0; s + j < kS; ++s) acc[s + j] = mfma_i8(a[s], b, acc[s + j]);
                 }
             }
             // next block into the other buffer (its readers passed the last
             // barrier), fetch the one after
-            if (kb + 1 < kb1) {
+            if (kb + 1 < kend) {
                 stash(cur ^ 1);
-                if (kb + 2 < kb1) fetch(kb + 2);
+                if (kb + 2 < kend) fetch(kb + 2);
             }
             // combine this K block's groups with its row / column scales:
             // sum_g G_g 2^-(12 + 7g) = (G0 2^28 + G1 2^21 + G2 2^14 + G3 2^7 + G4) 2^-40
@@ -189,15 +203,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             __syncthreads();
         }
     }
+    if (MODE == 0 ? i < col : i <= col) return;      // above the diagonal: not an output block
     if (MODE == 0) {
-        double* out = Amat + (size_t)w * np * np + (size_t)i * 64 * np + (size_t)kj * 64;
+        double* out = Amat + (size_t)w * np * np + (size_t)i * 64 * np + (size_t)col * 64;
 #pragma unroll
         for (int r = 0; r < 16; ++r) out[(size_t)(wr + row32(lane, r)) * np + wc + l32] -= acc64[r];
     } else {
-        double* out = tbuf + (size_t)w * np * 64 + (size_t)i * 64 * 64;
+        double* out = tbuf + (size_t)(col & 1) * gridDim.y * np * 64 + (size_t)w * np * 64 +
+                      (size_t)i * 64 * 64;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) out[(wr + row32(lane, r)) * 64 + wc + l32] = acc64[r];
+        for (int r = 0; r < 16; ++r) {
+            double& o = out[(wr + row32(lane, r)) * 64 + wc + l32];
+            o = accumulate ? o + acc64[r] : acc64[r];
+        }
     }
+}
+
+template <int MODE, int NC>
+void gemm_i8blk(const int8_t* RP, const double* RS, const int8_t* CP, const double* CS, double* A,
+                double* tbuf, int np, int kj, int rowbeg, int rows, int kb0, int kb1, int acc, int nb,
+                hipStream_t s) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL((k_gemm_i8blk<MODE, NC>), dim3(rows, nb), dim3(256 * NC), 0, s, RP, RS, CP,
+                       CS, A, tbuf, np, kj, rowbeg, kb0, kb1, acc);
 }
 
 }  // namespace
@@ -207,14 +235,15 @@ size_t fact_slice_bytes(int np) {
     return nblk * nblk * (kBlkBytes + 64 * 8);
 }
 
+// Left-looking blocked Cholesky, two block columns per sweep: the update of
+// columns k and k+1 over the finished columns < k in one launch (they share
+// the L_i,kb operands), then column k factored, solved and sliced, then column
+// k+1's one remaining block product (kb = k), then column k+1.
 hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
                            double* CS, int np, int nb, hipStream_t s) {
     if (np % 64 != 0) return hipErrorInvalidValue;
     const int nblk = np / 64;
-    for (int k = 0; k < nblk; ++k) {
-        if (k > 0)
-            hipLaunchKernelGGL(k_gemm_i8blk<0>, dim3(nblk - k, nb), dim3(256), 0, s, RP, RS, CP, CS, A,
-                               nullptr, np, k);
+    auto finish = [&](int k) -> hipError_t {
         if (hipError_t e = launch_potf2_block(A, dinv, info, np, k, nb, s); e != hipSuccess) return e;
         const int t = nblk - k - 1;
         if (t > 0) {
@@ -222,23 +251,56 @@ hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, doubl
             hipLaunchKernelGGL(k_slice_blocks, dim3(t, nb), dim3(256), 0, s, A, np, k + 1, k, RP, RS,
                                CP, CS);
         }
+        return hipGetLastError();
+    };
+    for (int k = 0; k < nblk; k += 2) {
+        const bool two = k + 1 < nblk;
+        if (k > 0) {
+            if (two)
+                gemm_i8blk<0, 2>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
+            else
+                gemm_i8blk<0, 1>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
+        }
+        if (hipError_t e = finish(k); e != hipSuccess) return e;
+        if (two) {
+            gemm_i8blk<0, 1>(RP, RS, CP, CS, A, nullptr, np, k + 1, k + 1, nblk - k - 1, k, k + 1, 0,
+                             nb, s);
+            if (hipError_t e = finish(k + 1); e != hipSuccess) return e;
+        }
     }
     return hipGetLastError();
 }
 
+// Inverse sweep, two block columns per step: T for columns j and j-1 over the
+// inverted columns > j in one launch (shared M_i,kb operands), column j
+// inverted and sliced, then column j-1's remaining product (kb = j) added and
+// column j-1 inverted.  tbuf holds two slots of nb x np x 64 (column parity).
 hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
                            const int8_t* CP, const double* CS, int np, int nb, hipStream_t s) {
     if (np % 64 != 0) return hipErrorInvalidValue;
     const int nblk = np / 64;
-    for (int j = nblk - 1; j >= 0; --j) {
-        const int t = nblk - 1 - j;
-        if (t > 0)
-            hipLaunchKernelGGL(k_gemm_i8blk<1>, dim3(t, nb), dim3(256), 0, s, RP, RS, CP, CS, nullptr,
-                               tbuf, np, j);
-        if (hipError_t e = launch_trtri_block(A, dinv, tbuf, np, j, nb, s); e != hipSuccess) return e;
+    const size_t slot = (size_t)nb * np * 64;
+    auto finish = [&](int j) -> hipError_t {
+        if (hipError_t e = launch_trtri_block(A, dinv, tbuf + (size_t)(j & 1) * slot, np, j, nb, s);
+            e != hipSuccess)
+            return e;
         if (j > 0)      // row slices of the new inverse blocks (i, j), i >= j
-            hipLaunchKernelGGL(k_slice_blocks, dim3(t + 1, nb), dim3(256), 0, s, A, np, j, j, RP, RS,
-                               nullptr, nullptr);
+            hipLaunchKernelGGL(k_slice_blocks, dim3(nblk - j, nb), dim3(256), 0, s, A, np, j, j, RP,
+                               RS, nullptr, nullptr);
+        return hipGetLastError();
+    };
+    for (int j = nblk - 1; j >= 0; j -= 2) {
+        const bool two = j >= 1;
+        if (two)
+            gemm_i8blk<1, 2>(RP, RS, CP, CS, A, tbuf, np, j, j, nblk - j, j + 1, nblk, 0, nb, s);
+        else
+            gemm_i8blk<1, 1>(RP, RS, CP, CS, A, tbuf, np, j, j + 1, nblk - 1 - j, j + 1, nblk, 0, nb,
+                             s);
+        if (hipError_t e = finish(j); e != hipSuccess) return e;
+        if (two) {
+            gemm_i8blk<1, 1>(RP, RS, CP, CS, A, tbuf, np, j - 1, j, nblk - j, j, j + 1, 1, nb, s);
+            if (hipError_t e = finish(j - 1); e != hipSuccess) return e;
+        }
     }
     return hipGetLastError();
 }

@@ -331,8 +331,11 @@ def _golden_case(name):
 def test_fp64_cd_vs_reference_path(hip, ctx, name):
     """Decomposed-ERI form (device pivoted Cholesky of the pair matrix, pair-space
     screening) against the reference path's all-state outputs: Sigma_c to
-    1e-10 absolute, QP energies to 1e-8 Ha or inside the noise envelope.  The
-    synthetic ERIs have rank <= 3 nmo, which the decomposition finds."""
+    1e-10 absolute; the QP energies are those the oracle's Pade + QP step
+    gives on the same Sigma_c (the QP step of these molecules is too
+    ill-conditioned for a tolerance on the QP energies alone: 1e-12 relative
+    noise on Sigma_c moves some by 0.1 Ha).  The synthetic ERIs have rank
+    <= 3 nmo, which the decomposition finds."""
     r, inp = _golden_case(name)
     say(f"case: FP64 decomposed-ERI form vs reference path, {name}")
     nf, npar = int(r["nfreq"]), int(r["npar"])
@@ -343,11 +346,13 @@ def test_fp64_cd_vs_reference_path(hip, ctx, name):
     npairs = inp.nmo * (inp.nmo + 1) // 2
     assert 0 < rank <= (npairs if name == "h2o_all" else min(npairs, 3 * inp.nmo))
     assert dsig < 1e-10
-    env = qp_envelope(inp, r["omega"], r["sigma_x_diag"], r["sigma_c"], list(range(inp.nmo)),
-                      npar, max(dsig / np.abs(r["sigma_c"]).max(), 1e-15))
+    states = list(range(inp.nmo))
+    qp_orc, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
+                             res.sigma_c, res.omegas, states, npar)
     dq = np.abs(res.qp - r["qp"])
-    say(f"   QP: {np.sum(dq < 1e-8)}/{inp.nmo} states within 1e-8 Ha; max {dq.max():.2e}")
-    assert np.all(dq <= np.maximum(1e-8, 10 * env)), (dq, env)
+    say(f"   QP vs reference: {np.sum(dq < 1e-8)}/{inp.nmo} within 1e-8 Ha (max {dq.max():.2e}); "
+        f"vs the oracle's Pade/QP on this Sigma_c: max {np.nanmax(np.abs(res.qp - qp_orc)):.2e}")
+    assert np.allclose(res.qp, qp_orc, rtol=0, atol=1e-12, equal_nan=True)
 
 
 def test_fp64_cd_h2o_state5_fixture(hip, ctx):
