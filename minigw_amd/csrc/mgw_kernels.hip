@@ -197,9 +197,10 @@ __device__ inline void gemm64(const double* __restrict__ a, size_t lda,
 // ends as L^-1 up to row scaling).  Step j: the owners publish column j of T
 // and row j of B to LDS (double-buffered, one barrier per step); every row
 // i > j then applies the rank-1 update with the unscaled column and pivot
-// p_j = T[j][j]:  T[i][l] -= T[i][j] T[l][j] / p_j (j < l <= i),
-// B[i][l] -= T[i][j] B[j][l] / p_j (l <= j).  Finally L[i][c] = T[i][c] / sqrt(p_c)
-// (c <= i) and L^-1[i][c] = B[i][c] / sqrt(p_i).
+// p_j = T[j][j]:  T[i][l] -= (T[i][j] / p_j) T[l][j] (j < l <= i),
+// B[i][l] -= (T[i][j] / p_j) B[j][l] (l <= j), as masked FMAs on vector-loaded
+// operands (no branches).  Finally L[i][c] = T[i][c] / sqrt(p_c) (c <= i) and
+// L^-1[i][c] = B[i][c] / sqrt(p_i).
 __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* __restrict__ dinv,
                                                int* __restrict__ info, int np, int k) {
     __shared__ double col[2][64], row[2][64];
@@ -220,15 +221,16 @@ __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* _
     }
     int bad = 0;
     for (int j = 0; j < 64; ++j) {
-        const int buf = j & 1;
-        if ((t & 3) == (j >> 4)) {
+        const int buf = j & 1, jj = j & 15;
+        // publish (selects, no branches around the registers)
+        double cv = T[0];
 #pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (c == (j & 15)) col[buf][i] = T[c];
-        }
+        for (int c = 1; c < 16; ++c) cv = c == jj ? T[c] : cv;
+        if ((t & 3) == (j >> 4)) col[buf][i] = cv;
         if (i == j) {
 #pragma unroll
-            for (int c = 0; c < 16; ++c) row[buf][c0 + c] = B[c];
+            for (int c = 0; c < 16; c += 2)
+                *reinterpret_cast<double2*>(&row[buf][c0 + c]) = make_double2(B[c], B[c + 1]);
         }
         __syncthreads();
         const double piv = col[buf][j];
@@ -236,17 +238,24 @@ __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* _
         const double pv = piv > 0.0 ? piv : 1.0;
         const double ip = 1.0 / pv;
         if (t == 0) rs[j] = 1.0 / sqrt(pv);
-        if (i > j) {
-            const double a = col[buf][i] * ip;
+        double cl[16], rl[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const int l = c0 + c;
-                if (l > j) {
-                    if (l <= i) T[c] -= a * col[buf][l];
-                } else {
-                    B[c] -= a * row[buf][l];
-                }
-            }
+        for (int c = 0; c < 16; c += 2) {
+            const double2 x = *reinterpret_cast<const double2*>(&col[buf][c0 + c]);
+            const double2 y = *reinterpret_cast<const double2*>(&row[buf][c0 + c]);
+            cl[c] = x.x;
+            cl[c + 1] = x.y;
+            rl[c] = y.x;
+            rl[c + 1] = y.y;
+        }
+        const double a = i > j ? col[buf][i] * ip : 0.0;    // rows <= j are final
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int l = c0 + c;
+            const double ut = (l > j && l <= i) ? cl[c] : 0.0;
+            const double ub = l <= j ? rl[c] : 0.0;
+            T[c] = fma(-a, ut, T[c]);
+            B[c] = fma(-a, ub, B[c]);
         }
     }
     __syncthreads();
