@@ -45,14 +45,13 @@ HA_MEV = 27.211386245988 * 1e3
 
 # HBM bytes per contraction launch (FETCH_SIZE + WRITE_SIZE, each scaled by
 # what the counter reads for a 2 GiB copy), averaged over the molecule's
-# launches, from two rocprofv3 --pmc runs of this bench; the commands and
-# per-dispatch figures are in profiles/kernels_c3_nmo200_mixed_fp64.md
-# (FP16 split and FP64), profiles/kernels_c3_nmo200_int8_fp64.md (int8) and
-# profiles/kernels_c3_nmo200_fp64_cd.md (decomposed-ERI form).
-TRAFFIC = {("mixed_int8", "c3", 200): (309.0e9 + 0.39e9),          # ..._int8_fp64.md
-           ("mixed", "c3", 200): (130.6e9 + 0.19e9),
-           ("fp64", "c3", 200): (1421.2e9 + 0.39e9),
-           ("fp64_cd", "c3", 200): (66.45e9 + 0.16e9)}       # ..._fp64_cd.md
+# launches, from two rocprofv3 --pmc runs of this bench with that form alone;
+# the commands and per-dispatch figures are in profiles/kernels_c3_nmo200_*.md
+# (int8, fp64, fp16split, fp64_cd).
+TRAFFIC = {("mixed_int8", "c3", 200): (308.94e9 + 0.39e9),
+           ("fp64", "c3", 200): (1421.51e9 + 0.39e9),
+           ("mixed", "c3", 200): (128.95e9 + 0.20e9),
+           ("fp64_cd", "c3", 200): (66.44e9 + 0.16e9)}
 
 
 def accuracy(hip, ctx, inp, mix, f64, npar: int, trials: int = 3) -> dict:
