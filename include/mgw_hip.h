@@ -257,8 +257,9 @@ int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* ru
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 
 /* Autotuner (no counterpart in the reference): times mgw_hip_g0w0 on the
- * resident ERI with 8, 16, 32 and 64 frequencies factored per batch (two
- * batches each) and, for the mixed forms, the contraction's tile unit shape
+ * resident ERI with 8, 16, 32 and 64 frequencies factored per batch (16 to
+ * 256 for MGW_PREC_FP64_CD; two batches each) and, for the mixed forms, the
+ * contraction's tile unit shape
  * (2, 4 or 8 frequencies per 32-tile unit sharing an XCD's L2) at that batch;
  * keeps the fastest per (nmo, nocc, nstates, precision) for every later
  * mgw_hip_g0w0 call of the process and returns the batch with its time per

@@ -298,7 +298,7 @@ size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
                + 4;                          // info
     if (precision == MGW_PREC_MIXED) b += (size_t)np * np * 4 + (size_t)np * 8;  // M planes
     if (precision == MGW_PREC_MIXED_INT8) b += (size_t)np * np * mgw::kSlices + (size_t)np * 8;
-    if (precision != MGW_PREC_FP64)     // factor block slices, second trtri slot
+    if (precision == MGW_PREC_MIXED || precision == MGW_PREC_MIXED_INT8)   // factor slices, 2nd trtri slot
         b += 2 * mgw::fact_slice_bytes(np) + (size_t)np * 64 * 8;
     return b;
 }
@@ -1062,7 +1062,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         double* Y = cplx ? as<double>(ctx->get(ctx->hY, (size_t)nb * np * mpad * 16, "complex panel"))
                          : nullptr;
         char *fr = nullptr, *fc = nullptr;
-        if (job->precision != MGW_PREC_FP64 && ctx->int8_factor) {
+        if ((mixed || i8) && ctx->int8_factor) {       // the mixed forms' int8 block products
             fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np), "factor row slices"));
             fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np), "factor column slices"));
         }
@@ -1427,7 +1427,10 @@ int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_ba
                      double* best_ms_per_freq) {
     return guarded(ctx, [&] {
         need(ctx && job && best_batch, "mgw_hip_autotune: NULL argument");
-        const int cands[4] = {8, 16, 32, 64};
+        // the decomposed form's pair-space batches are small: it takes up to 256
+        const std::vector<int> cands = job->precision == MGW_PREC_FP64_CD
+                                           ? std::vector<int>{16, 32, 64, 128, 256}
+                                           : std::vector<int>{8, 16, 32, 64};
         const int nmo = job->nmo;
         const int ns = job->nstates > 0 && job->states ? job->nstates : nmo;
         std::vector<double> qp(nmo), sre((size_t)ns * 128), sim((size_t)ns * 128);
@@ -1462,7 +1465,7 @@ int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_ba
         // mixed forms: the contraction's unit shape (frequencies per 32-tile
         // unit sharing an XCD's L2) at the chosen batch
         int best_uf = 0;
-        if (job->precision != MGW_PREC_FP64 && best > 0) {
+        if ((job->precision == MGW_PREC_MIXED || job->precision == MGW_PREC_MIXED_INT8) && best > 0) {
             double best_c = 0.0;
             for (int uf : {4, 2, 8}) {
                 mgw_g0w0_job j = *job;

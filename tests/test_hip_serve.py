@@ -41,13 +41,13 @@ def _jobs():
     for i, (nmo, nocc) in enumerate([(12, 3), (20, 4), (28, 6), (16, 4), (36, 7), (24, 5)]):
         mo, fermi, b, vxc = synthetic_factor(nmo, nocc, 1000 + i)
         out.append(dict(nmo=nmo, nocc=nocc, mo=mo, fermi=fermi, b=b, vxc=vxc,
-                        precision=i % 2))
+                        precision=(0, 1, 3, 2, 0, 3)[i]))
     return out
 
 
 def test_serve_alone_equals_mix(hip):
     """Each job's result in a mix of concurrent jobs equals the same job alone."""
-    say("case: 6 jobs (nmo 12..36, FP64 and mixed) on 4 workers vs alone")
+    say("case: 6 jobs (nmo 12..36, FP64, mixed, int8 and decomposed) on 4 workers vs alone")
     jobs = _jobs()
     with hip.HipServer(0, workers=4, hbm_budget=64 << 30, batch_freqs=16) as srv:
         tickets = [srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"],
@@ -99,7 +99,7 @@ def test_graph_replays_match_eager(hip):
     with hip.HipContext(0) as ctx, hip.HipContext(0) as side:
         ctx.upload_eri(inp.eri, inp.nocc)
         side.upload_eri(other.eri, other.nocc)
-        for prec in (hip.PREC_FP64, hip.PREC_MIXED):
+        for prec in (hip.PREC_FP64, hip.PREC_MIXED, hip.PREC_FP64_CD):
             ctx.set_graphs(False)
             eager = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
             side_ref = hip.run_g0w0(other, 40, 8, ctx=side, upload=False, precision=prec)
@@ -132,4 +132,13 @@ def test_autotune_picks_a_batch_and_keeps_results(hip):
         tb, uf = hip.autotune_result(inp, 100, 16, p)
         assert tb == nb and uf in (2, 4, 8)
         after = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=p)
+        assert np.array_equal(after.sigma_c, before.sigma_c)
+        # decomposed form: pair-space batches up to 256 frequencies, no unit shape
+        p = hip.PREC_FP64_CD
+        before = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=p)
+        nb, ms = hip.autotune(inp, 100, 16, p, ctx=ctx)
+        assert nb in (16, 32, 64, 128, 256) and ms > 0
+        assert hip.autotune_result(inp, 100, 16, p) == (nb, 0)
+        after = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=p)
+        assert after.timings["freq_batch"] == min(nb, 100)
         assert np.array_equal(after.sigma_c, before.sigma_c)
