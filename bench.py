@@ -367,14 +367,21 @@ def main() -> int:
                 "phases_ms": {k: round(v, 3) for k, v in cd_phase.items()},
                 "roofline": roofline("fp64_cd", cd_phase, n, m, args.nfreq, args.config, peaks)}
         out.update(report)
+    cd_too = prec != 3 and args.cd_steps > 0          # the same workloads in the decomposed form
     if args.batch_molecules > 0:
         log("bench: uneven-nmo batch ...")
         out["batch"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq, args.npar, prec)
+        if cd_too:
+            out["batch_decomposed"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq,
+                                                args.npar, 3)
     if args.serve_jobs > 0:
         out["serve"] = []
         for rate in args.serve_rates:
             log(f"bench: serving at {rate} jobs/s ...")
             out["serve"].append(run_serve(hip, args.serve_jobs, rate, args.workers, args.npar, prec))
+        if cd_too:
+            out["serve_decomposed"] = [run_serve(hip, args.serve_jobs, rate, args.workers, args.npar, 3)
+                                       for rate in args.serve_rates]
     if not args.no_cpu:
         log("bench: cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(inp, args.nfreq, args.cpu_sample_freqs)
