@@ -389,7 +389,7 @@ CdResult cd_decompose(mgw_hip_ctx* ctx, const PhLayout& L,
              "(largest difference %g, largest element %g); use MGW_PREC_FP64",
              ctx->eri_asym, ctx->eri_absmax);
 
-    const int nblk = mgw::cd_blocks(N), Npad = nblk * 64;
+    const int nblk = mgw::cd_blocks(N), Npad = mgw::cd_npad(N);
     const size_t wbytes = (size_t)Npad * 8 + (size_t)2 * nblk * 16 + (size_t)N * 8 + 16 +
                           (size_t)N * 4 + 16;
     char* wk = as<char>(ctx->get(ctx->cdwork, wbytes, "decomposition workspace"));

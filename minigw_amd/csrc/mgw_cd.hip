@@ -37,7 +37,8 @@ __device__ inline d4 mfma_f64(double a, double b, d4 c) {
 
 constexpr int kBK = 16;          // K slab staged through LDS per step
 constexpr int kLdsPad = 4;       // doubles of padding per LDS row
-constexpr int kCdU = 64;         // pairs per decomposition workgroup
+constexpr int kCdU = 16;         // pairs per decomposition workgroup
+constexpr int kCdP = 256 / kCdU; // partitions of the j sum per workgroup
 
 // (v, i) <- the larger of (v, i) and (v2, i2); ties go to the smaller index,
 // so every lane and every workgroup picks the same pivot.
@@ -57,6 +58,7 @@ __device__ inline void wave_argmax(double& v, int& i) {
 }
 
 // D[u] = M[u][u]; per-block maxima into bm[1] (the "previous step" of step 0).
+// One wave per block of kCdU pairs (lanes >= kCdU idle).
 __global__ __launch_bounds__(64) void k_cd_init(const double* __restrict__ eri,
                                                 const int* __restrict__ pq, int nmo, int N,
                                                 double* __restrict__ D, double2* __restrict__ bm,
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(64) void k_cd_init(const double* __restrict__ eri,
     const int u = blockIdx.x * kCdU + threadIdx.x;
     const size_t nmo2 = (size_t)nmo * nmo;
     double d = -1.0;
-    if (u < N) {
+    if (threadIdx.x < kCdU && u < N) {
         d = eri[(size_t)pq[u] * nmo2 + pq[u]];
         D[u] = d;
     }
@@ -79,14 +81,15 @@ __global__ __launch_bounds__(64) void k_cd_init(const double* __restrict__ eri,
 // and rank = k.  Otherwise column k:
 //   L[k][u] = (M[P][u] - sum_{j<k} L[j][u] L[j][P]) / sqrt(v),
 //   D[u] -= L[k][u]^2  (D[P] = 0),
-// with the j sum split over the 4 waves, and the block's new maximum into bm[k & 1].
+// with the j sum split kCdP ways (kCdU consecutive pairs per partition, one
+// 128-byte segment per column), and the block's new maximum into bm[k & 1].
 __global__ __launch_bounds__(256) void k_cd_step(const double* __restrict__ eri,
                                                  const int* __restrict__ pq, int nmo, int N,
                                                  int Npad, double* __restrict__ Lc,
                                                  double* __restrict__ D, double2* __restrict__ bm,
                                                  int* __restrict__ piv, double* __restrict__ pval,
                                                  int* __restrict__ rank, int k, double tol) {
-    __shared__ double part[4][kCdU];
+    __shared__ double part[kCdP][kCdU];
     __shared__ double rv[4];
     __shared__ int ri[4];
     if (*rank >= 0) return;                       // converged at an earlier pivot
@@ -113,23 +116,29 @@ __global__ __launch_bounds__(256) void k_cd_step(const double* __restrict__ eri,
         return;
     }
 
-    const int u = blockIdx.x * kCdU + lane;       // < Npad by the grid
-    const double* lu = Lc + u;
+    const int ul = t % kCdU, jp = t / kCdU;
+    const double* lu = Lc + blockIdx.x * kCdU + ul;
     const double* lp = Lc + vi;
     double s = 0.0;
-    for (int j = wave; j < k; j += 4) s = fma(lu[(size_t)j * Npad], lp[(size_t)j * Npad], s);
-    part[wave][lane] = s;
+    for (int j = jp; j < k; j += kCdP) s = fma(lu[(size_t)j * Npad], lp[(size_t)j * Npad], s);
+    part[jp][ul] = s;
     __syncthreads();
     if (wave != 0) return;
-    s = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    const int u = blockIdx.x * kCdU + lane;       // lanes >= kCdU carry no pair
+    const bool mine = lane < kCdU;
+    s = 0.0;
+    if (mine) {
+#pragma unroll
+        for (int q = 0; q < kCdP; ++q) s += part[q][lane];
+    }
     double l = 0.0, d = -1.0;
-    if (u < N) {
+    if (mine && u < N) {
         const double x = eri[(size_t)pq[vi] * ((size_t)nmo * nmo) + pq[u]];
         l = (x - s) / sqrt(v);
         d = (u == vi) ? 0.0 : D[u] - l * l;
         D[u] = d;
     }
-    Lc[(size_t)k * Npad + u] = l;
+    if (mine) Lc[(size_t)k * Npad + u] = l;
     int di = u;
     wave_argmax(d, di);
     if (lane == 0) bm[(size_t)(k & 1) * nblk + blockIdx.x] = make_double2(d, (double)di);
@@ -336,11 +345,12 @@ __global__ __launch_bounds__(256) void k_cd_asymmetry(const double* __restrict__
 }  // namespace
 
 int cd_blocks(int N) { return round_up(N, kCdU) / kCdU; }
+int cd_npad(int N) { return cd_blocks(N) * kCdU; }
 
 hipError_t launch_cd_init(const double* eri, const int* pq, int nmo, int N, double* D, double2* bm,
                           int* rank, hipStream_t s) {
     if (N <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cd_init, dim3(cd_blocks(N)), dim3(kCdU), 0, s, eri, pq, nmo, N, D, bm,
+    hipLaunchKernelGGL(k_cd_init, dim3(cd_blocks(N)), dim3(64), 0, s, eri, pq, nmo, N, D, bm,
                        rank);
     return hipGetLastError();
 }

@@ -191,11 +191,12 @@ hipError_t launch_probe_i8(const int8_t* a, const int8_t* b, int* d, int ntests,
 
 // Cholesky-decomposed ERI form (mgw_cd.hip).  Pairs u < N are unique orbital
 // pairs, pq[u] = p * nmo + q with p <= q.  L is column-major: column j of
-// all pairs at Lc + j * Npad, Npad = cd_blocks(N) * 64.  bm: 2 * cd_blocks(N)
+// all pairs at Lc + j * Npad, Npad = cd_npad(N).  bm: 2 * cd_blocks(N)
 // double2 per-block maxima.  launch_cd_steps runs pivots [k0, k1) and stops
 // (rank = the pivot count) once the largest residual diagonal is <= tol times
 // the first pivot; rank stays -1 while it has not converged.
 int cd_blocks(int N);
+int cd_npad(int N);
 hipError_t launch_cd_init(const double* eri, const int* pq, int nmo, int N, double* D, double2* bm,
                           int* rank, hipStream_t s);
 hipError_t launch_cd_steps(const double* eri, const int* pq, int nmo, int N, double* Lc, double* D,
