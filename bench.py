@@ -208,6 +208,10 @@ FORMS = {
 }
 
 
+RANDOM_PEAK = {"mixed_int8": "int8_mfma_32x32_random", "fp64": "fp64_mfma_random",
+               "fp64_cd": "fp64_mfma_random"}
+
+
 def roofline(form: str, ph: dict, n: int, m: int, nfreq: int, config: str, peaks: dict) -> dict:
     """Contraction kernel: algorithmic flops (or int8 ops) per launch over its
     average launch time (HIP events around the kernel on the context's stream)."""
@@ -225,7 +229,11 @@ def roofline(form: str, ph: dict, n: int, m: int, nfreq: int, config: str, peaks
             "frac": round(achieved * 1e12 / peak, 4),
             "traffic": TRAFFIC.get((form, config, nfreq)),
             "launch_ms": round(t_launch * 1e3, 3), "work_per_launch": work,
-            "measured_peak": round(peaks[key] / 1e12, 1)}
+            "measured_peak": round(peaks[key] / 1e12, 1),
+            # the same MFMA on random operands (the rate real data allows: the
+            # chip's MFMA throughput depends on the operand values)
+            "measured_peak_random_operands": (round(peaks[RANDOM_PEAK[form]] / 1e12, 1)
+                                              if form in RANDOM_PEAK else None)}
 
 
 def main() -> int:

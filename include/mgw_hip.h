@@ -303,7 +303,10 @@ int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const voi
  * 16x16 tiles, register-resident operands, 32 waves per CU), out[3] HBM copy
  * bandwidth (bytes read + written per second, 2 GiB buffers), out[4] FP16 and
  * out[5] BF16 MFMA with 32x32x16 tiles, out[6] int8 MFMA 32x32x32 and out[7]
- * int8 16x16x64 (op/s).  out holds 8 entries. */
+ * int8 16x16x64 (op/s), all with constant operands; out[8] int8 32x32x32 and
+ * out[9] FP64 16x16x4 with random operands (bytes in [-64, 64], doubles in
+ * [-1, 1)): MFMA throughput depends on the operand data, and these are the
+ * rates contractions on real data can approach.  out holds 10 entries. */
 int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out);
 
 /* Transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:62-93). */

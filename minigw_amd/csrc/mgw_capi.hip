@@ -1381,10 +1381,11 @@ int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
-        const double flop[7] = {2048.0 * 8,  16384.0 * 8, 16384.0 * 8, 32768.0 * 4,
-                                32768.0 * 4, 65536.0 * 4, 32768.0 * 8};
-        const int iters[7] = {2000, 4000, 4000, 4000, 4000, 4000, 4000};
-        for (int kind = 0; kind < 7; ++kind) {
+        const double flop[9] = {2048.0 * 8,  16384.0 * 8, 16384.0 * 8, 32768.0 * 4, 32768.0 * 4,
+                                65536.0 * 4, 32768.0 * 8, 65536.0 * 8, 2048.0 * 8};
+        const int iters[9] = {2000, 4000, 4000, 4000, 4000, 4000, 4000, 2000, 2000};
+        const int slot[9] = {0, 1, 2, 4, 5, 6, 7, 8, 9};
+        for (int kind = 0; kind < 9; ++kind) {
             const int blocks = cus * 8;
             hip_check(mgw::launch_mfma_peak(kind, blocks, 10, sink, ctx->stream), "mfma peak");
             hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
@@ -1394,7 +1395,7 @@ int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
             const double total = (double)blocks * 4 * iters[kind] * flop[kind];
-            out[kind < 3 ? kind : kind + 1] = total / (ms * 1e-3);
+            out[slot[kind]] = total / (ms * 1e-3);
         }
         double* a = reinterpret_cast<double*>(buf);
         double* b = a + count;

@@ -96,6 +96,71 @@ __global__ __launch_bounds__(256) void k_mfma_peak(int iters, float* sink) {
     }
 }
 
+// Hash of (block, lane, index) to a pseudo-random 32-bit word.
+__device__ inline unsigned mix32(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// The same MFMAs on random operands: eight (a, b) fragment pairs per lane,
+// cycled so that consecutive MFMAs see different data, as a contraction
+// does.  MFMA throughput on gfx950 depends on the operand data (power): the
+// constant-operand rates above are an upper bound that real slices (uniform
+// in [-64, 64]) or FP64 values do not reach.
+// kind 7: i8 32x32x32 with bytes uniform in [-64, 64]; kind 8: f64 16x16x4
+// with values in [-1, 1).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_mfma_peak_random(int iters, float* sink) {
+    const int lane = threadIdx.x & 63;
+    const unsigned seed = (blockIdx.x * 256u + threadIdx.x) * 2654435761u;
+    if constexpr (KIND == 7) {
+        typedef int i16v __attribute__((ext_vector_type(16)));
+        typedef int i4v __attribute__((ext_vector_type(4)));
+        i4v av[8], bv[8];
+        for (int r = 0; r < 8; ++r)
+            for (int q = 0; q < 4; ++q) {
+                unsigned wa = 0, wb = 0;
+                for (int by = 0; by < 4; ++by) {
+                    const unsigned ha = mix32(seed + 97u * (r * 16 + q * 4 + by)) % 129u;
+                    const unsigned hb = mix32(~seed + 89u * (r * 16 + q * 4 + by)) % 129u;
+                    wa |= ((unsigned)(int8_t)((int)ha - 64) & 0xffu) << (8 * by);
+                    wb |= ((unsigned)(int8_t)((int)hb - 64) & 0xffu) << (8 * by);
+                }
+                av[r][q] = (int)wa;
+                bv[r][q] = (int)wb;
+            }
+        i16v acc[4];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 16; ++j) acc[i][j] = 0;
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                acc[r & 3] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[r], bv[r], acc[r & 3], 0, 0, 0);
+        int s = 0;
+        for (int i = 0; i < 4; ++i) s += acc[i][0] + acc[i][15];
+        if (s == 12345 + lane) sink[threadIdx.x] = (float)s;
+    } else {
+        double av[8], bv[8];
+        for (int r = 0; r < 8; ++r) {
+            av[r] = (double)mix32(seed + 131u * r) * 0x1p-31 - 1.0;
+            bv[r] = (double)mix32(~seed + 137u * r) * 0x1p-31 - 1.0;
+        }
+        d4 acc[8];
+        for (int i = 0; i < 8; ++i) acc[i] = d4{0, 0, 0, 0};
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r], bv[r], acc[r], 0, 0, 0);
+        double s = 0;
+        for (int i = 0; i < 8; ++i) s += acc[i][0] + acc[i][3];
+        if (s == 12345.0 + lane) sink[threadIdx.x] = (float)s;
+    }
+}
+
 __global__ void k_stream_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -110,7 +175,10 @@ hipError_t launch_mfma_peak(int kind, int blocks, int iters, float* sink, hipStr
     else if (kind == 3) hipLaunchKernelGGL(k_mfma_peak<3>, dim3(blocks), dim3(256), 0, s, iters, sink);
     else if (kind == 4) hipLaunchKernelGGL(k_mfma_peak<4>, dim3(blocks), dim3(256), 0, s, iters, sink);
     else if (kind == 5) hipLaunchKernelGGL(k_mfma_peak<5>, dim3(blocks), dim3(256), 0, s, iters, sink);
-    else hipLaunchKernelGGL(k_mfma_peak<6>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 6) hipLaunchKernelGGL(k_mfma_peak<6>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else if (kind == 7)
+        hipLaunchKernelGGL(k_mfma_peak_random<7>, dim3(blocks), dim3(256), 0, s, iters, sink);
+    else hipLaunchKernelGGL(k_mfma_peak_random<8>, dim3(blocks), dim3(256), 0, s, iters, sink);
     return hipGetLastError();
 }
 

@@ -242,12 +242,14 @@ class HipContext:
         return d
 
     def measure_peaks(self) -> dict:
-        """Measured FP64 / FP16 / BF16 MFMA flop/s and HBM copy bytes/s of this device."""
-        o = np.zeros(8)
+        """Measured MFMA rates (flop/s or op/s; constant and random operands) and HBM
+        copy bytes/s of this device."""
+        o = np.zeros(10)
         self.call("mgw_hip_measure_peaks", _ptr(o))
         return {"fp64_mfma": o[0], "fp16_mfma": o[1], "bf16_mfma": o[2], "hbm_copy": o[3],
                 "fp16_mfma_32x32": o[4], "bf16_mfma_32x32": o[5], "int8_mfma_32x32": o[6],
-                "int8_mfma_16x16": o[7]}
+                "int8_mfma_16x16": o[7], "int8_mfma_32x32_random": o[8],
+                "fp64_mfma_random": o[9]}
 
     def selftest_tile(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         a, b = _f64(a), _f64(b)
