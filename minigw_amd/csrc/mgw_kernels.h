@@ -71,7 +71,8 @@ hipError_t launch_trtri_block(double* A, const double* dinv, const double* tbuf,
 // Cholesky and inverse sweep with the block products on int8 MFMA.  RP/RS and
 // CP/CS hold the row- and column-scaled slices of every 64 x 64 block
 // (nb x fact_slice_bytes(np) bytes each: planes first, then nb x (np/64)^2 x
-// 64 scales as doubles at the byte offset nb x (np/64)^2 x kSlices x 4096).
+// 64 scale exponents (int32 in 8-byte slots) at the byte offset
+// nb x (np/64)^2 x kSlices x 4096).
 // Both sweeps update two block columns per launch; launch_trtri_i8's tbuf
 // holds two slots of nb x np x 64 doubles.
 size_t fact_slice_bytes(int np);

@@ -29,8 +29,9 @@ __device__ inline size_t blk_index(int w, int nblk, int bi, int bj) {
 
 // One workgroup per 64 x 64 block (bi = b0 + blockIdx.x, bj) of A_w: row-scaled
 // slices into RP/RS (when RP != NULL) and, transposed, column-scaled slices
-// into CP/CS (when CP != NULL), both at block (bi, bj).  RS / CS hold the
-// inverse scales 2^e (the factor that restores a row from its slices).
+// into CP/CS (when CP != NULL), both at block (bi, bj).  RS / CS hold, as
+// int32, the exponents e of the inverse scales 2^e (the factor that restores
+// a row from its slices).
 __global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__ A, int np, int b0,
                                                       int bj, int8_t* __restrict__ RP,
                                                       double* __restrict__ RS,
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__
 #pragma unroll
             for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = s[i];
         }
-        if (t < 64) RS[bix * 64 + t] = 1.0 / rsc[t];     // 2^e, exact
+        if (t < 64) reinterpret_cast<int*>(RS)[bix * 64 + t] = ilogb(1.0 / rsc[t]);   // e of 2^e
     }
     if (CP) {
         int8_t* out = CP + bix * kBlkBytes;
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__
 #pragma unroll
             for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = s[i];
         }
-        if (t < 64) CS[bix * 64 + t] = 1.0 / csc[t];
+        if (t < 64) reinterpret_cast<int*>(CS)[bix * 64 + t] = ilogb(1.0 / csc[t]);
     }
 }
 
@@ -159,10 +160,10 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
             // this K block's inverse scales (powers of two, kept as exponents),
             // loaded now and used after the MFMAs
             int ea[16];
-            const double* sa = RS + a_blk(kb) * 64 + wr;
-            const int eb = ilogb(Bsc[b_blk(col, kb) * 64 + wc + l32]) - 40;
+            const int* sa = reinterpret_cast<const int*>(RS) + a_blk(kb) * 64 + wr;
+            const int eb = reinterpret_cast<const int*>(Bsc)[b_blk(col, kb) * 64 + wc + l32] - 40;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) ea[r] = ilogb(sa[row32(lane, r)]) + eb;
+            for (int r = 0; r < 16; ++r) ea[r] = sa[row32(lane, r)] + eb;
             i32x16 acc[kS];
 #pragma unroll
             for (int g = 0; g < kS; ++g)
