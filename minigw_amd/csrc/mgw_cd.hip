@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void k_cd_gather_x(const double* __restrict__ 
 // blockIdx.y = w).  256 threads = 4 waves of 64 x 64 (4 x 4 MFMA blocks of
 // v_mfma_f64_16x16x4f64); K slabs of 16 staged through LDS, the next slab
 // fetched into registers while the current one is multiplied.
-__global__ __launch_bounds__(256) void k_aux_dielectric(const double* __restrict__ Bt,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_aux_dielectric(const double* __restrict__ Bt,
                                                         const double* __restrict__ S,
                                                         double* __restrict__ C, int rpad, int np) {
     __shared__ double As[kBK][kCT + kLdsPad];

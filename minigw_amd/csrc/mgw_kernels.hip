@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_trtri_m(double* __restrict__ A,
 // K runs over [0, r0+128) (M lower triangular).  Epilogue: column sums of Z^2
 // over the tile's rows -> qpart[w][row tile][column].
 // 4 waves in 2 x 2, each a 64 x 64 sub-tile (4 x 4 MFMA blocks).
-__global__ __launch_bounds__(256) void k_contract_f64(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_contract_f64(
     const double* __restrict__ M, const double* __restrict__ S, const double* __restrict__ xp,
     double* __restrict__ qpart, int np, int mpad) {
     __shared__ double As[kBK][kCT + kLdsPad];
