@@ -1381,9 +1381,9 @@ int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out) {
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
-        const double flop[9] = {2048.0 * 8,  16384.0 * 8, 16384.0 * 8, 32768.0 * 4, 32768.0 * 4,
-                                65536.0 * 4, 32768.0 * 8, 65536.0 * 8, 2048.0 * 8};
-        const int iters[9] = {2000, 4000, 4000, 4000, 4000, 4000, 4000, 2000, 2000};
+        const double flop[9] = {2048.0 * 16, 16384.0 * 8, 16384.0 * 8, 32768.0 * 4, 32768.0 * 4,
+                                65536.0 * 4, 32768.0 * 8, 65536.0 * 8, 2048.0 * 16};
+        const int iters[9] = {1000, 4000, 4000, 4000, 4000, 4000, 4000, 2000, 1000};
         const int slot[9] = {0, 1, 2, 4, 5, 6, 7, 8, 9};
         for (int kind = 0; kind < 9; ++kind) {
             const int blocks = cus * 8;
