@@ -93,16 +93,27 @@ __device__ inline int blk_off(int row, int ch) { return row * 64 + ((ch ^ ((row 
 // Two output columns share every A operand block, halving the left-looking
 // sweeps' operand traffic.  256 * NC threads: waves 4l..4l+3 compute column
 // block l as 2 x 2 waves of 32 x 32.  Per K block the (1 + NC) x kS slice
-// blocks go through LDS (two buffers, register prefetch one block ahead),
-// 2 k chunks x 15 slice products per wave, then the int32 groups are combined
-// and scaled into FP64 accumulators.  Two waves per SIMD (<= 256 registers per
-// wave) so that one wave's group combination overlaps the other's MFMAs.
+// blocks and the operands' scale exponents move from global memory straight
+// into LDS (global_load_lds, no register staging) into one of two stage
+// buffers while the other feeds 2 k chunks x 15 slice products per wave; then
+// the int32 groups are combined and scaled into FP64 accumulators with the
+// exponents read from LDS.  The two stages are separate LDS objects and the K
+// loop is unrolled by two, so every LDS read names a buffer that no load in
+// flight writes and needs no wait on those loads.  Two waves per SIMD
+// (<= 256 registers per wave).
+template <int NC>
+struct I8Stage {
+    int8_t s[1 + NC][kS][4096];   // slice blocks: operand, slice, swizzled 64 x 64 bytes
+    int sc[1 + NC][64];           // scale exponents per row of each operand
+};
+
 template <int MODE, int NC>
 __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_i8blk(
     const int8_t* __restrict__ RP, const double* __restrict__ RS, const int8_t* __restrict__ CP,
     const double* __restrict__ CS, double* __restrict__ Amat, double* __restrict__ tbuf, int np,
     int kj, int rowbeg, int kb0, int kb1, int accumulate) {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][1 + NC][kS][4096];   // 80 / 120 KB
+    __shared__ __attribute__((aligned(16))) I8Stage<NC> st0;   // 2 x 40.5 / 60.75 KB
+    __shared__ __attribute__((aligned(16))) I8Stage<NC> st1;
     const int w = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nblk = np / 64;
     const int i = rowbeg + blockIdx.x;
@@ -111,97 +122,119 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int col = MODE == 0 ? kj + cb : kj - cb;
     const int wr = ((wave >> 1) & 1) * 32, wc = (wave & 1) * 32;
     const int8_t* Bbase = MODE == 0 ? RP : CP;
-    const double* Bsc = MODE == 0 ? RS : CS;
+    const int* Bsc = reinterpret_cast<const int*>(MODE == 0 ? RS : CS);
     auto a_blk = [&](int kb) { return blk_index(w, nblk, i, kb); };
     auto b_blk = [&](int c, int kb) {
         return MODE == 0 ? blk_index(w, nblk, c, kb) : blk_index(w, nblk, kb, c);
     };
+    auto op_col = [&](int op) { return MODE == 0 ? kj + op - 1 : kj - (op - 1); };
 
-    // loader: thread t moves chunk (row (t % 256) / 4, 16 B chunk t % 4) of the
-    // slice blocks t / 256, t / 256 + NC, ... (block = operand * kS + slice)
-    constexpr int kBlocks = (1 + NC) * kS;
-    constexpr int kPre = (kBlocks + NC - 1) / NC;
-    const int lrow = (t & 255) >> 2, lch = t & 3;
-    const int goff = lrow * 64 + lch * 16, loff = blk_off(lrow, lch);
-    u32x4 pre[kPre];
-    auto fetch = [&](int kb) {
+    // loads of K block kb into stage S: the (1 + NC) kS slice blocks in 1 KB
+    // pieces (16 rows), piece p = wave + 4 NC v; lane l of a piece fills LDS
+    // bytes 16 l.. of it, i.e. row l / 4, stored chunk l % 4, so it reads the
+    // logical chunk (l % 4) ^ ((row / 4) % 4) (the swizzle of blk_off).  The
+    // last wave's lanes < 16 (1 + NC) load the scale exponents (16 B each).
+    constexpr int kPieces = (1 + NC) * kS * 4;
+    constexpr int kWaves = 4 * NC;
+    auto issue = [&](int kb, I8Stage<NC>& S) {
 #pragma unroll
-        for (int v = 0; v < kPre; ++v) {
-            const int bk = (t >> 8) + NC * v;
-            if (bk < kBlocks) {
+        for (int v = 0; v < (kPieces + kWaves - 1) / kWaves; ++v) {
+            const int p = wave + kWaves * v;
+            if (p < kPieces) {
+                const int bk = p >> 2, rg = p & 3;
                 const int op = bk / kS, sl = bk % kS;
-                const int8_t* base =
-                    op == 0 ? RP + a_blk(kb) * kBlkBytes
-                            : Bbase + b_blk(MODE == 0 ? kj + op - 1 : kj - (op - 1), kb) * kBlkBytes;
-                pre[v] = *reinterpret_cast<const u32x4*>(base + (size_t)sl * 4096 + goff);
+                const int row = rg * 16 + (lane >> 2);
+                const int ch = (lane & 3) ^ ((row >> 2) & 3);
+                const int8_t* src =
+                    (op == 0 ? RP + a_blk(kb) * kBlkBytes : Bbase + b_blk(op_col(op), kb) * kBlkBytes) +
+                    (size_t)sl * 4096 + row * 64 + ch * 16;
+                __builtin_amdgcn_global_load_lds(
+                    src, (__attribute__((address_space(3))) void*)&S.s[op][sl][rg * 1024], 16, 0, 0);
             }
         }
-    };
-    auto stash = [&](int buf) {
-#pragma unroll
-        for (int v = 0; v < kPre; ++v) {
-            const int bk = (t >> 8) + NC * v;
-            if (bk < kBlocks)
-                *reinterpret_cast<u32x4*>(&lds[buf][bk / kS][bk % kS][loff]) = pre[v];
+        if (wave == kWaves - 1 && lane < 16 * (1 + NC)) {
+            const int op = lane >> 4;
+            const int* src = (op == 0 ? reinterpret_cast<const int*>(RS) + a_blk(kb) * 64
+                                      : Bsc + b_blk(op_col(op), kb) * 64) +
+                             (lane & 15) * 4;
+            __builtin_amdgcn_global_load_lds(
+                src, (__attribute__((address_space(3))) void*)&S.sc[0][0], 16, 0, 0);
         }
     };
+    // this wave's loads have landed in LDS (the barrier after it: everyone's)
+    auto loads_done = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };   // vmcnt(0)
 
     double acc64[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
     const int l32 = lane & 31, kgl = lane >> 5;
+    // the K block's slice products, then its scale exponents (row
+    // row32(lane, r) for r = 4q..4q+3 is 4 consecutive rows from wr + 8q + 4 kgl)
+    i32x16 acc[kS];
+    int ea[16];
+    auto products = [&](const I8Stage<NC>& S) {
+#pragma unroll
+        for (int g = 0; g < kS; ++g)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[g][r] = 0;
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc) {
+            const int ch = 2 * kc + kgl;
+            const int oa = blk_off(wr + l32, ch), ob = blk_off(wc + l32, ch);
+            u32x4 a[kS];
+#pragma unroll
+            for (int s = 0; s < kS; ++s) a[s] = *reinterpret_cast<const u32x4*>(&S.s[0][s][oa]);
+#pragma unroll
+            for (int j = 0; j < kS; ++j) {
+                const u32x4 b = *reinterpret_cast<const u32x4*>(&S.s[1 + cb][j][ob]);
+#pragma unroll
+                for (int s = 0; s + j < kS; ++s) acc[s + j] = mfma_i8(a[s], b, acc[s + j]);
+            }
+        }
+        const int eb = S.sc[1 + cb][wc + l32] - 40;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const u32x4 v = *reinterpret_cast<const u32x4*>(&S.sc[0][wr + 8 * q + 4 * kgl]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ea[4 * q + j] = (int)v[j] + eb;
+        }
+    };
+    // combine the groups with the row / column scales:
+    // sum_g G_g 2^-(12 + 7g) = (G0 2^28 + G1 2^21 + G2 2^14 + G3 2^7 + G4) 2^-40
+    // and for K = 64 (|G_g| <= 5 * 2^18) hi = G0 * 128 + G1 and
+    // lo = G3 * 128 + G4 fit in int32, z = hi 2^21 + G2 2^14 + lo < 2^47 is
+    // exact in FP64
+    auto combine = [&] {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int hi = acc[0][r] * 128 + acc[1][r];
+            const int lo = acc[3][r] * 128 + acc[4][r];
+            const double z = fma((double)hi, 0x1p21, fma((double)acc[2][r], 0x1p14, (double)lo));
+            acc64[r] += ldexp(z, ea[r]);
+        }
+    };
+    // per K block: products from one stage, wait for the other stage's loads,
+    // barrier (the other stage is complete, this one free), start the loads
+    // of the block after next into this stage, then combine while they fly
     if (kb0 < kend) {
-        fetch(kb0);
-        stash(0);
-        if (kb0 + 1 < kend) fetch(kb0 + 1);
+        issue(kb0, st0);
+        loads_done();
         __syncthreads();
-        for (int kb = kb0; kb < kend; ++kb) {
-            const int cur = (kb - kb0) & 1;
-            // this K block's inverse scales (powers of two, kept as exponents),
-            // loaded now and used after the MFMAs
-            int ea[16];
-            const int* sa = reinterpret_cast<const int*>(RS) + a_blk(kb) * 64 + wr;
-            const int eb = reinterpret_cast<const int*>(Bsc)[b_blk(col, kb) * 64 + wc + l32] - 40;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) ea[r] = sa[row32(lane, r)] + eb;
-            i32x16 acc[kS];
-#pragma unroll
-            for (int g = 0; g < kS; ++g)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) acc[g][r] = 0;
-#pragma unroll
-            for (int kc = 0; kc < 2; ++kc) {
-                const int ch = 2 * kc + kgl;
-                const int oa = blk_off(wr + l32, ch), ob = blk_off(wc + l32, ch);
-                u32x4 a[kS];
-#pragma unroll
-                for (int s = 0; s < kS; ++s) a[s] = *reinterpret_cast<const u32x4*>(&lds[cur][0][s][oa]);
-#pragma unroll
-                for (int j = 0; j < kS; ++j) {
-                    const u32x4 b = *reinterpret_cast<const u32x4*>(&lds[cur][1 + cb][j][ob]);
-#pragma unroll
-                    for (int s = 0; s + j < kS; ++s) acc[s + j] = mfma_i8(a[s], b, acc[s + j]);
-                }
-            }
-            // next block into the other buffer (its readers passed the last
-            // barrier), fetch the one after
-            if (kb + 1 < kend) {
-                stash(cur ^ 1);
-                if (kb + 2 < kend) fetch(kb + 2);
-            }
-            // combine this K block's groups with its row / column scales:
-            // sum_g G_g 2^-(12 + 7g) = (G0 2^28 + G1 2^21 + G2 2^14 + G3 2^7 + G4) 2^-40
-            // and for K = 64 (|G_g| <= 5 * 2^18) hi = G0 * 128 + G1 and
-            // lo = G3 * 128 + G4 fit in int32, z = hi 2^21 + G2 2^14 + lo < 2^47
-            // is exact in FP64
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int hi = acc[0][r] * 128 + acc[1][r];
-                const int lo = acc[3][r] * 128 + acc[4][r];
-                const double z = fma((double)hi, 0x1p21, fma((double)acc[2][r], 0x1p14, (double)lo));
-                acc64[r] += ldexp(z, ea[r]);
-            }
+        if (kb0 + 1 < kend) issue(kb0 + 1, st1);
+        for (int kb = kb0; kb < kend; kb += 2) {
+            products(st0);
+            loads_done();
             __syncthreads();
+            if (kb + 2 < kend) issue(kb + 2, st0);
+            __builtin_amdgcn_sched_barrier(0);
+            combine();
+            if (kb + 1 >= kend) break;
+            products(st1);
+            loads_done();
+            __syncthreads();
+            if (kb + 3 < kend) issue(kb + 3, st1);
+            __builtin_amdgcn_sched_barrier(0);
+            combine();
         }
     }
     if (MODE == 0 ? i < col : i <= col) return;      // above the diagonal: not an output block
