@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     };
     auto op_col = [&](int op) { return MODE == 0 ? kj + op - 1 : kj - (op - 1); };
 
-    // loads of K block kb into stage S: the (1 + NC) kS slice blocks in 1 KB
+    // loads of K block kb into stage S (two output columns): the (1 + NC) kS slice blocks in 1 KB
     // pieces (16 rows), piece p = wave + 4 NC v; lane l of a piece fills LDS
     // bytes 16 l.. of it, i.e. row l / 4, stored chunk l % 4, so it reads the
     // logical chunk (l % 4) ^ ((row / 4) % 4) (the swizzle of blk_off).  The
@@ -137,6 +137,33 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     constexpr int kPieces = (1 + NC) * kS * 4;
     constexpr int kWaves = 4 * NC;
     auto issue = [&](int kb, I8Stage<NC>& S) {
+        if constexpr (NC == 1) {
+            // one output column: almost always a single K block (the remainder
+            // of a two-column sweep), where plain loads and LDS stores beat the
+            // direct loads; thread t moves chunk t % 4 of row t / 4 of every
+            // slice block, threads < 32 the scale exponents
+            const int lrow = t >> 2, lch = t & 3;
+            u32x4 v[2 * kS], e;
+#pragma unroll
+            for (int bk = 0; bk < 2 * kS; ++bk) {
+                const int op = bk / kS, sl = bk % kS;
+                const int8_t* base =
+                    op == 0 ? RP + a_blk(kb) * kBlkBytes : Bbase + b_blk(op_col(op), kb) * kBlkBytes;
+                v[bk] = *reinterpret_cast<const u32x4*>(base + (size_t)sl * 4096 + lrow * 64 + lch * 16);
+            }
+            if (t < 32) {
+                const int op = t >> 4;
+                const int* src = (op == 0 ? reinterpret_cast<const int*>(RS) + a_blk(kb) * 64
+                                          : Bsc + b_blk(op_col(op), kb) * 64) +
+                                 (t & 15) * 4;
+                e = *reinterpret_cast<const u32x4*>(src);
+            }
+#pragma unroll
+            for (int bk = 0; bk < 2 * kS; ++bk)
+                *reinterpret_cast<u32x4*>(&S.s[bk / kS][bk % kS][blk_off(lrow, lch)]) = v[bk];
+            if (t < 32) *reinterpret_cast<u32x4*>(&S.sc[t >> 4][(t & 15) * 4]) = e;
+            return;
+        }
 #pragma unroll
         for (int v = 0; v < (kPieces + kWaves - 1) / kWaves; ++v) {
             const int p = wave + kWaves * v;
@@ -216,7 +243,16 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     // per K block: products from one stage, wait for the other stage's loads,
     // barrier (the other stage is complete, this one free), start the loads
     // of the block after next into this stage, then combine while they fly
-    if (kb0 < kend) {
+    if constexpr (NC == 1) {
+        // one stage (41.5 KB, two workgroups per CU): load, barrier, products
+        for (int kb = kb0; kb < kend; ++kb) {
+            if (kb > kb0) __syncthreads();      // the previous block's reads are done
+            issue(kb, st0);
+            __syncthreads();
+            products(st0);
+            combine();
+        }
+    } else if (kb0 < kend) {
         issue(kb0, st0);
         loads_done();
         __syncthreads();
