@@ -99,7 +99,7 @@ def test_graph_replays_match_eager(hip):
     with hip.HipContext(0) as ctx, hip.HipContext(0) as side:
         ctx.upload_eri(inp.eri, inp.nocc)
         side.upload_eri(other.eri, other.nocc)
-        for prec in (hip.PREC_FP64, hip.PREC_MIXED, hip.PREC_FP64_CD):
+        for prec in (hip.PREC_FP64, hip.PREC_MIXED, hip.PREC_MIXED_INT8, hip.PREC_FP64_CD):
             ctx.set_graphs(False)
             eager = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
             side_ref = hip.run_g0w0(other, 40, 8, ctx=side, upload=False, precision=prec)
