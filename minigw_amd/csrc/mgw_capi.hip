@@ -1034,7 +1034,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         double* qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
         int* iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
         double2* gq = as<double2>(
-            ctx->get(ctx->hsg, (size_t)nfreq * nfreq * nmo * sizeof(double2), "quadrature factors"));
+            ctx->get(ctx->hsg, mgw::sigma_workspace_bytes(nmo, ns, nfreq), "quadrature factors"));
         int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
         int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
 
@@ -1525,7 +1525,7 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
         const int N = nmo * (nmo + 1) / 2, Npad = mgw::round_up(N, 64);
         const int cap = std::min(N, mgw::round_up(std::max(256, 4 * nmo), 64));
         const int rpad = mgw::round_up(std::min(N, 4 * nmo), mgw::kCT);
-        const size_t fixed = eri + (size_t)nfreq * nfreq * nmo * 16 + (size_t)nfreq * m * 8 +
+        const size_t fixed = eri + mgw::sigma_workspace_bytes(nmo, ns, nfreq) + (size_t)nfreq * m * 8 +
                              (size_t)cap * Npad * 8 +
                              (size_t)Npad * 24 + (size_t)rpad * L.np * 8 +
                              (size_t)mpad * (rpad + 1) * 8 + ((size_t)N + L.np + mpad) * 4;
@@ -1533,7 +1533,7 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
         return fixed + (size_t)nb * (batch_bytes(rpad, mpad, MGW_PREC_FP64) + (size_t)L.np * 8);
     }
     const size_t v = (size_t)L.np * L.np * 8;
-    const size_t sg = (size_t)nfreq * nfreq * nmo * 16;     // quadrature factors
+    const size_t sg = mgw::sigma_workspace_bytes(nmo, ns, nfreq);     // quadrature factors
     size_t fixed = eri + v + sg + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
     fixed += precision == MGW_PREC_MIXED        ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8
              : precision == MGW_PREC_MIXED_INT8 ? (size_t)mpad * L.np * mgw::kSlices + (size_t)mpad * 8

@@ -150,7 +150,10 @@ hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int 
                               hipStream_t s);
 
 // sigma[s][n] = (1/pi) sum_f sum_k -(g0(k, n, f) q[f][s*nmo + k]) w_f; gbuf:
-// nfreq^2 * nmo complex scratch for the state-independent g0 factors.
+// sigma_workspace_bytes() of scratch: the state-independent g0 factors
+// (nfreq^2 * nmo complex) and, for real q, the partial sums of the product
+// on FP64 MFMA.
+size_t sigma_workspace_bytes(int nmo, int nstates, int nfreq);
 hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_energy,
                           const double* omega, const double* weight, double fermi, int nmo,
                           int nstates, int nfreq, double2* gbuf, double* sig_re, double* sig_im,

@@ -541,9 +541,9 @@ __global__ void k_sigma_g(const double* __restrict__ mo, const double* __restric
     }
 }
 
-// Sigma_c(s, i w_n) = -(1/pi) sum_f sum_k (g(n, f, k) q[f][s][k]) w_f, f outer and
-// k inner for every (s, n) as in the reference's loop; 16 n x 8 states per
-// workgroup, g and q staged through LDS in chunks of 64 k.
+// Sigma_c(s, i w_n) = -(1/pi) sum_f sum_k (g(n, f, k) q[f][s][k]) w_f for complex q
+// (eta != 0), f outer and k inner for every (s, n) as in the reference's loop;
+// 16 n x 8 states per workgroup, g and q staged through LDS in chunks of 64 k.
 constexpr int kSgN = 16, kSgS = 8, kSgK = 64;
 __global__ __launch_bounds__(128) void k_sigma_c(const double* __restrict__ q,
                                                  const double* __restrict__ qim,
@@ -601,6 +601,75 @@ __global__ __launch_bounds__(128) void k_sigma_c(const double* __restrict__ q,
     const double pi = 3.141592653589793238462643383279502884;
     sre[(size_t)s * nfreq + n] = are / pi;
     sim[(size_t)s * nfreq + n] = aim / pi;
+}
+
+// Sigma_c for real q (eta = 0) on FP64 MFMA: the sum over (f, k) of
+// w_f q[f][s][k] g(n, f, k) is a product of the states x (f, k) matrix
+// w_f q with the (f, k) x (Re g, Im g) matrix.  One workgroup: 64 states x
+// 32 frequencies n (columns c < 32: Re g(n0 + c), c >= 32: Im g(n0 + c - 32))
+// over the kSgF frequencies f of split blockIdx.y, K staged through LDS in
+// slabs of kBK values of k (zero beyond nmo); the splits' partial sums go to
+// part[split][s][2 n + {0, 1}] and k_sigma_reduce adds them in split order.
+constexpr int kSgF = 8;
+__global__ __launch_bounds__(256) void k_sigma_mm(const double* __restrict__ q,
+                                                  const double2* __restrict__ g,
+                                                  const double* __restrict__ weight, int nmo,
+                                                  int nstates, int nfreq, double* __restrict__ part) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    const int t = threadIdx.x;
+    const int tiles_s = (nstates + 63) / 64;
+    const int s0 = (blockIdx.x % tiles_s) * 64, n0 = (blockIdx.x / tiles_s) * 32;
+    const int f0 = blockIdx.y * kSgF, f1 = min(nfreq, f0 + kSgF);
+    const size_t m = (size_t)nstates * nmo;
+    const int am = t >> 2, akq = (t & 3) * 4;          // A: state s0 + am, k slab offsets akq..+3
+    const int bj = t >> 3, bkq = (t & 7) * 2;          // B: frequency n0 + bj, k slab offsets bkq..+1
+    Tile64 tile;
+    tile.zero();
+    for (int f = f0; f < f1; ++f) {
+        const double wf = weight[f];
+        for (int k0 = 0; k0 < nmo; k0 += kBK) {
+            __syncthreads();                           // the previous slab's reads are done
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = k0 + akq + u;
+                As[akq + u][am] = s0 + am < nstates && k < nmo
+                                      ? wf * q[(size_t)f * m + (size_t)(s0 + am) * nmo + k]
+                                      : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + bkq + u;
+                double2 v = make_double2(0.0, 0.0);
+                if (n0 + bj < nfreq && k < nmo) v = g[((size_t)(n0 + bj) * nfreq + f) * nmo + k];
+                Bs[bkq + u][bj] = v.x;
+                Bs[bkq + u][32 + bj] = v.y;
+            }
+            __syncthreads();
+            tile.step(As, Bs);
+        }
+    }
+    double* out = part + (size_t)blockIdx.y * nstates * 2 * nfreq;
+    tile.for_each([&](int r, int c, double v) {
+        const int s = s0 + r, n = n0 + (c & 31);
+        if (s < nstates && n < nfreq) out[(size_t)s * 2 * nfreq + 2 * n + (c >> 5)] = v;
+    });
+}
+
+__global__ void k_sigma_reduce(const double* __restrict__ part, int nsplit, int nstates, int nfreq,
+                               double* __restrict__ sre, double* __restrict__ sim) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nstates * nfreq) return;
+    const int s = e / nfreq, n = e % nfreq;
+    double re = 0.0, im = 0.0;
+    for (int y = 0; y < nsplit; ++y) {
+        const double* p = part + ((size_t)y * nstates + s) * 2 * nfreq + 2 * n;
+        re += p[0];
+        im += p[1];
+    }
+    const double pi = 3.141592653589793238462643383279502884;
+    sre[e] = -re / pi;
+    sim[e] = -im / pi;
 }
 
 __global__ void k_sigma_x(const double* __restrict__ eri, double* __restrict__ sx, int nmo,
@@ -911,6 +980,11 @@ hipError_t launch_wc_explicit(const double* M, const double* S, double* wc, int 
     return hipGetLastError();
 }
 
+size_t sigma_workspace_bytes(int nmo, int nstates, int nfreq) {
+    const size_t nsplit = (size_t)(nfreq + kSgF - 1) / kSgF;
+    return (size_t)nfreq * nfreq * nmo * sizeof(double2) + nsplit * nstates * 2 * nfreq * sizeof(double);
+}
+
 hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_energy,
                           const double* omega, const double* weight, double fermi, int nmo,
                           int nstates, int nfreq, double2* gbuf, double* sig_re, double* sig_im,
@@ -919,8 +993,20 @@ hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_
     const size_t total = (size_t)nfreq * nfreq * nmo;
     hipLaunchKernelGGL(k_sigma_g, dim3(grid_for(total)), dim3(256), 0, s, mo_energy, omega, fermi,
                        nmo, nfreq, gbuf);
-    hipLaunchKernelGGL(k_sigma_c, dim3((nfreq + kSgN - 1) / kSgN, (nstates + kSgS - 1) / kSgS),
-                       dim3(128), 0, s, q, q_im, gbuf, weight, nmo, nstates, nfreq, sig_re, sig_im);
+    if (q_im) {      // complex q (eta != 0)
+        hipLaunchKernelGGL(k_sigma_c, dim3((nfreq + kSgN - 1) / kSgN, (nstates + kSgS - 1) / kSgS),
+                           dim3(128), 0, s, q, q_im, gbuf, weight, nmo, nstates, nfreq, sig_re,
+                           sig_im);
+        return hipGetLastError();
+    }
+    // partial sums after the quadrature factors in the same workspace
+    double* part = reinterpret_cast<double*>(gbuf + total);
+    const int nsplit = (nfreq + kSgF - 1) / kSgF;
+    const int tiles = ((nstates + 63) / 64) * ((nfreq + 31) / 32);
+    hipLaunchKernelGGL(k_sigma_mm, dim3(tiles, nsplit), dim3(256), 0, s, q, gbuf, weight, nmo,
+                       nstates, nfreq, part);
+    hipLaunchKernelGGL(k_sigma_reduce, dim3((nstates * nfreq + 255) / 256), dim3(256), 0, s, part,
+                       nsplit, nstates, nfreq, sig_re, sig_im);
     return hipGetLastError();
 }
 
