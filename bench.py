@@ -4,7 +4,7 @@ A step is one molecule's G0W0 on the device: all imaginary frequencies of the
 transformed Gauss-Legendre grid, every state's Sigma_c(i w), Pade fit and QP
 fixed point, on the ERI already resident in HBM.  value = QP solves per second
 (one QP solve = one state's QP energy) of the mixed-precision form; the plain
-FP64 form runs beside it as the error judge and for the mixed/FP64 ratio, and
+FP64 form runs beside it as the error reference and for the mixed/FP64 ratio, and
 the decomposed-ERI FP64 form (pivoted Cholesky of the pair matrix on the
 device, screening in the rank-r pair space) is reported as
 "decomposed_fp64" with its own accuracy against the plain form.
@@ -46,12 +46,12 @@ HA_MEV = 27.211386245988 * 1e3
 # HBM bytes per contraction launch (FETCH_SIZE + WRITE_SIZE, each scaled by
 # what the counter reads for a 2 GiB copy), averaged over the molecule's
 # launches, from two rocprofv3 --pmc runs of this bench with that form alone;
-# the commands and per-dispatch figures are in profiles/kernels_c3_nmo200_*.md
+# the commands and per-dispatch figures are in profiles/kernels_nmo200_*.md
 # (int8, fp64, fp16split, fp64_cd).
-TRAFFIC = {("mixed_int8", "c3", 200): (308.90e9 + 0.39e9),
-           ("fp64", "c3", 200): (1422.44e9 + 0.39e9),
-           ("mixed", "c3", 200): (129.57e9 + 0.20e9),
-           ("fp64_cd", "c3", 200): (64.87e9 + 0.16e9)}
+TRAFFIC = {("mixed_int8", "nmo200", 200): (308.90e9 + 0.39e9),
+           ("fp64", "nmo200", 200): (1422.19e9 + 0.39e9),
+           ("mixed", "nmo200", 200): (130.71e9 + 0.20e9),
+           ("fp64_cd", "nmo200", 200): (64.88e9 + 0.16e9)}
 
 
 def accuracy(hip, ctx, inp, mix, f64, npar: int, trials: int = 3) -> dict:
@@ -241,12 +241,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="nmo200", choices=["nmo24", "nmo120", "nmo200"])
     ap.add_argument("--nfreq", type=int, default=200)
     ap.add_argument("--npar", type=int, default=16)
     ap.add_argument("--precision", default="mixed_int8", choices=list(FORMS))
     ap.add_argument("--alt-precision", default="mixed", choices=list(FORMS) + ["none"],
-                    help="second mixed form, timed and judged beside the headline one")
+                    help="second mixed form, timed and checked against FP64 beside the headline one")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cd-steps", type=int, default=3,
                     help="steps of the decomposed-ERI FP64 form reported beside (0: skip)")
@@ -310,7 +310,7 @@ def main() -> int:
                 log(f"bench: {what} step {i + 1}/{steps} {ts[-1] * 1e3:.1f} ms")
             return r, ts, ctx.last_timings()
 
-        # plain FP64 form: error judge and ratio; the other mixed form beside it
+        # plain FP64 form: error reference and ratio; the other mixed form beside it
         f64 = alt_res = None
         if prec != 0 and args.fp64_steps > 0:
             f64, f64_lat, f64_phase = timed(0, args.fp64_steps, "fp64")

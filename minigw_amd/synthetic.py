@@ -142,9 +142,9 @@ def synthetic_input(nmo: int, nocc: int, seed: int) -> GwInput:
 
 # Named workloads (nmo, nocc, seed).  Seeds are fixed here.
 CONFIGS = {
-    "c1": (24, 5, 20240601),
-    "c2": (120, 20, 20240602),
-    "c3": (200, 40, 20240603),
+    "nmo24": (24, 5, 20240601),
+    "nmo120": (120, 20, 20240602),
+    "nmo200": (200, 40, 20240603),
 }
 BATCH_SEED = 20240604
 SERVE_SEED = 20240605

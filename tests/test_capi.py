@@ -61,9 +61,9 @@ def test_grid_rejects_bad_size():
 
 def test_estimate_bytes_scales_like_eri_plus_batch():
     from minigw_amd import hip
-    c3 = hip.estimate_bytes(200, 40, 200, 0, 16, hip.PREC_FP64)
+    nmo200 = hip.estimate_bytes(200, 40, 200, 0, 16, hip.PREC_FP64)
     eri = 200 ** 4 * 8
-    assert eri < c3 < eri + 16 * 6400 ** 2 * 8 * 1.2 + 2e9
-    assert hip.estimate_bytes(200, 40, 200, 0, 32, hip.PREC_FP64) > c3
-    assert hip.estimate_bytes(200, 40, 200, 0, 16, hip.PREC_MIXED) > c3
+    assert eri < nmo200 < eri + 16 * 6400 ** 2 * 8 * 1.2 + 2e9
+    assert hip.estimate_bytes(200, 40, 200, 0, 32, hip.PREC_FP64) > nmo200
+    assert hip.estimate_bytes(200, 40, 200, 0, 16, hip.PREC_MIXED) > nmo200
     assert hip.estimate_bytes(1, 0, 10) == 0

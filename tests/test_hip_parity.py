@@ -57,8 +57,8 @@ def test_mfma_f64_tile_layout(ctx):
     a = rng.standard_normal((64, 64))
     b = rng.standard_normal((64, 64))
     c = ctx.selftest_tile(a, b)
-    out = os.path.join(os.path.dirname(GOLD), "..", "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("MGW_PROBE_DIR")      # optional dump of the probe for inspection
+    if out and os.path.isdir(out):
         np.savez(os.path.join(out, "tile_selftest.npz"), a=a, b=b, c=c)
     assert np.allclose(c, a @ b.T, rtol=1e-13, atol=1e-12)
 
@@ -181,7 +181,7 @@ def test_all_states_vs_reference_path(hip, ctx, name):
     if name in ("h2o_all", "h2o_eta"):
         inp = h2o_input()
     elif name == "syn_c1":
-        inp = synthetic_input(*CONFIGS["c1"])
+        inp = synthetic_input(*CONFIGS["nmo24"])
     else:
         inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
                       fermi_energy=float(r["fermi_energy"]))
@@ -202,7 +202,7 @@ def test_all_states_vs_reference_path(hip, ctx, name):
 def test_pade_qp_kernel_exact_on_same_sigma(hip, ctx):
     """Pade + QP on the device equals the oracle's bit for bit when Sigma_c agrees."""
     say("case: device Pade/QP vs oracle on the device's own Sigma_c, nmo=24")
-    inp = synthetic_input(*CONFIGS["c1"])
+    inp = synthetic_input(*CONFIGS["nmo24"])
     res = hip.run_g0w0(inp, 64, 16, ctx=ctx)
     qp, it = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
                           res.sigma_c, res.omegas, list(range(inp.nmo)), 16)
@@ -214,7 +214,7 @@ def test_pade_qp_entry_point_matches_oracle(hip, ctx):
     """mgw_hip_pade_qp on a host Sigma_c (oracle's, perturbed, a state
     subset) equals the oracle's solve_qp bit for bit."""
     say("case: mgw_hip_pade_qp vs oracle solve_qp, nmo=24, states 0,4,5,6,23")
-    inp = synthetic_input(*CONFIGS["c1"])
+    inp = synthetic_input(*CONFIGS["nmo24"])
     want = orc.run_g0w0(inp, 48, 12)
     rng = np.random.default_rng(3)
     states = [0, 4, 5, 6, 23]
@@ -320,7 +320,7 @@ def _golden_case(name):
     if name == "h2o_all":
         inp = h2o_input()
     elif name == "syn_c1":
-        inp = synthetic_input(*CONFIGS["c1"])
+        inp = synthetic_input(*CONFIGS["nmo24"])
     else:
         inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
                       fermi_energy=float(r["fermi_energy"]))

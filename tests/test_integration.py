@@ -1,5 +1,6 @@
-"""The reference-side binding shown in INTEGRATION.md compiles against the
-reference's own headers (syntax check; skipped where the tree is absent)."""
+"""The reference-side binding of the integration guide (its ```cpp file=...```
+blocks) compiles against the reference's own headers (syntax check; skipped
+where the tree is absent)."""
 import os
 import re
 import subprocess

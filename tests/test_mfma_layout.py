@@ -25,8 +25,8 @@ def ctx():
 
 
 def _dump(name, **arrs):
-    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("MGW_PROBE_DIR")      # optional dump of the probes for inspection
+    if out and os.path.isdir(out):
         np.savez_compressed(os.path.join(out, name), **arrs)
 
 

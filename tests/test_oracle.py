@@ -297,6 +297,6 @@ def test_oracle_vs_reference_h2o_eta():
 def test_oracle_vs_reference_synthetic_c1():
     r = ref_case("syn_c1")
     from minigw_amd.synthetic import CONFIGS
-    inp = synthetic_input(*CONFIGS["c1"])
+    inp = synthetic_input(*CONFIGS["nmo24"])
     assert int(r["nmo"]) == inp.nmo and float(r["fermi_energy"]) == inp.fermi_energy
     _check_chain(inp, r, 1e-11)
