@@ -60,6 +60,13 @@ struct GraphKey {
 struct mgw_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the factorization runs parts of a frequency batch on `stream` and the
+    // side streams (forked and joined with events): each part's chain of
+    // small, latency-bound launches then runs beside the others' block products
+    static constexpr int kSides = 3;
+    hipStream_t side[kSides] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[kSides] = {nullptr, nullptr, nullptr};
+    int factor_streams = 2;           // MGW_FACTOR_STREAMS: 1, 2 or 4 parts
     std::string error;
     size_t budget = 0;     // 0 = no cap
     size_t held = 0;
@@ -162,6 +169,11 @@ struct mgw_hip_ctx {
         (void)hipSetDevice(device);
         drop_graphs();
         for (Buf* b : all) release(*b);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (int i = 0; i < kSides; ++i) {
+            if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
     void sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
@@ -305,23 +317,52 @@ size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
 
 // Factor + invert nb symmetrised dielectric matrices already in A (lower).
 // With slice buffers (mixed forms) the block products run on int8 MFMA.
+// Blocked Cholesky and inverse of nb matrices (frequencies) of order np; with
+// slice buffers, the block products on int8 MFMA.  Every buffer holds one
+// record per frequency at a fixed stride (the inverse's workspace tbuf one per
+// slot, 2 slots with int8 products), so the batch splits into parts of
+// consecutive frequencies that run on their own streams.
 void factor_batch(mgw_hip_ctx* ctx, double* A, double* dinv, int* info, double* tbuf, int np,
                   int nb, char* rslices = nullptr, char* cslices = nullptr) {
-    if (rslices && cslices) {
-        const size_t nblk = (size_t)(np / mgw::kNB);
-        const size_t planes = (size_t)nb * nblk * nblk * mgw::kSlices * 4096;
-        int8_t* RP = reinterpret_cast<int8_t*>(rslices);
-        int8_t* CP = reinterpret_cast<int8_t*>(cslices);
-        double* RS = reinterpret_cast<double*>(rslices + planes);
-        double* CS = reinterpret_cast<double*>(cslices + planes);
-        hip_check(mgw::launch_potrf_i8(A, dinv, info, RP, RS, CP, CS, np, nb, ctx->stream),
-                  "potrf (int8 products) launch");
-        hip_check(mgw::launch_trtri_i8(A, dinv, tbuf, RP, RS, CP, CS, np, nb, ctx->stream),
-                  "trtri (int8 products) launch");
+    const size_t nblk = (size_t)(np / mgw::kNB);
+    const bool i8 = rslices && cslices;
+    const size_t planes = (size_t)nb * nblk * nblk * mgw::kSlices * 4096;
+    auto run = [&](int f0, int count, hipStream_t s) {
+        double* Af = A + (size_t)f0 * np * np;
+        double* Df = dinv + (size_t)f0 * nblk * 4096;
+        int* If = info + f0;
+        double* Tf = tbuf + (size_t)f0 * (i8 ? 2 : 1) * np * mgw::kNB;
+        if (i8) {
+            int8_t* RP = reinterpret_cast<int8_t*>(rslices) + (size_t)f0 * nblk * nblk * mgw::kSlices * 4096;
+            int8_t* CP = reinterpret_cast<int8_t*>(cslices) + (size_t)f0 * nblk * nblk * mgw::kSlices * 4096;
+            // scale exponents: one int per row of each 64 x 64 block
+            double* RS = reinterpret_cast<double*>(rslices + planes + (size_t)f0 * nblk * nblk * 64 * 4);
+            double* CS = reinterpret_cast<double*>(cslices + planes + (size_t)f0 * nblk * nblk * 64 * 4);
+            hip_check(mgw::launch_potrf_i8(Af, Df, If, RP, RS, CP, CS, np, count, s),
+                      "potrf (int8 products) launch");
+            hip_check(mgw::launch_trtri_i8(Af, Df, Tf, RP, RS, CP, CS, np, count, s),
+                      "trtri (int8 products) launch");
+        } else {
+            hip_check(mgw::launch_potrf(Af, Df, If, np, count, s), "potrf launch");
+            hip_check(mgw::launch_trtri(Af, Df, Tf, np, count, s), "trtri launch");
+        }
+    };
+    const int parts = std::min(nb, ctx->factor_streams);
+    if (parts < 2) {
+        run(0, nb, ctx->stream);
         return;
     }
-    hip_check(mgw::launch_potrf(A, dinv, info, np, nb, ctx->stream), "potrf launch");
-    hip_check(mgw::launch_trtri(A, dinv, tbuf, np, nb, ctx->stream), "trtri launch");
+    hip_check(hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
+    for (int i = 1; i < parts; ++i)
+        hip_check(hipStreamWaitEvent(ctx->side[i - 1], ctx->fork_ev, 0), "hipStreamWaitEvent");
+    for (int i = 0; i < parts; ++i) {
+        const int f0 = (int)((long)nb * i / parts), f1 = (int)((long)nb * (i + 1) / parts);
+        run(f0, f1 - f0, i == 0 ? ctx->stream : ctx->side[i - 1]);
+    }
+    for (int i = 1; i < parts; ++i) {
+        hip_check(hipEventRecord(ctx->join_ev[i - 1], ctx->side[i - 1]), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i - 1], 0), "hipStreamWaitEvent");
+    }
 }
 
 void check_info(mgw_hip_ctx* ctx, const int* dinfo, int count, const char* what) {
@@ -518,10 +559,17 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
             const int uf = std::atoi(env);
             if (uf == 1 || uf == 2 || uf == 4 || uf == 8 || uf == 16 || uf == 32) ctx->split_uf = uf;
         }
-        const hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (const char* env = std::getenv("MGW_FACTOR_STREAMS"))
+            ctx->factor_streams = env[0] == '1' ? 1 : env[0] == '4' ? 4 : 2;
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
+        for (int i = 0; i < mgw_hip_ctx::kSides && e == hipSuccess; ++i) {
+            e = hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming);
+        }
         if (e != hipSuccess) {
             delete ctx;
-            hip_check(e, "hipStreamCreateWithFlags");
+            hip_check(e, "stream / event creation");
         }
         *out = ctx;
     });
