@@ -67,6 +67,7 @@ struct mgw_hip_ctx {
     hipStream_t side[kSides] = {nullptr, nullptr, nullptr};
     hipEvent_t fork_ev = nullptr, join_ev[kSides] = {nullptr, nullptr, nullptr};
     int factor_streams = 2;           // MGW_FACTOR_STREAMS: 1, 2 or 4 parts
+    int split_order = 4096;           // MGW_FACTOR_SPLIT_ORDER: smallest order that splits
     std::string error;
     size_t budget = 0;     // 0 = no cap
     size_t held = 0;
@@ -347,7 +348,9 @@ void factor_batch(mgw_hip_ctx* ctx, double* A, double* dinv, int* info, double* 
             hip_check(mgw::launch_trtri(Af, Df, Tf, np, count, s), "trtri launch");
         }
     };
-    const int parts = std::min(nb, ctx->factor_streams);
+    // small orders gain nothing from the split (and the request loop already
+    // runs several jobs at once): one stream below split_order
+    const int parts = np >= ctx->split_order ? std::min(nb, ctx->factor_streams) : 1;
     if (parts < 2) {
         run(0, nb, ctx->stream);
         return;
@@ -561,9 +564,12 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         }
         if (const char* env = std::getenv("MGW_FACTOR_STREAMS"))
             ctx->factor_streams = env[0] == '1' ? 1 : env[0] == '4' ? 4 : 2;
+        if (const char* env = std::getenv("MGW_FACTOR_SPLIT_ORDER")) ctx->split_order = std::max(0, std::atoi(env));
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
-        for (int i = 0; i < mgw_hip_ctx::kSides && e == hipSuccess; ++i) {
+        // only the side streams the split uses: idle streams still take hardware queues,
+        // which the contexts of the request loop's workers share
+        for (int i = 0; i < ctx->factor_streams - 1 && e == hipSuccess; ++i) {
             e = hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming);
         }

@@ -142,3 +142,32 @@ def test_autotune_picks_a_batch_and_keeps_results(hip):
         after = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=p)
         assert after.timings["freq_batch"] == min(nb, 100)
         assert np.array_equal(after.sigma_c, before.sigma_c)
+
+
+def test_split_factorization_matches_one_stream(hip, monkeypatch):
+    """The factorization split over two (and four) streams, forced on a small
+    molecule, gives bit for bit the one-stream results, eagerly and replayed
+    from captured graphs (the fork and join are captured too)."""
+    say("case: factorization on 1 / 2 / 4 streams, nmo=30, eager and graph replays")
+    inp = synthetic_input(30, 6, 44)
+    base = {}
+    monkeypatch.setenv("MGW_FACTOR_STREAMS", "1")
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri(inp.eri, inp.nocc)
+        for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8):
+            base[prec] = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
+    monkeypatch.setenv("MGW_FACTOR_SPLIT_ORDER", "0")
+    for streams in ("2", "4"):
+        monkeypatch.setenv("MGW_FACTOR_STREAMS", streams)
+        with hip.HipContext(0) as ctx:
+            ctx.upload_eri(inp.eri, inp.nocc)
+            for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8):
+                ctx.set_graphs(False)
+                eager = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
+                assert np.array_equal(eager.sigma_c, base[prec].sigma_c), (streams, prec)
+                ctx.set_graphs(True)
+                for rep in range(3):
+                    res = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
+                    assert res.timings["graph_batches"] > 0
+                    assert np.array_equal(res.sigma_c, base[prec].sigma_c), (streams, prec, rep)
+                    assert np.array_equal(res.qp, base[prec].qp, equal_nan=True)
