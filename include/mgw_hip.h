@@ -80,7 +80,12 @@ int mgw_hip_device_for_local_rank(size_t local_rank, size_t tasks_per_gpu, int* 
 
 /* Create / destroy a context bound to one device.  Replaces the construction
  * of DeviceScreeningWorkspace + DevicePqPhPanelView
- * (src/gw/sigma_c_driver.cpp:489-491). */
+ * (src/gw/sigma_c_driver.cpp:489-491).  Read at creation: MGW_INT8_FACTOR
+ * (0: FP64 block products in the mixed forms' factorization),
+ * MGW_SPLIT_UNIT_FREQS (unit shape of the mixed contractions),
+ * MGW_FACTOR_STREAMS (1, 2 or 4 streams for the parts of a frequency batch
+ * in the factorization; default 2) and MGW_FACTOR_SPLIT_ORDER (smallest
+ * matrix order that is split; default 4096).  None changes a result. */
 int mgw_hip_create(int device, mgw_hip_ctx** out);
 int mgw_hip_destroy(mgw_hip_ctx* ctx);
 
