@@ -85,7 +85,8 @@ int mgw_hip_device_for_local_rank(size_t local_rank, size_t tasks_per_gpu, int* 
  * MGW_SPLIT_UNIT_FREQS (unit shape of the mixed contractions),
  * MGW_FACTOR_STREAMS (1, 2 or 4 streams for the parts of a frequency batch
  * in the factorization; default 2) and MGW_FACTOR_SPLIT_ORDER (smallest
- * matrix order that is split; default 4096).  None changes a result. */
+ * matrix order that is split; default 4096).  The last three change
+ * scheduling only, never a result. */
 int mgw_hip_create(int device, mgw_hip_ctx** out);
 int mgw_hip_destroy(mgw_hip_ctx* ctx);
 
