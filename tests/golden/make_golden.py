@@ -148,7 +148,7 @@ def _save_case(name: str, inp: GwInput, res: dict, with_input: bool) -> None:
 CASES = {
     "h2o_all": (lambda: read_h2o_input(), 200, 16, -1, 0.0, False),
     "h2o_eta": (lambda: read_h2o_input(), 40, 16, -1, 0.01, False),
-    "syn_c1": (lambda: synthetic_input(24, 5, 20240601), 200, 16, -1, 0.0, False),
+    "syn_nmo24": (lambda: synthetic_input(24, 5, 20240601), 200, 16, -1, 0.0, False),
     "syn_small": (lambda: synthetic_input(10, 3, 7), 48, 16, -1, 0.0, True),
     "syn_small_eta": (lambda: synthetic_input(10, 3, 7), 48, 12, -1, 0.05, True),
     "syn_odd": (lambda: synthetic_input(13, 4, 11), 32, 8, -1, 0.0, True),

@@ -173,14 +173,14 @@ def test_h2o_state5_fixture(hip, ctx):
     assert np.isnan(res.qp[0]) and res.qp_iterations[4] > 0
 
 
-@pytest.mark.parametrize("name", ["h2o_all", "syn_c1", "syn_small", "syn_odd", "h2o_eta",
+@pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta",
                                   "syn_small_eta"])
 def test_all_states_vs_reference_path(hip, ctx, name):
     r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
     say(f"case: all states vs reference path, {name}")
     if name in ("h2o_all", "h2o_eta"):
         inp = h2o_input()
-    elif name == "syn_c1":
+    elif name == "syn_nmo24":
         inp = synthetic_input(*CONFIGS["nmo24"])
     else:
         inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
@@ -319,7 +319,7 @@ def _golden_case(name):
     r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
     if name == "h2o_all":
         inp = h2o_input()
-    elif name == "syn_c1":
+    elif name == "syn_nmo24":
         inp = synthetic_input(*CONFIGS["nmo24"])
     else:
         inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
@@ -327,7 +327,7 @@ def _golden_case(name):
     return r, inp
 
 
-@pytest.mark.parametrize("name", ["h2o_all", "syn_c1", "syn_small", "syn_odd"])
+@pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd"])
 def test_fp64_cd_vs_reference_path(hip, ctx, name):
     """Decomposed-ERI form (device pivoted Cholesky of the pair matrix, pair-space
     screening) against the reference path's all-state outputs: Sigma_c to

@@ -294,8 +294,8 @@ def test_oracle_vs_reference_h2o_eta():
     _check_chain(h2o_input(), ref_case("h2o_eta"), 1e-11)
 
 
-def test_oracle_vs_reference_synthetic_c1():
-    r = ref_case("syn_c1")
+def test_oracle_vs_reference_synthetic_nmo24():
+    r = ref_case("syn_nmo24")
     from minigw_amd.synthetic import CONFIGS
     inp = synthetic_input(*CONFIGS["nmo24"])
     assert int(r["nmo"]) == inp.nmo and float(r["fermi_energy"]) == inp.fermi_energy
