@@ -244,6 +244,30 @@ def test_multiblock_vs_oracle(hip, ctx, nmo, nocc, nfreq, eta):
     assert d < 1e-11
 
 
+@pytest.mark.parametrize("nmo,nocc", [(2, 1), (9, 1), (9, 8), (7, 3)])
+def test_extreme_occupations_vs_oracle(hip, ctx, nmo, nocc):
+    """One particle-hole pair (nmo = 2), one occupied or one virtual orbital:
+    nph of 1 to 12, all inside one zero-padded 256-wide tile.  FP64 to the
+    oracle at 1e-11, the decomposed-ERI form at 1e-10, the int8 form within
+    1e-9 of FP64; QP energies of the FP64 form equal the oracle's fixed point
+    on the same Sigma_c."""
+    say(f"case: extreme occupation nmo={nmo} nocc={nocc}")
+    inp = synthetic_input(nmo, nocc, 5)
+    want = orc.run_g0w0(inp, 32, 8)
+    scale = np.abs(want.sigma_c).max()
+    res = hip.run_g0w0(inp, 32, 8, ctx=ctx)
+    d = np.abs(res.sigma_c - want.sigma_c).max() / scale
+    cd = hip.run_g0w0(inp, 32, 8, ctx=ctx, upload=False, precision=hip.PREC_FP64_CD)
+    dcd = np.abs(cd.sigma_c - want.sigma_c).max() / scale
+    mix = hip.run_g0w0(inp, 32, 8, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+    dmix = np.abs(mix.sigma_c - res.sigma_c).max() / scale
+    say(f"   rel |dSigma_c|: FP64 {d:.2e}, decomposed {dcd:.2e}, int8 vs FP64 {dmix:.2e}")
+    assert d < 1e-11 and dcd < 1e-10 and dmix < 1e-9
+    qp, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
+                         res.sigma_c, res.omegas, list(range(nmo)), 8)
+    assert np.allclose(res.qp, qp, rtol=0, atol=1e-12, equal_nan=True)
+
+
 def test_subset_states_and_errors(hip, ctx):
     say("case: state subset + job errors")
     inp = synthetic_input(16, 4, 5)
