@@ -11,8 +11,10 @@
  * Contexts are not thread-safe: one context per host thread, as the
  * reference's device workspaces are one per MPI rank
  * (BackendCapabilities.thread_safe = false, src/linalg/linalg_cublas.cu:208-214).
- * All device work of a context runs on one HIP stream of default priority;
- * every call returns after its results are on the host.
+ * All device work of a context runs on its own HIP streams of default
+ * priority: one main stream, plus side streams that the factorization forks
+ * from it and joins back within a call; every call returns after its results
+ * are on the host.
  *
  * Arrays are row-major (C order) like the reference's Matrix / Tensor4
  * (include/types.hpp:56-59,182-185).  Complex values are passed as separate
@@ -266,7 +268,7 @@ int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
  * resident ERI with 8, 16, 32 and 64 frequencies factored per batch (16 to
  * 256 for MGW_PREC_FP64_CD; two batches each) and, for the mixed forms, the
  * contraction's tile unit shape
- * (2, 4 or 8 frequencies per 32-tile unit sharing an XCD's L2) at that batch;
+ * (2, 4, 8 or 16 frequencies per 32-tile unit sharing an XCD's L2) at that batch;
  * keeps the fastest per (nmo, nocc, nstates, precision) for every later
  * mgw_hip_g0w0 call of the process and returns the batch with its time per
  * frequency.  MGW_SPLIT_UNIT_FREQS, when set, overrides the unit shape. */
