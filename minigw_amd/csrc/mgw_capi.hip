@@ -48,12 +48,17 @@ void hip_check(hipError_t e, const char* what) {
 }  // namespace
 
 // ------------------------------------------------------------------ context
+// Everything a captured frequency batch bakes into its kernel arguments: the
+// true particle-hole count n (rows past it get S = 0), the ph padded order
+// nph_pad (a stride of the pair-space dielectric), the factored order fp (=
+// nph_pad, or the padded rank in the decomposed form), the column counts, the
+// batch, the form and the buffers.
 struct GraphKey {
-    int np, mpad, m, b, precision, uf;
+    int n, nph_pad, fp, mpad, m, b, precision, uf;
     std::array<const void*, 21> ptr;
     bool operator==(const GraphKey& o) const {
-        return np == o.np && mpad == o.mpad && m == o.m && b == o.b && precision == o.precision &&
-               uf == o.uf && ptr == o.ptr;
+        return n == o.n && nph_pad == o.nph_pad && fp == o.fp && mpad == o.mpad && m == o.m &&
+               b == o.b && precision == o.precision && uf == o.uf && ptr == o.ptr;
     }
 };
 
@@ -1220,7 +1225,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             if (ctx->use_graphs && !cplx) {
                 // Replay a graph of this batch when one with the same shape and
                 // buffers was captured before (the offset is read on the device).
-                const GraphKey key{fp, mpad, m, b, job->precision, uf,
+                const GraphKey key{n, np, fp, mpad, m, b, job->precision, uf,
                                    {A, S, dinv, tbuf, info, qpart, dq, xp, xh, xl, xs, mh, ml, ms, dV,
                                     ddelta, dom, dcolmap, f0p, fr, fc}};
                 hipGraphExec_t exec = ctx->find_graph(key);

@@ -2,11 +2,22 @@
 //
 // Compiled only by oracle/Makefile against the reference sources where they
 // lie; it is never part of the product library.  It feeds a raw binary input
-// to gw::run_g0w0 (src/gw/gw.cpp:125-191) with the self-contained
-// ReferenceBackend (src/linalg/linalg.cpp:44-91, Gauss-Jordan inverse) -- the
-// backend that produced regression_tests/h2o_serial -- and writes everything
-// the parity tests compare: the frequency grid, Sigma_x, Sigma_c(i w) for the
-// selected states and the QP energies.
+// to gw::run_g0w0 (src/gw/gw.cpp:125-191) and writes everything the parity
+// tests compare: the frequency grid, Sigma_x, Sigma_c(i w) for the selected
+// states and the QP energies.
+//
+// Backends: gw_ref uses the self-contained ReferenceBackend
+// (src/linalg/linalg.cpp:44-91, Gauss-Jordan inverse) -- the backend that
+// produced regression_tests/h2o_serial.  gw_ref_blas (built with
+// GW_HAS_BLAS_LAPACK_BACKEND and linked to a LAPACK provider) takes the
+// backend name as its third argument and resolves it through the reference's
+// own factory make_local_linalg_backend (src/linalg/backend_factory.cpp:22-66),
+// so "blas-lapack" runs BlasLapackBackend (zgetrf/zgetrs + zgemm,
+// src/linalg/linalg_blas_lapack.cpp:106-183).  A fourth argument "kernel-openmp"
+// runs the frequencies serially with OpenMP kernel loops (the reference CLI's
+// --frequency-parallel serial --kernel-parallel openmp), the schedule the CPU
+// baseline is timed with.  The run's GwTimings are printed on stdout as one
+// line starting with "timings".
 //
 // Input file (little endian):
 //   int64 nmo, nocc, nfreq, npar, state (-1 = all states, else 0-based),
@@ -18,7 +29,9 @@
 //   double omega[nfreq], weight[nfreq], sigma_x_diag[nmo],
 //          sigma_c[nmo*nfreq*2] (row-major state x freq, re/im interleaved),
 //          qp_energy[nmo] (NaN for unselected states)
+#include "cli.hpp"
 #include "gw/gw.hpp"
+#include "linalg/backend_factory.hpp"
 
 #include <cstdint>
 #include <cstdio>
@@ -26,6 +39,7 @@
 #include <fstream>
 #include <iostream>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace {
@@ -44,8 +58,8 @@ void write_from(std::ofstream& out, const T* src, std::size_t count) {
 } // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::cerr << "usage: gw_ref <input.bin> <output.bin>\n";
+    if (argc < 3 || argc > 5) {
+        std::cerr << "usage: gw_ref <input.bin> <output.bin> [backend] [kernel-openmp]\n";
         return 2;
     }
     try {
@@ -77,12 +91,27 @@ int main(int argc, char** argv) {
             settings.selected_state_0based = static_cast<std::size_t>(hdr[4]);
         }
         settings.eta = fe[1];
-        if (hdr[5] > 1) {
+        if (argc >= 4) {
+            gw::Cli cli;
+            cli.linalg_backend = argv[3];
+            settings.linalg_backend = gw::make_local_linalg_backend(cli);
+        }
+        const bool kernel_openmp = argc >= 5 && std::string(argv[4]) == "kernel-openmp";
+        if (kernel_openmp) {
+            settings.execution.frequency_parallel_mode = gw::FrequencyParallelMode::Serial;
+            settings.execution.openmp_kernel_loops = true;
+        } else if (hdr[5] > 1) {
             settings.execution.frequency_parallel_mode = gw::FrequencyParallelMode::OpenMP;
             settings.execution.openmp_frequency_parallel = true;
         }
 
         const gw::GwResult result = gw::run_g0w0(input, settings);
+        const gw::GwTimings& t = result.timings;
+        std::cout << "timings total_wall " << t.total_wall_seconds << " sigma_c_wall "
+                  << t.sigma_c_wall_seconds << " invert_epsilon " << t.invert_epsilon_seconds
+                  << " sigma_c " << t.sigma_c_seconds << " build_pi0 " << t.build_pi0_seconds
+                  << " screening_setup " << t.screening_setup_seconds << " pade " << t.pade_seconds
+                  << " backend " << settings.linalg_backend->name() << std::endl;
 
         std::ofstream out(argv[2], std::ios::binary);
         const std::int64_t ohdr[2] = {static_cast<std::int64_t>(nmo), hdr[2]};

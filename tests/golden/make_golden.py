@@ -15,6 +15,11 @@ its HDF5 tools; the GPU runs and the test-suite read the .npz files it writes.
    built by oracle/Makefile from the reference sources, ReferenceBackend) for
    all states of H2O, of the seeded synthetic molecule nmo=24/nocc=5 and of
    small synthetic cases at eta = 0 and eta != 0.
+4. ``ref_*_blas.npz``: the same cases through the reference's BLAS/LAPACK
+   backend (oracle/_ref/gw_ref_blas: BlasLapackBackend, zgetrf/zgetrs/zgemm
+   from MKL), selected by the reference's own backend factory.  The spread
+   between the two reference backends is what a correct FP64 implementation
+   may differ by.
 
 Usage: python tests/golden/make_golden.py [--only NAME ...]
 """
@@ -39,6 +44,7 @@ from minigw_amd.synthetic import GwInput, synthetic_input  # noqa: E402
 REF = "/root/reference"
 H5DUMP = "/opt/conda/bin/h5dump"
 GW_REF = os.path.join(REPO, "oracle", "_ref", "gw_ref")
+GW_REF_BLAS = os.path.join(REPO, "oracle", "_ref", "gw_ref_blas")
 
 
 def _h5_attr(path: str, name: str) -> str:
@@ -108,21 +114,32 @@ def write_h2o_fixtures() -> None:
     np.savez_compressed(os.path.join(HERE, "h2o_serial_fixtures.npz"), **fx)
 
 
+def write_reference_input(path: str, inp: GwInput, nfreq: int, npar: int, state: int,
+                          eta: float, threads: int) -> None:
+    """The raw input file oracle/ref_driver.cpp reads."""
+    with open(path, "wb") as fh:
+        np.array([inp.nmo, inp.nocc, nfreq, npar, state, threads], dtype="<i8").tofile(fh)
+        np.array([inp.fermi_energy, eta], dtype="<f8").tofile(fh)
+        np.ascontiguousarray(inp.mo_energy, dtype="<f8").tofile(fh)
+        np.ascontiguousarray(inp.vxc, dtype="<f8").tofile(fh)
+        np.ascontiguousarray(inp.eri, dtype="<f8").tofile(fh)
+
+
 def run_reference(inp: GwInput, nfreq: int, npar: int, state: int, eta: float,
-                  threads: int = 8) -> dict:
-    """Run oracle/_ref/gw_ref on one input; returns its outputs."""
-    if not os.path.exists(GW_REF):
+                  threads: int = 8, backend: str = "reference") -> dict:
+    """Run oracle/_ref/gw_ref (backend "reference") or oracle/_ref/gw_ref_blas
+    (backend "blas-lapack", frequencies OpenMP-parallel, one LAPACK thread
+    each) on one input; returns its outputs."""
+    exe = GW_REF if backend == "reference" else GW_REF_BLAS
+    if not os.path.exists(exe):
         raise RuntimeError("build oracle/_ref first: make -C oracle")
     with tempfile.TemporaryDirectory() as td:
         ip, op = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        with open(ip, "wb") as fh:
-            np.array([inp.nmo, inp.nocc, nfreq, npar, state, threads], dtype="<i8").tofile(fh)
-            np.array([inp.fermi_energy, eta], dtype="<f8").tofile(fh)
-            np.ascontiguousarray(inp.mo_energy, dtype="<f8").tofile(fh)
-            np.ascontiguousarray(inp.vxc, dtype="<f8").tofile(fh)
-            np.ascontiguousarray(inp.eri, dtype="<f8").tofile(fh)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        subprocess.run([GW_REF, ip, op], check=True, env=env, stdout=subprocess.DEVNULL)
+        write_reference_input(ip, inp, nfreq, npar, state, eta, threads)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), MKL_NUM_THREADS="1",
+                   MKL_THREADING_LAYER="GNU")
+        cmd = [exe, ip, op] if backend == "reference" else [exe, ip, op, backend]
+        subprocess.run(cmd, check=True, env=env, stdout=subprocess.DEVNULL)
         raw = np.fromfile(op, dtype="<f8")
     nmo = int(raw[:2].view("<i8")[0])
     f = int(raw[:2].view("<i8")[1])
@@ -133,7 +150,7 @@ def run_reference(inp: GwInput, nfreq: int, npar: int, state: int, eta: float,
     sc = raw[o:o + 2 * nmo * f].view(np.complex128).reshape(nmo, f); o += 2 * nmo * f
     qp = raw[o:o + nmo]
     return dict(omega=omega, weight=weight, sigma_x_diag=sx, sigma_c=sc, qp=qp,
-                nfreq=f, npar=npar, eta=eta, state=state)
+                nfreq=f, npar=npar, eta=eta, state=state, backend=backend)
 
 
 def _save_case(name: str, inp: GwInput, res: dict, with_input: bool) -> None:
@@ -155,22 +172,28 @@ CASES = {
 }
 
 
+# cases also run through the BLAS/LAPACK backend (ref_<name>_blas.npz)
+BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd")
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*")
     args = ap.parse_args()
-    names = args.only or ["h2o"] + list(CASES)
+    names = args.only or ["h2o"] + list(CASES) + [c + "_blas" for c in BLAS_CASES]
     if "h2o" in names:
         write_h2o_fixtures()
         print("wrote h2o fixtures", flush=True)
     for name in names:
-        if name not in CASES:
+        base = name[:-5] if name.endswith("_blas") else name
+        if base not in CASES:
             continue
-        fac, nfreq, npar, state, eta, with_input = CASES[name]
+        fac, nfreq, npar, state, eta, with_input = CASES[base]
         inp = fac()
         t = time.time()
-        res = run_reference(inp, nfreq, npar, state, eta)
-        _save_case(name, inp, res, with_input)
+        backend = "blas-lapack" if name.endswith("_blas") else "reference"
+        res = run_reference(inp, nfreq, npar, state, eta, backend=backend)
+        _save_case(name, inp, res, with_input and backend == "reference")
         print(f"wrote ref_{name}.npz in {time.time() - t:.1f} s", flush=True)
 
 

@@ -113,6 +113,27 @@ def test_graph_replays_match_eager(hip):
                 assert np.array_equal(s2.sigma_c, side_ref.sigma_c)
 
 
+def test_graph_cache_tells_molecules_of_one_shape_apart(hip):
+    """Two molecules with the same nmo and a different nocc pad to the same
+    orders (n = 144 and 161, both padded to 256) and reuse the same buffers on
+    one context: the second must not replay the first's captured batches,
+    which carry n as a kernel argument."""
+    say("case: graphs on one context, nmo=30 nocc=6 then nocc=7 (same padded shapes)")
+    first, second = synthetic_input(30, 6, 45), synthetic_input(30, 7, 46)
+    for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8, hip.PREC_FP64_CD):
+        with hip.HipContext(0) as ctx:
+            ctx.upload_eri(second.eri, second.nocc)
+            eager = hip.run_g0w0(second, 100, 16, ctx=ctx, upload=False, precision=prec)
+            ctx.set_graphs(True)
+            ctx.upload_eri(first.eri, first.nocc)
+            hip.run_g0w0(first, 100, 16, ctx=ctx, upload=False, precision=prec)
+            ctx.upload_eri(second.eri, second.nocc)
+            res = hip.run_g0w0(second, 100, 16, ctx=ctx, upload=False, precision=prec)
+            assert res.timings["graph_batches"] > 0
+            assert np.array_equal(res.sigma_c, eager.sigma_c), prec
+            assert np.array_equal(res.qp, eager.qp, equal_nan=True), prec
+
+
 def test_autotune_picks_a_batch_and_keeps_results(hip):
     say("case: frequency-batch and unit-shape autotuner, nmo=30")
     inp = synthetic_input(30, 6, 43)
