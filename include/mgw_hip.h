@@ -182,10 +182,33 @@ typedef struct mgw_g0w0_out {
     double* sigma_c_im;         /* nstates x nfreq, may be NULL               */
     double* qp;                 /* nmo, NaN for states not computed           */
     int32_t* qp_iterations;     /* nmo, may be NULL                           */
+    double* q;                  /* nfreq x (nstates * nmo), may be NULL: the  */
+                                /* quadratic forms q[f][s * nmo + k] =        */
+                                /* x_{p_s k}^T W_c(i w_f) x_{p_s k} (real     */
+                                /* part); Sigma_c is their quadrature         */
 } mgw_g0w0_out;
 
 /* One molecule, all frequencies, on the ERI made resident by mgw_hip_upload_eri. */
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
+
+/* Error judge and FP64 fallback of the mixed forms (no counterpart in the
+ * reference, whose only failure check is the factorization's devInfo,
+ * src/workspace/device_screening_workspace.cu:324-334).  With nsample > 0,
+ * every MGW_PREC_MIXED / MGW_PREC_MIXED_INT8 run of mgw_hip_g0w0 on ctx
+ * recomputes nsample of its frequencies (evenly spaced) with every step in
+ * FP64 and compares their quadratic forms with the form's: the error is the
+ * largest |q_form - q_fp64| over the largest |q_fp64| of a frequency.  When
+ * it exceeds tol (tol = 0: 1e-8 for MIXED_INT8, 1e-6 for MIXED) or the
+ * form's factorization breaks down, and fallback != 0, the call reruns the
+ * job in MGW_PREC_FP64 and returns that result.  nsample = 0 turns it off. */
+int mgw_hip_set_judge(mgw_hip_ctx* ctx, int32_t nsample, double tol, int32_t fallback);
+
+/* Outcome of the last mgw_hip_g0w0 on ctx: the judge's error (-1 if not
+ * judged), the frequencies it sampled, and fell_back = 0 (the form's own
+ * result), 1 (rerun in FP64 after a failed judge) or 2 (rerun in FP64 after
+ * a factorization breakdown). */
+int mgw_hip_last_judge(const mgw_hip_ctx* ctx, double* max_rel_err, int32_t* samples,
+                       int32_t* fell_back);
 
 /* Pade fit and QP fixed point on the device for a given Sigma_c(i w): the
  * reference's solve_qp_energies (src/gw/qp_solver.cpp:10-47) with
@@ -222,21 +245,39 @@ typedef struct mgw_serve_job {
     int32_t kdim;               /* exactly one of eri / eri_factor is given   */
 } mgw_serve_job;
 
-/* Start `workers` threads on `device`, each with its own context and stream.
- * Jobs are admitted in arrival order while the bytes reserved by running jobs
- * plus the next job's estimate fit hbm_budget; batch_freqs caps the
- * frequencies factored at once per job. */
+/* Start `workers` threads on `device`, each with its own context and stream
+ * (graphs on).  Jobs are admitted in arrival order while the bytes reserved
+ * by running jobs plus the next job's estimate fit hbm_budget; batch_freqs
+ * caps the frequencies factored at once per job (mgw_hip_set_batch_limit).
+ * Mixed-precision jobs run under the error judge with FP64 fallback
+ * (mgw_hip_set_judge: 2 sampled frequencies, the form's tolerance) unless
+ * mgw_hip_server_set_judge changes it. */
 int mgw_hip_server_create(int device, int32_t workers, size_t hbm_budget, int32_t batch_freqs,
                           mgw_hip_server** out);
 
 /* Cancel jobs not yet started, wait for running ones, free everything. */
 int mgw_hip_server_destroy(mgw_hip_server* server);
 
+/* Error judge of the jobs submitted from now on (mgw_hip_set_judge's
+ * arguments; nsample = 0 turns it off). */
+int mgw_hip_server_set_judge(mgw_hip_server* server, int32_t nsample, double tol,
+                             int32_t fallback);
+
+/* Fault injection for tests of failure handling: each job started from now
+ * on gets scale (0 < scale <= 1) times its estimate as its context's device
+ * budget, so that a scale below 1 makes the job's allocations fail with
+ * MGW_ERR_MEMORY as an HBM shortfall would.  Default 1. */
+int mgw_hip_server_set_budget_scale(mgw_hip_server* server, double scale);
+
 /* Queue a job.  mo_energy / vxc_diag / states are copied; the ERI or factor
  * and the output buffers must stay valid until mgw_hip_server_wait reports the
  * job finished. */
 int mgw_hip_server_submit(mgw_hip_server* server, const mgw_serve_job* job,
                           const mgw_g0w0_out* out, int64_t* ticket);
+
+/* Cancel a job: a pending job finishes at once, a running one at its next
+ * frequency batch, both with MGW_ERR_STATE; a finished job is not changed. */
+int mgw_hip_server_cancel(mgw_hip_server* server, int64_t ticket);
 
 /* Wait for a job (timeout_s < 0: no limit).  *status = -1 if still pending or
  * running at the timeout, else the job's mgw_status; latency_ms = submit to
@@ -246,6 +287,15 @@ int mgw_hip_server_wait(mgw_hip_server* server, int64_t ticket, double timeout_s
 
 /* Error message of a failed job ("" if none). */
 const char* mgw_hip_server_job_error(mgw_hip_server* server, int64_t ticket);
+
+/* Judge outcome of a finished job: its error (-1 if not judged) and whether
+ * it was rerun in FP64 (mgw_hip_last_judge's fell_back). */
+int mgw_hip_server_job_judge(mgw_hip_server* server, int64_t ticket, double* max_rel_err,
+                             int32_t* fell_back);
+
+/* Forget a finished job: its record (status, error, judge outcome) is kept
+ * until this call or the server's destruction. */
+int mgw_hip_server_release(mgw_hip_server* server, int64_t ticket);
 
 /* Jobs finished so far, jobs running now, most jobs running at once, bytes
  * reserved by running jobs. */
@@ -279,6 +329,18 @@ int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_ba
  * frequencies (0 = not tuned / default). */
 int mgw_hip_autotune_result(const mgw_g0w0_job* job, int32_t* batch, int32_t* unit_freqs);
 
+/* Stop flag of ctx (no counterpart in the reference): mgw_hip_g0w0 reads
+ * *flag (atomically) before each frequency batch and, when it is non-zero,
+ * waits for the work in flight and fails with MGW_ERR_STATE ("cancelled"),
+ * writing no results.  NULL removes the flag; the caller keeps it valid while
+ * it is set. */
+int mgw_hip_set_cancel_flag(mgw_hip_ctx* ctx, const int32_t* flag);
+
+/* Cap on the frequencies mgw_hip_g0w0 factors at once on ctx (0 = none: the
+ * largest batch that fits the device budget, at most 64, or the autotuned
+ * one).  Scheduling only; results do not depend on it. */
+int mgw_hip_set_batch_limit(mgw_hip_ctx* ctx, int32_t max_freqs);
+
 /* Run each frequency batch of mgw_hip_g0w0 as a captured hipGraph, replayed
  * for later batches and molecules of the same shape on the same buffers
  * (enable = 1), or launch kernels directly (0, the default, which also gives
@@ -290,32 +352,6 @@ int mgw_hip_set_graphs(mgw_hip_ctx* ctx, int32_t enable);
  * (largest) pivot; every residual entry is then bounded by that value.
  * 0 <= tol < 1, default 1e-12.  No counterpart in the reference. */
 int mgw_hip_set_cd_tolerance(mgw_hip_ctx* ctx, double tol);
-
-/* Diagnostic with no counterpart in the reference: C = A B^T for 64 x 64
- * row-major A, B through the factorization kernels' FP64 MFMA tile (checks
- * the operand / result lane layout on a new device). */
-int mgw_hip_selftest_tile(mgw_hip_ctx* ctx, const double* a, const double* b, double* c);
-
-/* Diagnostic with no counterpart in the reference: ntests single-MFMA runs
- * with per-lane inputs, raw per-lane results (lane layout check).
- * kind 0: v_mfma_f64_16x16x4, a/b double[ntests][64], d double[ntests][64][4];
- * kind 1/2: v_mfma_f32_16x16x32 bf16/f16, a/b uint16[ntests][64][8],
- * d float[ntests][64][4]; kind 3/4: v_mfma_f32_32x32x16 bf16/f16, same
- * inputs, d float[ntests][64][16]; kind 5: v_mfma_i32_32x32x32_i8,
- * a/b int8[ntests][64][16], d int32[ntests][64][16]. */
-int mgw_hip_probe_mfma(mgw_hip_ctx* ctx, int32_t kind, int32_t ntests, const void* a,
-                       const void* b, void* d);
-
-/* Diagnostic with no counterpart in the reference: measured peaks of this
- * device -- out[0] FP64 MFMA, out[1] FP16 MFMA, out[2] BF16 MFMA (flop/s,
- * 16x16 tiles, register-resident operands, 32 waves per CU), out[3] HBM copy
- * bandwidth (bytes read + written per second, 2 GiB buffers), out[4] FP16 and
- * out[5] BF16 MFMA with 32x32x16 tiles, out[6] int8 MFMA 32x32x32 and out[7]
- * int8 16x16x64 (op/s), all with constant operands; out[8] int8 32x32x32 and
- * out[9] FP64 16x16x4 with random operands (bytes in [-64, 64], doubles in
- * [-1, 1)): MFMA throughput depends on the operand data, and these are the
- * rates contractions on real data can approach.  out holds 10 entries. */
-int mgw_hip_measure_peaks(mgw_hip_ctx* ctx, double* out);
 
 /* Transformed Gauss-Legendre grid (src/gw/frequency_grids.cpp:62-93). */
 int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight);

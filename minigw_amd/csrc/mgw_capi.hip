@@ -2,6 +2,7 @@
 // host runtime behind it: device buffers, the frequency grid, batching of
 // imaginary frequencies, and the hoisted G0W0 driver.
 #include "mgw_hip.h"
+#include "mgw_hip_diag.h"
 
 #include "mgw_kernels.h"
 
@@ -122,6 +123,10 @@ struct mgw_hip_ctx {
 
     // frequency batch forced by the autotuner while it times candidates (0 = auto)
     int forced_nb = 0;
+    // caller's cap on the frequencies factored at once (mgw_hip_set_batch_limit, 0 = none)
+    int max_nb = 0;
+    // caller's stop flag, polled between frequency batches (mgw_hip_set_cancel_flag)
+    const int32_t* cancel = nullptr;
     // frequencies per 32-tile unit of the mixed contractions: MGW_SPLIT_UNIT_FREQS
     // (0 = the autotuned value, else 4); forced_uf while the autotuner times one
     int split_uf = 0;
@@ -163,13 +168,21 @@ struct mgw_hip_ctx {
     double cd_tol = 1e-12;            // stop when the residual diagonal <= cd_tol * first pivot
     bool int8_factor = true;          // mixed forms: block products on int8 MFMA (MGW_INT8_FACTOR=0: FP64)
     bool wc_complex = false;
+    // error judge of the mixed forms (mgw_hip_set_judge) and its last outcome
+    int judge_n = 0;                  // sampled frequencies (0 = off)
+    double judge_tol = 0.0;           // <= 0: the form's default
+    bool judge_fallback = false;      // rerun in FP64 when the judge fails
+    double judge_err = -1.0;          // largest relative q error of the last judged run (-1: none)
+    int judge_used = 0;               // frequencies the last run judged
+    int fell_back = 0;                // the last mgw_hip_g0w0 reran in FP64 (1: judge, 2: numeric)
+    Buf hqj;                          // the judge's q row
 
     mgw_hip_ctx() {
         all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
-               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg};
+               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -624,6 +637,44 @@ int mgw_hip_set_cd_tolerance(mgw_hip_ctx* ctx, double tol) {
     });
 }
 
+int mgw_hip_set_cancel_flag(mgw_hip_ctx* ctx, const int32_t* flag) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_cancel_flag: ctx is NULL");
+        ctx->cancel = flag;
+    });
+}
+
+int mgw_hip_set_batch_limit(mgw_hip_ctx* ctx, int32_t max_freqs) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_batch_limit: ctx is NULL");
+        need(max_freqs >= 0, "mgw_hip_set_batch_limit: max_freqs must be >= 0");
+        ctx->max_nb = max_freqs;
+    });
+}
+
+int mgw_hip_set_judge(mgw_hip_ctx* ctx, int32_t nsample, double tol, int32_t fallback) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_judge: ctx is NULL");
+        need(nsample >= 0 && nsample <= 64, "mgw_hip_set_judge: nsample must be in [0, 64]");
+        need(tol >= 0.0 && std::isfinite(tol), "mgw_hip_set_judge: tol must be finite and >= 0");
+        ctx->judge_n = nsample;
+        ctx->judge_tol = tol;
+        ctx->judge_fallback = fallback != 0;
+    });
+}
+
+int mgw_hip_last_judge(const mgw_hip_ctx* ctx, double* max_rel_err, int32_t* samples,
+                       int32_t* fell_back) {
+    if (!ctx) {
+        g_thread_error = "mgw_hip_last_judge: ctx is NULL";
+        return MGW_ERR_ARG;
+    }
+    if (max_rel_err) *max_rel_err = ctx->judge_err;
+    if (samples) *samples = ctx->judge_used;
+    if (fell_back) *fell_back = ctx->fell_back;
+    return MGW_OK;
+}
+
 int mgw_grid_transformed_legendre(int32_t nfreq, double* omega, double* weight) {
     return guarded(nullptr, [&] {
         need(nfreq > 0, "generate_transformed_legendre_grid: num_freq_points must be positive");
@@ -921,8 +972,11 @@ int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t wid
 }
 
 // ---------------------------------------------------- hoisted G0W0
-int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
-    return guarded(ctx, [&] {
+namespace {
+// One molecule in the job's form.  Returns false, having written nothing to
+// `out`, when the error judge of a mixed form failed and the context asks for
+// the FP64 fallback.
+bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         need(ctx && job && out, "mgw_hip_g0w0: NULL argument");
         const int nmo = job->nmo, nocc = job->nocc, nfreq = job->nfreq, npar = job->npar;
         need(nmo > 1 && nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
@@ -1084,6 +1138,14 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                                           reinterpret_cast<int8_t*>(xh), xs, n, np, mpad, st),
                       "slice X");
         }
+        // the error judge's FP64 X (mixed forms), before the batch takes the room
+        double* xj = nullptr;
+        if ((mixed || i8) && ctx->judge_n > 0) {
+            xj = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X (judge)"));
+            hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xj, n,
+                                         np, mpad, st),
+                      "pack X (judge)");
+        }
         hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
 
         // per-molecule outputs before the batch, which takes the room that is left
@@ -1095,7 +1157,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         double2* gq = as<double2>(
             ctx->get(ctx->hsg, mgw::sigma_workspace_bytes(nmo, ns, nfreq), "quadrature factors"));
         int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
-        int* f0p = as<int>(ctx->get(ctx->f0buf, sizeof(int), "batch offset"));
+        int* f0p = as<int>(ctx->get(ctx->f0buf, 2 * sizeof(int), "batch offset"));
 
         // frequency batch size from the device budget
         size_t free_b = 0, total_b = 0;
@@ -1109,6 +1171,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const Tuned tn = tuned(nmo, nocc, ns, job->precision);
         const int pick = ctx->forced_nb ? ctx->forced_nb : tn.nb;
         if (pick > 0) nb = std::min(nb, pick);
+        if (ctx->max_nb > 0) nb = std::min(nb, ctx->max_nb);
         // untuned unit shape: measured at nmo=200, 16 frequencies per unit for
         // the int8 contraction (1-1.5 % faster than 4), 4 for the FP16 split
         const int uf = ctx->forced_uf ? ctx->forced_uf
@@ -1220,6 +1283,10 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
         };
         for (int f0 = 0; f0 < nfreq; f0 += nb) {
+            if (ctx->cancel && __atomic_load_n(ctx->cancel, __ATOMIC_ACQUIRE) != 0) {
+                ctx->sync();
+                fail(MGW_ERR_STATE, "mgw_hip_g0w0: cancelled after %d of %d frequencies", f0, nfreq);
+            }
             const int b = std::min(nb, nfreq - f0);
             hip_check(mgw::launch_set_int(f0p, f0, st), "set offset");
             if (ctx->use_graphs && !cplx) {
@@ -1267,6 +1334,45 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         check_info(ctx, info, nfreq, "potrf");
         ctx->wc_ready = false;
 
+        // Error judge of the mixed forms: a sample of the frequencies recomputed
+        // with every step in FP64 (FP64 dielectric, factorization, contraction),
+        // its q compared with the form's.  The relative error is the largest
+        // |q_form - q_fp64| over the largest |q_fp64| of each sampled frequency.
+        if ((mixed || i8) && ctx->judge_n > 0) {
+            const int nj = std::min(ctx->judge_n, nfreq);
+            const double tol = ctx->judge_tol > 0.0 ? ctx->judge_tol : i8 ? 1e-8 : 1e-6;
+            double* qj = as<double>(ctx->get(ctx->hqj, (size_t)m * 8, "judge q"));
+            std::vector<double> qa(m), qb(m);
+            double worst = 0.0;
+            for (int i = 0; i < nj; ++i) {
+                const int f = (int)(((int64_t)2 * i + 1) * nfreq / (2 * nj));
+                hip_check(mgw::launch_set_int(f0p, f, st), "set offset");
+                hip_check(mgw::launch_set_int(f0p + 1, 0, st), "set offset");
+                hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, 1, st), "scale");
+                hip_check(mgw::launch_dielectric(dV, S, A, np, 1, st), "dielectric");
+                factor_batch(ctx, A, dinv, info + f, tbuf, np, 1);
+                hip_check(mgw::launch_contract_f64(A, S, xj, qpart, np, mpad, 1, st), "contract");
+                hip_check(mgw::launch_reduce_q(qpart, dcolmap, qj, np / mgw::kCT, m, mpad, f0p + 1, 1,
+                                               st),
+                          "reduce q");
+                ctx->d2h(qa.data(), qj, m);
+                ctx->d2h(qb.data(), dq + (size_t)f * m, m);
+                ctx->sync();
+                double e = 0.0, sc = 0.0;
+                for (int c = 0; c < m; ++c) {
+                    e = std::max(e, std::fabs(qa[c] - qb[c]));
+                    sc = std::max(sc, std::fabs(qa[c]));
+                }
+                const double rel = sc > 0.0 ? e / sc : (e > 0.0 ? INFINITY : 0.0);
+                worst = std::max(worst, std::isnan(rel) ? INFINITY : rel);
+            }
+            check_info(ctx, info, nfreq, "potrf (judge)");
+            ctx->judge_err = worst;
+            ctx->judge_used = nj;
+            if (!(worst <= tol) && ctx->judge_fallback) return false;
+        }
+        if (out->q) ctx->d2h(out->q, dq, (size_t)nfreq * m);
+
         hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
         hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, gq,
                                       sre, sim, st),
@@ -1305,6 +1411,35 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->timings[11] = t_graph;
         ctx->timings[12] = cd_rank;
         ctx->timings[13] = cd_ms;
+        return true;
+}
+}  // namespace
+
+int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
+    return guarded(ctx, [&] {
+        need(ctx && job && out, "mgw_hip_g0w0: NULL argument");
+        ctx->fell_back = 0;
+        ctx->judge_err = -1.0;
+        ctx->judge_used = 0;
+        const bool mixed_form =
+            job->precision == MGW_PREC_MIXED || job->precision == MGW_PREC_MIXED_INT8;
+        bool done = false;
+        try {
+            done = g0w0_body(ctx, job, out);
+        } catch (const MgwError& e) {
+            // a mixed form's factorization breakdown falls back like a failed judge
+            if (!(mixed_form && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC)) throw;
+            ctx->fell_back = 2;
+        }
+        if (done) return;
+        if (!ctx->fell_back) ctx->fell_back = 1;
+        mgw_g0w0_job plain = *job;
+        plain.precision = MGW_PREC_FP64;
+        const double err = ctx->judge_err;
+        const int used = ctx->judge_used;
+        g0w0_body(ctx, &plain, out);
+        ctx->judge_err = err;
+        ctx->judge_used = used;
     });
 }
 

@@ -4,15 +4,23 @@
 // while others run overlap on the GPU.  Admission is by an HBM estimate:
 // a job starts when the bytes reserved by running jobs plus its estimate
 // (resident ERI + V + frequency batch + contraction scratch,
-// mgw_hip_estimate_bytes) fit the server's budget; jobs start in arrival
-// order.  A job whose estimate exceeds the whole budget fails with
-// MGW_ERR_MEMORY instead of waiting forever.  After each job a worker frees
-// its device buffers, so the reservation is the job's footprint only.
-// Destroying the server cancels jobs that have not started and waits for the
-// running ones.
+// mgw_hip_estimate_bytes, plus the error judge's FP64 panel for the mixed
+// forms) fit the server's budget; jobs start in arrival order.  A job whose
+// estimate exceeds the whole budget fails with MGW_ERR_MEMORY instead of
+// waiting forever.  After each job a worker frees its device buffers, so the
+// reservation is the job's footprint only.
+//
+// Mixed-precision jobs run under the context's error judge: a sample of their
+// frequencies is recomputed in FP64, and a failed judge or a factorization
+// breakdown reruns the job in FP64 (mgw_hip_set_judge).  A job can be
+// cancelled before it starts or between two of its frequency batches.
+// Finished jobs keep their record (status, error, judge outcome) until
+// mgw_hip_server_release.  Destroying the server cancels jobs that have not
+// started and waits for the running ones.
 #include "mgw_hip.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -39,9 +47,23 @@ struct ServeJob {
     mgw_g0w0_out out{};
     size_t estimate = 0;
     int status = -1;          // -1 pending, -2 running, else final mgw_status
+    int32_t cancel = 0;       // polled by the running job between frequency batches
+    double judge_err = -1.0;
+    int32_t fell_back = 0;
     std::string error;
     Clock::time_point t_submit, t_start, t_end;
 };
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Device bytes of the error judge of a mixed job: its FP64 panel of the
+// contraction columns (mpad x np) and one row of quadratic forms.
+size_t judge_bytes(const mgw_g0w0_job& g) {
+    const size_t ns = g.states && g.nstates > 0 ? (size_t)g.nstates : (size_t)g.nmo;
+    const size_t m = ns * (size_t)g.nmo;
+    const size_t n = (size_t)g.nocc * (size_t)(g.nmo - g.nocc);
+    return round_up(m, 256) * round_up(n, 256) * 8 + m * 8;
+}
 
 }  // namespace
 
@@ -49,6 +71,11 @@ struct mgw_hip_server {
     int device = 0;
     size_t budget = 0;
     int32_t batch_freqs = 32;
+    // error judge of mixed jobs, applied to each worker's context
+    int32_t judge_n = 2;
+    double judge_tol = 0.0;
+    int32_t judge_fallback = 1;
+    double budget_scale = 1.0;        // fault-injection hook: per-job device budget scale
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServeJob*> pending;
@@ -59,7 +86,14 @@ struct mgw_hip_server {
     int64_t completed = 0;
     bool stop = false;
     std::vector<std::thread> workers;
-    std::string error;
+
+    // a job that ends before it starts (lock held)
+    void finish_unstarted(ServeJob* j, int status, const char* why) {
+        j->status = status;
+        j->error = why;
+        j->t_start = j->t_end = Clock::now();
+        ++completed;
+    }
 
     void work() {
         mgw_hip_ctx* ctx = nullptr;
@@ -67,33 +101,34 @@ struct mgw_hip_server {
         if (rc == MGW_OK) mgw_hip_set_graphs(ctx, 1);   // batches of a job replay one graph
         for (;;) {
             ServeJob* j = nullptr;
+            int32_t jn = 0, jf = 0;
+            double jt = 0.0, scale = 1.0;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] {
                     if (stop) return true;
                     if (pending.empty()) return false;
                     ServeJob* h = pending.front();
-                    return h->estimate > budget || reserved + h->estimate <= budget;
+                    return h->cancel || h->estimate > budget || reserved + h->estimate <= budget;
                 });
                 if (stop) {
                     while (!pending.empty()) {
-                        ServeJob* c = pending.front();
+                        finish_unstarted(pending.front(), MGW_ERR_STATE,
+                                         "server stopped before the job started");
                         pending.pop_front();
-                        c->status = MGW_ERR_STATE;
-                        c->error = "server stopped before the job started";
-                        c->t_start = c->t_end = Clock::now();
                     }
                     cv_done.notify_all();
                     break;
                 }
                 j = pending.front();
                 pending.pop_front();
-                if (j->estimate > budget || rc != MGW_OK) {
-                    j->status = rc != MGW_OK ? rc : MGW_ERR_MEMORY;
-                    j->error = rc != MGW_OK ? "worker could not create a device context"
-                                            : "job estimate exceeds the server's HBM budget";
-                    j->t_start = j->t_end = Clock::now();
-                    ++completed;
+                if (j->cancel || j->estimate > budget || rc != MGW_OK) {
+                    if (j->cancel)
+                        finish_unstarted(j, MGW_ERR_STATE, "cancelled before the job started");
+                    else if (rc != MGW_OK)
+                        finish_unstarted(j, rc, "worker could not create a device context");
+                    else
+                        finish_unstarted(j, MGW_ERR_MEMORY, "job estimate exceeds the server's HBM budget");
                     cv_done.notify_all();
                     cv_work.notify_all();
                     continue;
@@ -103,19 +138,32 @@ struct mgw_hip_server {
                 peak_running = std::max(peak_running, running);
                 j->status = -2;
                 j->t_start = Clock::now();
+                jn = judge_n;
+                jt = judge_tol;
+                jf = judge_fallback;
+                scale = budget_scale;
             }
-            int st = mgw_hip_set_memory_budget(ctx, j->estimate);
+            int st = mgw_hip_set_memory_budget(ctx, std::max<size_t>(1, (size_t)((double)j->estimate * scale)));
+            if (st == MGW_OK) st = mgw_hip_set_batch_limit(ctx, batch_freqs);
+            if (st == MGW_OK) st = mgw_hip_set_judge(ctx, jn, jt, jf);
+            if (st == MGW_OK) st = mgw_hip_set_cancel_flag(ctx, &j->cancel);
             if (st == MGW_OK) {
                 st = j->factor ? mgw_hip_upload_eri_factor(ctx, j->factor, j->job.nmo, j->job.nocc, j->kdim)
                                : mgw_hip_upload_eri(ctx, j->eri, j->job.nmo, j->job.nocc);
             }
             if (st == MGW_OK) st = mgw_hip_g0w0(ctx, &j->job, &j->out);
             std::string err = st == MGW_OK ? "" : mgw_hip_last_error(ctx);
+            double jerr = -1.0;
+            int32_t fell = 0;
+            if (st == MGW_OK) mgw_hip_last_judge(ctx, &jerr, nullptr, &fell);
+            mgw_hip_set_cancel_flag(ctx, nullptr);
             mgw_hip_release_buffers(ctx);
             {
                 std::lock_guard<std::mutex> lk(mu);
                 j->status = st;
                 j->error = err;
+                j->judge_err = jerr;
+                j->fell_back = fell;
                 j->t_end = Clock::now();
                 reserved -= j->estimate;
                 --running;
@@ -125,6 +173,11 @@ struct mgw_hip_server {
             cv_work.notify_all();
         }
         if (ctx) mgw_hip_destroy(ctx);
+    }
+
+    ServeJob* find(int64_t ticket) {
+        auto it = jobs.find(ticket);
+        return it == jobs.end() ? nullptr : it->second.get();
     }
 };
 
@@ -154,6 +207,22 @@ int mgw_hip_server_destroy(mgw_hip_server* s) {
     return MGW_OK;
 }
 
+int mgw_hip_server_set_judge(mgw_hip_server* s, int32_t nsample, double tol, int32_t fallback) {
+    if (!s || nsample < 0 || nsample > 64 || !(tol >= 0.0)) return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->judge_n = nsample;
+    s->judge_tol = tol;
+    s->judge_fallback = fallback != 0;
+    return MGW_OK;
+}
+
+int mgw_hip_server_set_budget_scale(mgw_hip_server* s, double scale) {
+    if (!s || !(scale > 0.0 && scale <= 1.0)) return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->budget_scale = scale;
+    return MGW_OK;
+}
+
 int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw_g0w0_out* out,
                           int64_t* ticket) {
     if (!s || !job || !out || !ticket || !job->job.mo_energy || !job->job.vxc_diag || !out->qp)
@@ -161,6 +230,7 @@ int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw
     if ((job->eri == nullptr) == (job->eri_factor == nullptr)) return MGW_ERR_ARG;
     const mgw_g0w0_job& g = job->job;
     if (g.nmo <= 1 || g.nocc <= 0 || g.nocc >= g.nmo || g.nfreq <= 0) return MGW_ERR_ARG;
+    if (g.states && g.nstates > g.nmo) return MGW_ERR_ARG;
     auto j = std::make_unique<ServeJob>();
     j->job = g;
     j->mo.assign(g.mo_energy, g.mo_energy + g.nmo);
@@ -182,10 +252,12 @@ int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw
     j->estimate = mgw_hip_estimate_bytes(g.nmo, g.nocc, g.nfreq, g.nstates, nb, g.precision);
     if (job->eri_factor) {
         const size_t n2 = (size_t)g.nmo * g.nmo;
-        j->estimate += ((n2 + 63) / 64 * 64) * ((size_t)(job->kdim + 15) / 16 * 16) * 8;
+        j->estimate += round_up(n2, 64) * round_up((size_t)job->kdim, 16) * 8;
     }
-    j->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->judge_n > 0 && (g.precision == MGW_PREC_MIXED || g.precision == MGW_PREC_MIXED_INT8))
+        j->estimate += judge_bytes(g);
+    j->t_submit = Clock::now();
     j->ticket = s->next_ticket++;
     *ticket = j->ticket;
     s->pending.push_back(j.get());
@@ -194,13 +266,30 @@ int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw
     return MGW_OK;
 }
 
+int mgw_hip_server_cancel(mgw_hip_server* s, int64_t ticket) {
+    if (!s) return MGW_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ServeJob* j = s->find(ticket);
+        if (!j) return MGW_ERR_ARG;
+        if (j->status >= 0) return MGW_OK;                 // already finished
+        __atomic_store_n(&j->cancel, 1, __ATOMIC_RELEASE);
+        if (j->status == -1) {                             // pending: finish it here
+            s->pending.erase(std::find(s->pending.begin(), s->pending.end(), j));
+            s->finish_unstarted(j, MGW_ERR_STATE, "cancelled before the job started");
+        }
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+    return MGW_OK;
+}
+
 int mgw_hip_server_wait(mgw_hip_server* s, int64_t ticket, double timeout_s, int32_t* status,
                         double* latency_ms, double* run_ms) {
     if (!s || !status) return MGW_ERR_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(ticket);
-    if (it == s->jobs.end()) return MGW_ERR_ARG;
-    ServeJob* j = it->second.get();
+    ServeJob* j = s->find(ticket);
+    if (!j) return MGW_ERR_ARG;
     auto done = [&] { return j->status >= 0; };
     if (timeout_s < 0) {
         s->cv_done.wait(lk, done);
@@ -212,15 +301,34 @@ int mgw_hip_server_wait(mgw_hip_server* s, int64_t ticket, double timeout_s, int
     if (latency_ms)
         *latency_ms = std::chrono::duration<double, std::milli>(j->t_end - j->t_submit).count();
     if (run_ms) *run_ms = std::chrono::duration<double, std::milli>(j->t_end - j->t_start).count();
-    if (j->status != MGW_OK) s->error = j->error;
     return MGW_OK;
 }
 
 const char* mgw_hip_server_job_error(mgw_hip_server* s, int64_t ticket) {
     if (!s) return "";
     std::lock_guard<std::mutex> lk(s->mu);
+    ServeJob* j = s->find(ticket);
+    return j ? j->error.c_str() : "unknown ticket";
+}
+
+int mgw_hip_server_job_judge(mgw_hip_server* s, int64_t ticket, double* max_rel_err,
+                             int32_t* fell_back) {
+    if (!s) return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    ServeJob* j = s->find(ticket);
+    if (!j || j->status < 0) return MGW_ERR_ARG;
+    if (max_rel_err) *max_rel_err = j->judge_err;
+    if (fell_back) *fell_back = j->fell_back;
+    return MGW_OK;
+}
+
+int mgw_hip_server_release(mgw_hip_server* s, int64_t ticket) {
+    if (!s) return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->jobs.find(ticket);
-    return it == s->jobs.end() ? "unknown ticket" : it->second->error.c_str();
+    if (it == s->jobs.end() || it->second->status < 0) return MGW_ERR_ARG;
+    s->jobs.erase(it);
+    return MGW_OK;
 }
 
 int mgw_hip_server_stats(mgw_hip_server* s, int64_t* completed, int32_t* running,

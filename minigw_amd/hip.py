@@ -50,7 +50,7 @@ class G0W0Job(C.Structure):
 
 class G0W0Out(C.Structure):
     _fields_ = [("omega", _dp), ("weight", _dp), ("sigma_x_diag", _dp), ("sigma_c_re", _dp),
-                ("sigma_c_im", _dp), ("qp", _dp), ("qp_iterations", _ip)]
+                ("sigma_c_im", _dp), ("qp", _dp), ("qp_iterations", _ip), ("q", _dp)]
 
 
 class ServeJob(C.Structure):
@@ -87,6 +87,8 @@ def _load():
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
         "mgw_hip_set_graphs": (C.c_int, [vp, C.c_int32]),
         "mgw_hip_set_cd_tolerance": (C.c_int, [vp, C.c_double]),
+        "mgw_hip_set_judge": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32]),
+        "mgw_hip_last_judge": (C.c_int, [vp, _dp, _ip, _ip]),
         "mgw_hip_autotune": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_double)]),
         "mgw_hip_autotune_result": (C.c_int, [C.POINTER(G0W0Job), C.POINTER(C.c_int32),
@@ -103,6 +105,13 @@ def _load():
         "mgw_hip_server_wait": (C.c_int, [vp, C.c_int64, C.c_double, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mgw_hip_server_job_error": (C.c_char_p, [vp, C.c_int64]),
+        "mgw_hip_server_set_judge": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32]),
+        "mgw_hip_server_set_budget_scale": (C.c_int, [vp, C.c_double]),
+        "mgw_hip_server_cancel": (C.c_int, [vp, C.c_int64]),
+        "mgw_hip_server_job_judge": (C.c_int, [vp, C.c_int64, _dp, _ip]),
+        "mgw_hip_server_release": (C.c_int, [vp, C.c_int64]),
+        "mgw_hip_set_cancel_flag": (C.c_int, [vp, _ip]),
+        "mgw_hip_set_batch_limit": (C.c_int, [vp, C.c_int32]),
         "mgw_hip_server_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                            C.POINTER(C.c_int32), C.POINTER(C.c_size_t)]),
     }
@@ -211,6 +220,25 @@ class HipContext:
         """Stopping tolerance of the PREC_FP64_CD pair-matrix decomposition
         (relative to its first pivot; default 1e-12)."""
         self.call("mgw_hip_set_cd_tolerance", float(tol))
+
+    def set_cancel_flag(self, flag):
+        """flag: a ctypes c_int32 polled between frequency batches (None removes it)."""
+        self._cancel = flag
+        self.call("mgw_hip_set_cancel_flag", None if flag is None else C.pointer(flag))
+
+    def set_batch_limit(self, max_freqs: int):
+        self.call("mgw_hip_set_batch_limit", int(max_freqs))
+
+    def set_judge(self, nsample: int, tol: float = 0.0, fallback: bool = True):
+        """Error judge of the mixed forms: nsample frequencies recomputed in FP64
+        (0 = off); with fallback, a failed judge or a factorization breakdown
+        reruns the job in FP64."""
+        self.call("mgw_hip_set_judge", int(nsample), float(tol), 1 if fallback else 0)
+
+    def last_judge(self) -> dict:
+        e, n, f = C.c_double(0), C.c_int32(0), C.c_int32(0)
+        self.call("mgw_hip_last_judge", C.byref(e), C.byref(n), C.byref(f))
+        return {"max_rel_err": e.value, "samples": n.value, "fell_back": f.value}
 
     def release_buffers(self):
         self.call("mgw_hip_release_buffers")
@@ -330,12 +358,14 @@ class HipG0W0Result:
     qp: np.ndarray             # (nmo,), NaN for unselected states
     qp_iterations: np.ndarray
     timings: dict
+    q: np.ndarray | None = None   # (nfreq, nstates * nmo) quadratic forms, if asked for
 
 
 def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.0,
              precision: int = PREC_FP64, ctx: HipContext | None = None,
-             upload: bool = True) -> HipG0W0Result:
-    """G0W0 for one molecule on the device (ERI uploaded unless upload=False)."""
+             upload: bool = True, want_q: bool = False) -> HipG0W0Result:
+    """G0W0 for one molecule on the device (ERI uploaded unless upload=False);
+    want_q also returns the quadratic forms q[f, s * nmo + k]."""
     ctx = ctx or HipContext()
     nmo = inp.nmo
     if upload:
@@ -349,10 +379,12 @@ def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.
     om, w, sx = np.zeros(nfreq), np.zeros(nfreq), np.zeros(nmo)
     sre, sim = np.zeros((ns, nfreq)), np.zeros((ns, nfreq))
     qp, it = np.zeros(nmo), np.zeros(nmo, dtype=np.int32)
-    out = G0W0Out(_ptr(om), _ptr(w), _ptr(sx), _ptr(sre), _ptr(sim), _ptr(qp), _ptr(it, _ip))
+    q = np.zeros((nfreq, ns * nmo)) if want_q else None
+    out = G0W0Out(_ptr(om), _ptr(w), _ptr(sx), _ptr(sre), _ptr(sim), _ptr(qp), _ptr(it, _ip),
+                  _ptr(q))
     ctx.call("mgw_hip_g0w0", C.byref(job), C.byref(out))
     return HipG0W0Result(om, w, list(range(nmo)) if st is None else st.tolist(), sx,
-                         sre + 1j * sim, qp, it, ctx.last_timings())
+                         sre + 1j * sim, qp, it, ctx.last_timings(), q)
 
 
 def pade_qp(sigma_c, omegas, sigma_x_diag, mo_energy, vxc_diag, fermi: float, npar: int = 16,
@@ -429,20 +461,38 @@ class HipServer:
         self._keep[t.value] = (res, eri, fac, st, nmo)
         return t.value
 
+    def set_judge(self, nsample: int, tol: float = 0.0, fallback: bool = True):
+        """Error judge of mixed jobs submitted from now on (0 samples: off)."""
+        _check("mgw_hip_server_set_judge",
+               lib.mgw_hip_server_set_judge(self.h, int(nsample), float(tol), 1 if fallback else 0))
+
+    def set_budget_scale(self, scale: float):
+        """Fault injection: per-job device budget = scale x estimate."""
+        _check("mgw_hip_server_set_budget_scale", lib.mgw_hip_server_set_budget_scale(self.h, scale))
+
+    def cancel(self, ticket: int):
+        _check("mgw_hip_server_cancel", lib.mgw_hip_server_cancel(self.h, ticket))
+
     def wait(self, ticket: int, timeout: float = -1.0):
-        """Returns (HipG0W0Result, latency_ms, run_ms); raises HipError on failure."""
+        """Returns (HipG0W0Result, latency_ms, run_ms); raises HipError on failure.
+        The result's timings hold the judge outcome ("judge_err", "fell_back").
+        The job's record is released once it has been reported."""
         st, lat, run = C.c_int32(0), C.c_double(0), C.c_double(0)
         _check("mgw_hip_server_wait", lib.mgw_hip_server_wait(self.h, ticket, timeout, C.byref(st),
                                                                C.byref(lat), C.byref(run)))
         if st.value == -1:
             raise TimeoutError(f"job {ticket} not finished")
         res, _, _, stt, nmo = self._keep.pop(ticket)
+        je, fb = C.c_double(-1.0), C.c_int32(0)
+        lib.mgw_hip_server_job_judge(self.h, ticket, C.byref(je), C.byref(fb))
+        msg = lib.mgw_hip_server_job_error(self.h, ticket)
+        lib.mgw_hip_server_release(self.h, ticket)
         if st.value != MGW_OK:
-            msg = lib.mgw_hip_server_job_error(self.h, ticket)
             raise HipError("mgw_hip_server_wait", st.value, msg.decode() if msg else "")
         out = HipG0W0Result(res["omega"], res["weight"],
                             list(range(nmo)) if stt is None else stt.tolist(), res["sx"],
-                            res["sre"] + 1j * res["sim"], res["qp"], res["it"], {})
+                            res["sre"] + 1j * res["sim"], res["qp"], res["it"],
+                            {"judge_err": je.value, "fell_back": fb.value})
         return out, lat.value, run.value
 
     def stats(self) -> dict:

@@ -172,6 +172,29 @@ def pq_panel(eri: np.ndarray, ph: PhBasis, p: int, k0: int, width: int) -> np.nd
     return np.ascontiguousarray(blk.reshape(width, ph.size).T)
 
 
+def ph_rows(ph: PhBasis) -> np.ndarray:
+    """Row of each ph pair (i, a) in an (nmo^2)-row pair layout: i * nmo + a_mo."""
+    return ph.i_occ.astype(np.int64) * ph.nmo + ph.a_mo.astype(np.int64)
+
+
+def v_ph_from_factor(b: np.ndarray, ph: PhBasis) -> np.ndarray:
+    """V_ph of an ERI given by its factor, eri[pq][rs] = sum_K B[pq, K] B[rs, K]
+    (the synthetic inputs): V[jb, ia] = (jb|ia) = B_jb . B_ia.  The same matrix
+    as v_ph_matrix on the expanded ERI (src/gw/sigma.cpp:56-71), without
+    forming the nmo^4 tensor."""
+    bp = b[ph_rows(ph)]
+    return bp @ bp.T
+
+
+def pq_columns_from_factor(b: np.ndarray, ph: PhBasis, cols) -> np.ndarray:
+    """X[ia, c] = (p_c k_c | i a) for the pairs cols = [(p, k), ...] of an ERI
+    given by its factor: the panel columns of pq_panel
+    (src/workspace/pq_ph_panel.cpp:25-47) for arbitrary (p, k)."""
+    nmo = ph.nmo
+    rows = np.array([p * nmo + k for p, k in cols], dtype=np.int64)
+    return b[ph_rows(ph)] @ b[rows].T
+
+
 # --------------------------------------------------------------------------
 # a7-a9: W_c = (I - diag(Pi0) V)^-1 diag(Pi0)   (src/workspace/screening_workspace.cpp:21-66,
 #        BlasLapackBackend::inverse = getrf + getrs against I, src/linalg/linalg_blas_lapack.cpp:106-140)
@@ -195,6 +218,20 @@ def quadratic_forms_panel(w_c: np.ndarray, x: np.ndarray) -> np.ndarray:
     if w_c.shape != (n, n) or x.shape[0] != n:
         raise ValueError("HostScreeningWorkspace::quadratic_forms_panel: inconsistent dimensions")
     y = w_c @ x
+    return np.einsum("ij,ij->j", x, y)
+
+
+def quadratic_forms_solve(v: np.ndarray, pi0: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """q_j = x_j^T W_c x_j with W_c x_j = eps^-1 (Pi0 * x_j): the same LU of eps
+    as compute_w_c (getrf), solved against the Pi0-scaled panel instead of the
+    identity (getrs with the panel as right-hand side), so large orders cost
+    n^2 per column instead of n^3 (src/workspace/screening_workspace.cpp:21-102
+    in exact arithmetic)."""
+    n = v.shape[0]
+    eps = -pi0[:, None] * v
+    eps[np.diag_indices(n)] += 1.0
+    lu, piv = sla.lu_factor(eps, check_finite=False)
+    y = sla.lu_solve((lu, piv), pi0[:, None] * x, check_finite=False)
     return np.einsum("ij,ij->j", x, y)
 
 

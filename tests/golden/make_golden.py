@@ -173,7 +173,7 @@ CASES = {
 
 
 # cases also run through the BLAS/LAPACK backend (ref_<name>_blas.npz)
-BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd")
+BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta", "syn_small_eta")
 
 
 def main() -> None:
