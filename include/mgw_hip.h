@@ -198,7 +198,7 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
  * recomputes nsample of its frequencies (evenly spaced) with every step in
  * FP64 and compares their quadratic forms with the form's: the error is the
  * largest |q_form - q_fp64| over the largest |q_fp64| of a frequency.  When
- * it exceeds tol (tol = 0: 1e-8 for MIXED_INT8, 1e-6 for MIXED) or the
+ * it exceeds tol (tol = 0: 1e-7 for MIXED_INT8, 1e-5 for MIXED) or the
  * form's factorization breaks down, and fallback != 0, the call reruns the
  * job in MGW_PREC_FP64 and returns that result.  nsample = 0 turns it off. */
 int mgw_hip_set_judge(mgw_hip_ctx* ctx, int32_t nsample, double tol, int32_t fallback);

@@ -1340,7 +1340,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         // |q_form - q_fp64| over the largest |q_fp64| of each sampled frequency.
         if ((mixed || i8) && ctx->judge_n > 0) {
             const int nj = std::min(ctx->judge_n, nfreq);
-            const double tol = ctx->judge_tol > 0.0 ? ctx->judge_tol : i8 ? 1e-8 : 1e-6;
+            const double tol = ctx->judge_tol > 0.0 ? ctx->judge_tol : i8 ? 1e-7 : 1e-5;
             double* qj = as<double>(ctx->get(ctx->hqj, (size_t)m * 8, "judge q"));
             std::vector<double> qa(m), qb(m);
             double worst = 0.0;

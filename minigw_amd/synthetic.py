@@ -160,3 +160,19 @@ def batch_molecules(count: int = 32, lo: int = 24, hi: int = 120, seed: int = BA
         nocc = max(1, int(round(nmo / 5)))
         out.append((nmo, nocc, seed + 1 + m))
     return out
+
+
+def serve_jobs(count: int, rate: float, lo: int = 24, hi: int = 200, seed: int = SERVE_SEED):
+    """(arrival_s, nmo, nocc, nfreq, seed) for a Poisson job stream at `rate`
+    jobs/s: exponential inter-arrival times, nmo log-uniform in [lo, hi],
+    nocc = round(nmo / 5), nfreq uniform in 100..200, per-job seeds."""
+    rng = SplitMix64(seed)
+    u = rng.uniform(3 * count)
+    out = []
+    t_arr = 0.0
+    for i in range(count):
+        t_arr += -math.log(1.0 - u[3 * i]) / rate
+        nmo = int(round(math.exp(math.log(lo) + u[3 * i + 1] * (math.log(hi) - math.log(lo)))))
+        nfreq = 100 + int(u[3 * i + 2] * 101)
+        out.append((t_arr, nmo, max(1, int(round(nmo / 5))), nfreq, seed + 1 + i))
+    return out

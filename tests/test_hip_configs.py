@@ -1,0 +1,179 @@
+"""The benchmark's workloads at their full sizes on the device (needs an MI355X).
+
+* nmo=120 (nph=2000), FP64: the quadratic forms q(w') of all 14 400 (state,
+  k) columns against the oracle's LU solve at 8 of the 200 frequencies
+  (1e-11), Sigma_c from the device's q by the oracle's quadrature, and the
+  device Pade/QP bit for bit against the oracle's on the device's Sigma_c.
+* nmo=200 (nph=6400): FP64 and int8-slice q against the oracle on a column
+  subset at two frequencies; int8 against FP64 over all 200 frequencies.
+* The 32-molecule uneven-nmo batch through the request loop (graphs, error
+  judge): every molecule bitwise equal to its run alone, two against the
+  oracle.
+* A Poisson job stream over nmo 24..200 through the request loop: every job
+  bitwise equal to its run alone.
+
+The ERIs are built on the device from their seeded factor
+(mgw_hip_upload_eri_factor); the oracle takes V_ph and the panel columns from
+the same factor (oracle.v_ph_from_factor / pq_columns_from_factor).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from minigw_amd.synthetic import (CONFIGS, GwInput, batch_molecules, eri_from_factor,
+                                  serve_jobs, synthetic_factor)
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from minigw_amd import hip as h
+    return h
+
+
+def factor_molecule(nmo, nocc, seed):
+    mo, fermi, b, vxc = synthetic_factor(nmo, nocc, seed)
+    return GwInput(mo_energy=mo, eri=None, vxc=vxc, nocc=nocc, fermi_energy=fermi), b
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_nmo120_fp64_vs_oracle(hip):
+    nmo, nocc, seed = CONFIGS["nmo120"]
+    say(f"case: nmo={nmo} nocc={nocc} nph=2000 FP64, 200 frequencies, all states")
+    inp, b = factor_molecule(nmo, nocc, seed)
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri_factor(b, nmo, nocc)
+        t = time.perf_counter()
+        res = hip.run_g0w0(inp, 200, 16, ctx=ctx, upload=False, want_q=True)
+        say(f"   device run {time.perf_counter() - t:.2f} s")
+    ph = orc.particle_hole_basis(inp.mo_energy, nocc)
+    v = orc.v_ph_from_factor(b, ph)
+    x = orc.pq_columns_from_factor(b, ph, [(p, k) for p in range(nmo) for k in range(nmo)])
+    for f in (0, 25, 50, 100, 150, 175, 190, 199):
+        want = orc.quadratic_forms_solve(v, orc.pi0_diag_real(res.omegas[f], ph.delta), x)
+        e = rel_err(res.q[f], want)
+        say(f"   frequency {f} (w = {res.omegas[f]:.3e}): rel |dq| {e:.2e}")
+        assert e < 1e-11
+    q = res.q.reshape(200, nmo, nmo)
+    assert np.all(q < 0)                                    # W_c negative definite
+    assert np.array_equal(q, q.transpose(0, 2, 1))          # q(p,k) = q(k,p)
+    w = orc.transformed_legendre_grid(200)[1]
+    sig = orc.sigma_from_q(q, inp.mo_energy, inp.fermi_energy, res.omegas, w)
+    e = rel_err(res.sigma_c, sig)
+    say(f"   Sigma_c vs the oracle's quadrature of the device q: rel {e:.2e}")
+    assert e < 1e-12
+    qp, it = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
+                          res.sigma_c, res.omegas, list(range(nmo)), 16)
+    assert np.array_equal(qp, res.qp) and np.array_equal(it, res.qp_iterations)
+    rows = np.arange(nmo)[:, None] * nmo + np.arange(nocc)[None, :]   # (p i | p i)
+    sx = -np.einsum("pik,pik->p", b[rows], b[rows])
+    assert np.allclose(res.sigma_x_diag, sx, rtol=1e-13, atol=1e-14)
+
+
+def test_nmo200_fp64_and_int8(hip):
+    nmo, nocc, seed = CONFIGS["nmo200"]
+    say(f"case: nmo={nmo} nocc={nocc} nph=6400, FP64 and int8 forms, 200 frequencies")
+    inp, b = factor_molecule(nmo, nocc, seed)
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri_factor(b, nmo, nocc)
+        t = time.perf_counter()
+        f64 = hip.run_g0w0(inp, 200, 16, ctx=ctx, upload=False, want_q=True)
+        say(f"   FP64 run {time.perf_counter() - t:.2f} s")
+        t = time.perf_counter()
+        i8 = hip.run_g0w0(inp, 200, 16, ctx=ctx, upload=False, want_q=True,
+                          precision=hip.PREC_MIXED_INT8)
+        say(f"   int8 run {time.perf_counter() - t:.2f} s")
+    es = rel_err(i8.sigma_c, f64.sigma_c)
+    eq = np.array([rel_err(i8.q[f], f64.q[f]) for f in range(200)])
+    say(f"   int8 vs FP64: Sigma_c rel {es:.2e}; q rel per frequency: median {np.median(eq):.2e}, "
+        f"worst {eq.max():.2e} at frequency {int(eq.argmax())} (w = {f64.omegas[eq.argmax()]:.3e}); "
+        f"deciles {np.array2string(np.percentile(eq, np.arange(0, 101, 10)), precision=1)}")
+    assert es < 1e-9 and eq.max() < 1e-7
+    ph = orc.particle_hole_basis(inp.mo_energy, nocc)
+    v = orc.v_ph_from_factor(b, ph)
+    cols = [(p, k) for p in (0, nocc - 1, nocc, nmo - 1) for k in range(nmo)]
+    idx = [p * nmo + k for p, k in cols]
+    x = orc.pq_columns_from_factor(b, ph, cols)
+    for f in (3, 120):
+        want = orc.quadratic_forms_solve(v, orc.pi0_diag_real(f64.omegas[f], ph.delta), x)
+        e64, e8 = rel_err(f64.q[f, idx], want), rel_err(i8.q[f, idx], want)
+        say(f"   frequency {f}: {len(cols)} columns vs oracle: FP64 rel {e64:.2e}, int8 rel {e8:.2e}")
+        assert e64 < 1e-11 and e8 < 1e-7
+    assert np.all(f64.q < 0)
+
+
+def _alone(hip, ctx, inp, b, nfreq, npar, prec):
+    ctx.upload_eri_factor(b, inp.nmo, inp.nocc)
+    return hip.run_g0w0(inp, nfreq, npar, ctx=ctx, upload=False, precision=prec)
+
+
+def test_uneven_batch_alone_equals_batched(hip):
+    mols = batch_molecules(32)
+    say(f"case: 32-molecule batch, nmo {min(m[0] for m in mols)}..{max(m[0] for m in mols)}, "
+        f"int8 form, 200 frequencies, request loop with 4 workers")
+    data = [factor_molecule(*m) for m in mols]
+    prec = hip.PREC_MIXED_INT8
+    with hip.HipServer(0, workers=4, hbm_budget=200 << 30, batch_freqs=64) as srv:
+        t = time.perf_counter()
+        tickets = [srv.submit(factor=b, mo_energy=i.mo_energy, vxc=i.vxc, nocc=i.nocc,
+                              fermi=i.fermi_energy, nfreq=200, npar=16, precision=prec)
+                   for i, b in data]
+        batched = [srv.wait(tk)[0] for tk in tickets]
+        st = srv.stats()
+        say(f"   batch {time.perf_counter() - t:.2f} s, peak concurrent {st['peak_running']}")
+    assert all(0.0 < r.timings["judge_err"] < 1e-7 and r.timings["fell_back"] == 0 for r in batched)
+    with hip.HipContext(0) as ctx:
+        ctx.set_judge(2)
+        ctx.set_batch_limit(64)
+        for (inp, b), r in zip(data, batched):
+            alone = _alone(hip, ctx, inp, b, 200, 16, prec)
+            assert np.array_equal(alone.sigma_c, r.sigma_c), inp.nmo
+            assert np.array_equal(alone.qp, r.qp, equal_nan=True)
+    for k in np.argsort([m[0] for m in mols])[:2]:
+        inp, b = data[k]
+        full = GwInput(mo_energy=inp.mo_energy, eri=eri_from_factor(b, inp.nmo), vxc=inp.vxc,
+                       nocc=inp.nocc, fermi_energy=inp.fermi_energy)
+        want = orc.run_g0w0(full, 200, 16)
+        e = rel_err(batched[k].sigma_c, want.sigma_c)
+        say(f"   nmo={inp.nmo}: batched int8 Sigma_c vs oracle rel {e:.2e}")
+        assert e < 1e-9
+
+
+def test_poisson_stream_alone_equals_mix(hip):
+    specs = serve_jobs(8, 4.0)
+    say(f"case: Poisson stream, 8 jobs at 4/s, nmo {[s[1] for s in specs]}, int8 form")
+    data = [factor_molecule(nmo, nocc, seed) for _, nmo, nocc, _, seed in specs]
+    prec = hip.PREC_MIXED_INT8
+    with hip.HipServer(0, workers=4, hbm_budget=200 << 30, batch_freqs=32) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for (ta, nmo, nocc, nfreq, _), (inp, b) in zip(specs, data):
+            delay = ta - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(factor=b, mo_energy=inp.mo_energy, vxc=inp.vxc, nocc=nocc,
+                                      fermi=inp.fermi_energy, nfreq=nfreq, npar=16,
+                                      precision=prec))
+        mixed = [srv.wait(tk) for tk in tickets]
+        st = srv.stats()
+    say(f"   stream {time.perf_counter() - t0:.2f} s, peak concurrent {st['peak_running']}, "
+        f"latencies {[round(m[1]) for m in mixed]} ms")
+    assert st["peak_running"] >= 2
+    with hip.HipContext(0) as ctx:
+        ctx.set_judge(2)
+        ctx.set_batch_limit(32)
+        for (_, nmo, _, nfreq, _), (inp, b), (r, lat, run) in zip(specs, data, mixed):
+            alone = _alone(hip, ctx, inp, b, nfreq, 16, prec)
+            assert np.array_equal(alone.sigma_c, r.sigma_c), nmo
+            assert np.array_equal(alone.qp, r.qp, equal_nan=True), nmo
+            assert lat >= run > 0

@@ -1,4 +1,6 @@
 """Request loop and device-side ERI construction (needs an MI355X)."""
+import time
+
 import numpy as np
 import pytest
 
@@ -192,3 +194,112 @@ def test_split_factorization_matches_one_stream(hip, monkeypatch):
                     assert res.timings["graph_batches"] > 0
                     assert np.array_equal(res.sigma_c, base[prec].sigma_c), (streams, prec, rep)
                     assert np.array_equal(res.qp, base[prec].qp, equal_nan=True)
+
+
+def test_error_judge_passes_and_forced_fallback_gives_fp64(hip):
+    """The judge of the int8 form measures its q error on sampled frequencies
+    (~1e-9, below its 1e-7 default) and leaves the result alone; a tolerance
+    no mixed form can meet forces the FP64 fallback, whose result is bit for
+    bit the FP64 form's.  Without fallback the failed judge is only reported."""
+    say("case: error judge and FP64 fallback, nmo=30, int8 form")
+    inp = synthetic_input(30, 6, 47)
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri(inp.eri, inp.nocc)
+        plain = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        f64 = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_FP64)
+        assert ctx.last_judge() == {"max_rel_err": -1.0, "samples": 0, "fell_back": 0}
+        ctx.set_judge(3)
+        judged = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        j = ctx.last_judge()
+        say(f"   judge: {j}")
+        assert j["samples"] == 3 and 0.0 < j["max_rel_err"] < 1e-7 and j["fell_back"] == 0
+        assert np.array_equal(judged.sigma_c, plain.sigma_c)
+        ctx.set_judge(3, tol=1e-30)
+        fb = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        assert ctx.last_judge()["fell_back"] == 1
+        assert np.array_equal(fb.sigma_c, f64.sigma_c) and np.array_equal(fb.qp, f64.qp, equal_nan=True)
+        ctx.set_judge(3, tol=1e-30, fallback=False)
+        kept = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        j = ctx.last_judge()
+        assert j["fell_back"] == 0 and j["max_rel_err"] > 1e-30
+        assert np.array_equal(kept.sigma_c, plain.sigma_c)
+        with pytest.raises(hip.HipError) as e:
+            ctx.set_judge(65)
+        assert e.value.code == hip.MGW_ERR_ARG
+
+
+def test_cancel_flag_stops_between_batches(hip):
+    """A stop flag set before the call (set from the host, the kernels all run
+    to completion) ends mgw_hip_g0w0 with MGW_ERR_STATE; cleared, the same
+    context runs the job normally."""
+    import ctypes
+    say("case: cancel flag, nmo=30, 8 frequencies per batch")
+    inp = synthetic_input(30, 6, 48)
+    flag = ctypes.c_int32(1)
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri(inp.eri, inp.nocc)
+        ctx.set_batch_limit(8)
+        ref = hip.run_g0w0(inp, 64, 16, ctx=ctx, upload=False)
+        assert ref.timings["freq_batch"] == 8
+        ctx.set_cancel_flag(flag)
+        with pytest.raises(hip.HipError) as e:
+            hip.run_g0w0(inp, 64, 16, ctx=ctx, upload=False)
+        assert e.value.code == hip.MGW_ERR_STATE and "cancelled" in str(e.value)
+        flag.value = 0
+        again = hip.run_g0w0(inp, 64, 16, ctx=ctx, upload=False)
+        assert np.array_equal(again.sigma_c, ref.sigma_c)
+        ctx.set_cancel_flag(None)
+
+
+def test_serve_faults_cancel_and_fallback(hip):
+    """Failure handling of the request loop, each condition forced from the
+    host: a job whose device budget is cut (fault-injection hook) fails with
+    MGW_ERR_MEMORY and releases its reservation, the next job runs; a pending
+    job is cancelled at once and a running one at its next frequency batch
+    (MGW_ERR_STATE), without disturbing the job beside it; a judge
+    tolerance no mixed form can meet makes the loop rerun a job in FP64."""
+    say("case: request loop faults: budget cut, cancellation, judge fallback")
+    small = _jobs()[:2]
+    kw = dict(nfreq=40, npar=12)
+    with hip.HipServer(0, workers=1, hbm_budget=64 << 30, batch_freqs=8) as srv:
+        srv.set_budget_scale(0.01)
+        j = small[0]
+        t = srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"], fermi=j["fermi"], **kw)
+        with pytest.raises(hip.HipError) as e:
+            srv.wait(t)
+        assert e.value.code == hip.MGW_ERR_MEMORY
+        assert srv.stats()["reserved_bytes"] == 0
+        srv.set_budget_scale(1.0)
+        ok, _, _ = srv.wait(srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"],
+                                       fermi=j["fermi"], **kw))
+        assert np.all(np.isfinite(ok.sigma_c))
+        # one worker: a long job runs while a second waits; cancel the waiting one,
+        # then the running one
+        mo, fermi, b, vxc = synthetic_factor(120, 24, 49)
+        t_long = srv.submit(factor=b, mo_energy=mo, vxc=vxc, nocc=24, fermi=fermi, nfreq=200,
+                            npar=16)
+        t_wait = srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"],
+                            fermi=j["fermi"], **kw)
+        srv.cancel(t_wait)
+        with pytest.raises(hip.HipError) as e:
+            srv.wait(t_wait)
+        assert e.value.code == hip.MGW_ERR_STATE and "before the job started" in str(e.value)
+        while srv.stats()["running"] == 0:
+            time.sleep(0.001)
+        srv.cancel(t_long)
+        with pytest.raises(hip.HipError) as e:
+            srv.wait(t_long)
+        assert e.value.code == hip.MGW_ERR_STATE and "cancelled" in str(e.value)
+        # judge fallback through the loop
+        srv.set_judge(2, tol=1e-30)
+        t = srv.submit(factor=j["b"], mo_energy=j["mo"], vxc=j["vxc"], nocc=j["nocc"],
+                       fermi=j["fermi"], precision=hip.PREC_MIXED_INT8, **kw)
+        fb, _, _ = srv.wait(t)
+        assert fb.timings["fell_back"] == 1
+        st = srv.stats()
+        assert st["running"] == 0 and st["reserved_bytes"] == 0
+    with hip.HipContext(0) as ctx:
+        ctx.upload_eri_factor(j["b"], j["nmo"], j["nocc"])
+        inp = GwInput(mo_energy=j["mo"], eri=None, vxc=j["vxc"], nocc=j["nocc"], fermi_energy=j["fermi"])
+        f64 = hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False)
+    assert np.array_equal(fb.sigma_c, f64.sigma_c)
