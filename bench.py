@@ -32,13 +32,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def contraction_flops(n: int, m: int, precision: int) -> float:
-    """Algorithmic flops of one frequency's contraction Z = (L^-1 S) X, L^-1
-    lower triangular: 2 * (n(n+1)/2) * m for the m contracted columns (the
-    unordered (p,k) pairs when q(p,k) = q(k,p)); three products in the FP16
-    split form, fifteen int8 slice products in the int8 form."""
-    f = float(n) * (n + 1) * m
-    return {1: 3.0, 2: 15.0}.get(precision, 1.0) * f
+def contraction_flops(n: int, m: int) -> float:
+    """Algorithmic flops of one frequency's contraction q_j = -||(L^-1 S) x_j||^2,
+    L^-1 lower triangular: 2 * (n(n+1)/2) * m for the m contracted columns
+    (the unordered (p,k) pairs when q(p,k) = q(k,p)).  One product whatever
+    the arithmetic: the split forms' extra slice products are not useful work."""
+    return float(n) * (n + 1) * m
+
+
+def contraction_bytes(n: int, m: int, nb: float, form: str) -> float:
+    """Algorithmic HBM bytes of one contraction launch over nb frequencies: the
+    operands once each in the form's representation (the triangle of M per
+    frequency, the panel X once per launch) and the per-row-tile partial sums
+    written (128-row tiles)."""
+    per = {"mixed_int8": 5.0, "mixed": 4.0, "fp64": 8.0, "fp64_cd": 8.0}[form]
+    return per * (n * (n + 1) / 2 * nb + n * m) + 8.0 * (n / 128) * m * nb
 
 
 HA_MEV = 27.211386245988 * 1e3
@@ -109,25 +117,69 @@ def h2o_check(hip, ctx, npar: int, prec: int) -> dict:
     return accuracy(hip, ctx, inp, mix, f64, npar)
 
 
-def cpu_baseline(inp, nfreq: int, sample_freqs: int) -> dict:
-    """Oracle (numpy + LAPACK/BLAS, all host cores) on `sample_freqs` of the
-    molecule's imaginary frequencies, scaled to QP solves/s for all nfreq."""
-    from threadpoolctl import threadpool_info
+def host_threads() -> int:
+    """Host threads the CPU baseline uses: the job's OpenMP share when set, else nproc."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_port(inp, nfreq: int, sample_freqs: int, threads: int) -> dict:
+    """The oracle (numpy + LAPACK/BLAS, `threads` host threads) on `sample_freqs`
+    of the molecule's imaginary frequencies in the hoisted schedule (each w'
+    once), scaled to QP solves/s for all nfreq."""
+    from threadpoolctl import threadpool_limits
     from oracle import oracle as orc
 
     ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
     omegas, _ = orc.transformed_legendre_grid(nfreq)
     pick = np.linspace(0, nfreq - 1, sample_freqs).round().astype(int)
-    t0 = time.perf_counter()
-    orc.quadratic_forms_all(inp.eri, ph, omegas[pick], list(range(inp.nmo)), 0.0)
-    dt = time.perf_counter() - t0
+    with threadpool_limits(threads):
+        t0 = time.perf_counter()
+        orc.quadratic_forms_all(inp.eri, ph, omegas[pick], list(range(inp.nmo)), 0.0)
+        dt = time.perf_counter() - t0
     per_freq = dt / sample_freqs
-    threads = max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
-    return {"value": inp.nmo / (per_freq * nfreq), "unit": "QP solves/s", "cores": int(threads),
-            "kind": "port",
+    return {"value": inp.nmo / (per_freq * nfreq), "unit": "QP solves/s", "cores": threads,
+            "kind": "port", "schedule": "hoisted (each w' once)",
             "sample": f"{sample_freqs} of {nfreq} frequencies (W_c by getrf/getrs + W_c X gemm, "
                       f"all {inp.nmo} states), {dt:.1f} s, scaled x{nfreq / sample_freqs:.0f}; "
                       f"Sigma_c/Pade/QP excluded (negligible)"}
+
+
+def cpu_baseline(config: str, inp, nfreq: int, sample_freqs: int) -> dict:
+    """The reference's own CPU path on the host cores: oracle/_ref/gw_ref_blas
+    (the reference's run_g0w0 with its BLAS/LAPACK backend, frequencies serial,
+    OpenMP kernel loops and LAPACK threads = the cores used) on the same
+    molecule at 1 frequency, i.e. one of the F^2 identical (w_n, w') units its
+    schedule repeats, extrapolated to F = nfreq exactly in operation count
+    (oracle/reference_timing.py).  value = the reference's own F^2 schedule;
+    the same units done once per w' (the hoisted schedule the device runs)
+    and the numpy port are reported beside it.  Falls back to the port when
+    the reference binary or its LAPACK provider is missing."""
+    from minigw_amd.synthetic import CONFIGS
+    from oracle import reference_timing as rt
+
+    threads = host_threads()
+    port = cpu_port(inp, nfreq, sample_freqs, threads)
+    if not rt.available():
+        port["note"] = "reference binary oracle/_ref/gw_ref_blas not built: the port instead"
+        return port
+    nmo, nocc, seed = CONFIGS[config]
+    try:
+        r = rt.run(nmo, nocc, seed, 1, threads, timeout=600)
+    except Exception as e:                      # noqa: BLE001 (reported, not hidden)
+        port["note"] = f"reference binary failed ({e}); the port instead"
+        return port
+    ex = rt.extrapolate(r, nfreq)
+    return {"value": ex["reference_qp_solves_per_s"], "unit": "QP solves/s", "cores": threads,
+            "kind": "reference", "schedule": "reference (W_c recomputed for every (w_n, w'): F^2 units)",
+            "sample": f"1 of the {nfreq}^2 (w_n, w') units of the reference's BLAS/LAPACK backend "
+                      f"on this molecule ({r['unit_s']:.1f} s: complex zgetrf + zgetrs against I, "
+                      f"zgemm panels of all {nmo} states), + V_ph setup {r['setup_s']:.2f} s, "
+                      f"extrapolated to {nfreq} frequencies ({ex['reference_schedule_s']:.3g} s)",
+            "unit_s": r["unit_s"],
+            "hoisted_value": ex["hoisted_qp_solves_per_s"],
+            "hoisted_note": "the reference's units done once per w' (F units), the device's schedule",
+            "port": port}
 
 
 def _factor_job(nmo: int, nocc: int, seed: int):
@@ -162,39 +214,42 @@ def run_batch(hip, count: int, workers: int, nfreq: int, npar: int, prec: int) -
             "peak_concurrent_jobs": st["peak_running"]}
 
 
-def run_serve(hip, jobs: int, rate: float, workers: int, npar: int, prec: int) -> dict:
-    """Poisson arrivals at `rate` jobs/s; nmo log-uniform in [24, 200],
-    nocc = round(nmo / 5), 100-200 frequencies; latency = arrival to result."""
-    import math
-    from minigw_amd.synthetic import SplitMix64, SERVE_SEED
-    rng = SplitMix64(SERVE_SEED)
-    u = rng.uniform(3 * jobs)
-    specs = []
-    t_arr = 0.0
-    for i in range(jobs):
-        t_arr += -math.log(1.0 - u[3 * i]) / rate
-        nmo = int(round(math.exp(math.log(24) + u[3 * i + 1] * (math.log(200) - math.log(24)))))
-        nfreq = 100 + int(u[3 * i + 2] * 101)
-        specs.append((t_arr, nmo, max(1, int(round(nmo / 5))), nfreq, SERVE_SEED + 1 + i))
-    data = [_factor_job(nmo, nocc, seed) for _, nmo, nocc, _, seed in specs]
-    with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=32) as srv:
-        t0 = time.perf_counter()
-        tickets = []
-        for (ta, nmo, nocc, nfreq, _), d in zip(specs, data):
-            delay = ta - (time.perf_counter() - t0)
-            if delay > 0:
-                time.sleep(delay)
-            tickets.append(srv.submit(factor=d["b"], mo_energy=d["mo"], vxc=d["vxc"], nocc=nocc,
-                                      fermi=d["fermi"], nfreq=nfreq, npar=npar, precision=prec))
-        lat = [srv.wait(t)[1] for t in tickets]
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-    solves = sum(s[1] for s in specs)
-    return {"jobs": jobs, "rate_per_s": rate, "qp_solves_per_s": round(solves / wall, 2),
-            "p50_ms": round(float(np.percentile(lat, 50)), 1),
-            "p99_ms": round(float(np.percentile(lat, 99)), 1), "wall_s": round(wall, 2),
-            "workers": workers, "peak_concurrent_jobs": st["peak_running"],
-            "nmo_range": [min(s[1] for s in specs), max(s[1] for s in specs)]}
+def run_serve(hip, jobs: int, rates, workers: int, npar: int, prec: int) -> list:
+    """Poisson job streams (minigw_amd.synthetic.serve_jobs: nmo log-uniform
+    in [24, 200], nocc = round(nmo / 5), 100-200 frequencies) at each rate of
+    `rates` jobs/s, the same jobs at every rate; latency = arrival to result.
+    Mixed jobs run under the loop's error judge with FP64 fallback."""
+    from minigw_amd.synthetic import serve_jobs
+    base = serve_jobs(jobs, 1.0)
+    data = [_factor_job(nmo, nocc, seed) for _, nmo, nocc, _, seed in base]
+    out = []
+    for rate in rates:
+        log(f"bench: serving {jobs} jobs at {rate} jobs/s ...")
+        with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=32) as srv:
+            t0 = time.perf_counter()
+            tickets = []
+            for (ta, nmo, nocc, nfreq, _), d in zip(base, data):
+                delay = ta / rate - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit(factor=d["b"], mo_energy=d["mo"], vxc=d["vxc"], nocc=nocc,
+                                          fermi=d["fermi"], nfreq=nfreq, npar=npar, precision=prec))
+            done = [srv.wait(t) for t in tickets]
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        lat = [d[1] for d in done]
+        jerr = [d[0].timings["judge_err"] for d in done]
+        solves = sum(s[1] for s in base)
+        out.append({"jobs": jobs, "rate_per_s": rate, "qp_solves_per_s": round(solves / wall, 2),
+                    "jobs_per_s": round(jobs / wall, 3),
+                    "p50_ms": round(float(np.percentile(lat, 50)), 1),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 1), "wall_s": round(wall, 2),
+                    "workers": workers, "peak_concurrent_jobs": st["peak_running"],
+                    "nmo_range": [min(s[1] for s in base), max(s[1] for s in base)],
+                    "judged_jobs": int(sum(e >= 0 for e in jerr)),
+                    "judge_max_rel_err": max(jerr), "fp64_fallbacks":
+                        int(sum(d[0].timings["fell_back"] != 0 for d in done))})
+    return out
 
 
 FORMS = {
@@ -213,8 +268,14 @@ RANDOM_PEAK = {"mixed_int8": "int8_mfma_32x32_random", "fp64": "fp64_mfma_random
 
 
 def roofline(form: str, ph: dict, n: int, m: int, nfreq: int, config: str, peaks: dict) -> dict:
-    """Contraction kernel: algorithmic flops (or int8 ops) per launch over its
-    average launch time (HIP events around the kernel on the context's stream)."""
+    """Contraction kernel: algorithmic flops per launch (one product,
+    n(n+1) m per frequency on the executed columns) over its average launch
+    time (HIP events around the kernel on the context's stream), against the
+    vendor dense peak of the instruction the form issues (int8 5.0 POP/s,
+    FP16 2.5 PFLOP/s, FP64 78.6 TFLOP/s).  mfma_pipe_frac counts every slice
+    product the form issues (15 int8, 3 FP16): how busy the MFMA pipe is, not
+    useful work.  traffic_over_algorithmic: measured HBM bytes per launch over
+    the operands' bytes read once."""
     code, kernel, products, peak, key, _ = FORMS[form]
     launches = max(ph["contract_launches"], 1.0)
     t_launch = ph["contract"] / launches * 1e-3
@@ -222,12 +283,17 @@ def roofline(form: str, ph: dict, n: int, m: int, nfreq: int, config: str, peaks
     # launches are nb frequencies each except the last: average work per launch;
     # the decomposed form contracts over the rank r of the ERI's pair matrix
     k = int(ph["cd_rank"]) if form == "fp64_cd" else n
-    work = contraction_flops(k, cols, 0) * products * nfreq / launches
-    achieved = work / t_launch / 1e12
-    return {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 2),
-            "peak": peak / 1e12, "unit": "TOP/s" if form == "mixed_int8" else "TFLOP/s",
-            "frac": round(achieved * 1e12 / peak, 4),
-            "traffic": TRAFFIC.get((form, config, nfreq)),
+    nb = nfreq / launches
+    work = contraction_flops(k, cols) * nb
+    achieved = work / t_launch
+    traffic = TRAFFIC.get((form, config, nfreq))
+    alg_bytes = contraction_bytes(k, cols, nb, form)
+    return {"bound": "mfma", "kernel": kernel, "achieved": round(achieved / 1e12, 2),
+            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": traffic, "algorithmic_bytes": alg_bytes,
+            "traffic_over_algorithmic": round(traffic / alg_bytes, 1) if traffic else None,
+            "mfma_pipe_frac": round(achieved * products / peak, 4),
+            "products_per_element": products,
             "launch_ms": round(t_launch * 1e3, 3), "work_per_launch": work,
             "measured_peak": round(peaks[key] / 1e12, 1),
             # the same MFMA on random operands (the rate real data allows: the
@@ -253,8 +319,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample-freqs", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-molecules", type=int, default=32)
-    ap.add_argument("--serve-jobs", type=int, default=12)
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[1.0])
+    ap.add_argument("--serve-jobs", type=int, default=50)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[1.0, 3.0, 10.0])
+    ap.add_argument("--decomposed-workloads", action="store_true",
+                    help="also run the batch and serving workloads in the decomposed-ERI form")
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--autotune", action="store_true")
     args = ap.parse_args()
@@ -375,7 +443,7 @@ def main() -> int:
                 "phases_ms": {k: round(v, 3) for k, v in cd_phase.items()},
                 "roofline": roofline("fp64_cd", cd_phase, n, m, args.nfreq, args.config, peaks)}
         out.update(report)
-    cd_too = prec != 3 and args.cd_steps > 0          # the same workloads in the decomposed form
+    cd_too = prec != 3 and args.decomposed_workloads  # the same workloads in the decomposed form
     if args.batch_molecules > 0:
         log("bench: uneven-nmo batch ...")
         out["batch"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq, args.npar, prec)
@@ -383,16 +451,13 @@ def main() -> int:
             out["batch_decomposed"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq,
                                                 args.npar, 3)
     if args.serve_jobs > 0:
-        out["serve"] = []
-        for rate in args.serve_rates:
-            log(f"bench: serving at {rate} jobs/s ...")
-            out["serve"].append(run_serve(hip, args.serve_jobs, rate, args.workers, args.npar, prec))
+        out["serve"] = run_serve(hip, args.serve_jobs, args.serve_rates, args.workers, args.npar, prec)
         if cd_too:
-            out["serve_decomposed"] = [run_serve(hip, args.serve_jobs, rate, args.workers, args.npar, 3)
-                                       for rate in args.serve_rates]
+            out["serve_decomposed"] = run_serve(hip, args.serve_jobs, args.serve_rates, args.workers,
+                                                args.npar, 3)
     if not args.no_cpu:
         log("bench: cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(inp, args.nfreq, args.cpu_sample_freqs)
+        out["cpu_baseline"] = cpu_baseline(args.config, inp, args.nfreq, args.cpu_sample_freqs)
     print(json.dumps(out), flush=True)
     return 0
 

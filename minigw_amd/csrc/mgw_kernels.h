@@ -58,11 +58,12 @@ hipError_t launch_dielectric(const double* v, const double* S, double* A, int np
 hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipStream_t s);
 
 // Single steps of the blocked factorization (block column k / j): the
-// diagonal block's factor and inverse, the panel solve below it, and the
-// inverse factor's column j from T (tbuf) -- shared by the FP64 and the
-// int8-update forms.
+// diagonal block's factor and inverse (with pre != 0 after the update
+// A_kk -= L_k,k-1 L_k,k-1^T), the panel solve below it, and the inverse
+// factor's column j from T (tbuf) -- shared by the FP64 and the int8-update
+// forms.
 hipError_t launch_potf2_block(double* A, double* dinv, int* info, int np, int k, int nb,
-                              hipStream_t s);
+                              hipStream_t s, int pre = 0);
 hipError_t launch_trsm_block(double* A, const double* dinv, int np, int k, int nb, hipStream_t s);
 hipError_t launch_trtri_block(double* A, const double* dinv, const double* tbuf, int np, int j,
                               int nb, hipStream_t s);

@@ -17,20 +17,12 @@
 //   b: B[k][j], j = lane % 16, k = lane / 16      (4 x 16 block)
 //   d[r]: D[i][j], i = lane / 16 + 4 * r, j = lane % 16   (checked on the device)
 #include "mgw_kernels.h"
+#include "mgw_tile64.h"
 
 #include <hip/hip_runtime.h>
 
 namespace mgw {
 namespace {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ inline d4 mfma_f64(double a, double b, d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-constexpr int kBK = 16;          // K slab staged through LDS per step
-constexpr int kLdsPad = 4;       // doubles of padding per LDS row
 
 // ----------------------------------------------------------------- simple maps
 __global__ void k_build_v(const double* __restrict__ eri, const int64_t* __restrict__ rowbase,
@@ -89,107 +81,6 @@ __global__ void k_dielectric(const double* __restrict__ v, const double* __restr
     }
 }
 
-// ------------------------------------------------------- 64 x 64 MFMA tile core
-// LDS slabs: As[k][m], Bs[k][n] for k < kBK.  256 threads = 4 waves in 2 x 2,
-// each wave a 32 x 32 sub-tile (2 x 2 MFMA blocks).
-struct Tile64 {
-    d4 acc[2][2];
-    __device__ void zero() {
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    }
-    __device__ void step(const double (*As)[64 + kLdsPad], const double (*Bs)[64 + kLdsPad]) {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-        const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-        for (int kk = 0; kk < kBK; kk += 4) {
-            const double a0 = As[kk + lk][wm + li], a1 = As[kk + lk][wm + 16 + li];
-            const double b0 = Bs[kk + lk][wn + li], b1 = Bs[kk + lk][wn + 16 + li];
-            acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-            acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-            acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-            acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
-        }
-    }
-    // visit(row, col, value) over this thread's 16 results
-    template <class F>
-    __device__ void for_each(F f) const {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-        for (int bi = 0; bi < 2; ++bi)
-            for (int bj = 0; bj < 2; ++bj)
-                for (int r = 0; r < 4; ++r)
-                    f(wm + bi * 16 + (lane >> 4) + 4 * r, wn + bj * 16 + (lane & 15), acc[bi][bj][r]);
-    }
-};
-
-// As[k][m] = src[(row0 + m) * ld + k0 + k], m < 64, k < kBK (row-major operand, K along a row)
-__device__ inline void load_rows_kmajor(double (*As)[64 + kLdsPad], const double* __restrict__ src,
-                                        size_t ld, int row0, int k0) {
-    const int t = threadIdx.x, m = t >> 2, kq = (t & 3) * 4;
-    const double* p = src + (size_t)(row0 + m) * ld + k0 + kq;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) As[kq + u][m] = p[u];
-}
-
-// Bs[k][n] = src[(k0 + k) * ld + col0 + n], k < kBK, n < 64 (row-major operand, K down the rows)
-__device__ inline void load_kn(double (*Bs)[64 + kLdsPad], const double* __restrict__ src,
-                               size_t ld, int k0, int col0) {
-    const int t = threadIdx.x, k = t >> 4, nq = (t & 15) * 4;
-    const double* p = src + (size_t)(k0 + k) * ld + col0 + nq;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) Bs[k][nq + u] = p[u];
-}
-
-// acc += A[0:64, kbeg:kend] * op(B) with A row-major (lda), op(B) = B^T for a
-// row-major 64-row B (NT) or B[kbeg:kend, 0:64] row-major (NN).  The next K
-// slab is fetched into registers while the current one is multiplied.
-template <bool NN>
-__device__ inline void gemm64(const double* __restrict__ a, size_t lda,
-                              const double* __restrict__ b, size_t ldb, int kbeg, int kend,
-                              Tile64& tile, double (*As)[64 + kLdsPad],
-                              double (*Bs)[64 + kLdsPad]) {
-    const int t = threadIdx.x;
-    const int am = t >> 2, akq = (t & 3) * 4;
-    const int bk = t >> 4, bnq = (t & 15) * 4;
-    double2 ra[2], rb[2];
-    auto fetch = [&](int k0) {
-        const double2* pa = reinterpret_cast<const double2*>(a + (size_t)am * lda + k0 + akq);
-        ra[0] = pa[0];
-        ra[1] = pa[1];
-        const double2* pb =
-            NN ? reinterpret_cast<const double2*>(b + (size_t)(k0 + bk) * ldb + bnq)
-               : reinterpret_cast<const double2*>(b + (size_t)am * ldb + k0 + akq);
-        rb[0] = pb[0];
-        rb[1] = pb[1];
-    };
-    if (kbeg >= kend) return;
-    fetch(kbeg);
-    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-        __syncthreads();
-        As[akq + 0][am] = ra[0].x;
-        As[akq + 1][am] = ra[0].y;
-        As[akq + 2][am] = ra[1].x;
-        As[akq + 3][am] = ra[1].y;
-        if (NN) {
-            Bs[bk][bnq + 0] = rb[0].x;
-            Bs[bk][bnq + 1] = rb[0].y;
-            Bs[bk][bnq + 2] = rb[1].x;
-            Bs[bk][bnq + 3] = rb[1].y;
-        } else {
-            Bs[akq + 0][am] = rb[0].x;
-            Bs[akq + 1][am] = rb[0].y;
-            Bs[akq + 2][am] = rb[1].x;
-            Bs[akq + 3][am] = rb[1].y;
-        }
-        __syncthreads();
-        if (k0 + kBK < kend) fetch(k0 + kBK);
-        tile.step(As, Bs);
-    }
-    __syncthreads();
-}
-
 // ------------------------------------------------------------------ potrf pieces
 // Diagonal block k of every matrix: Cholesky and the inverse of its factor,
 // held in registers: thread t owns row i = t / 4, columns c0 = 16 (t % 4) ..
@@ -201,21 +92,35 @@ __device__ inline void gemm64(const double* __restrict__ a, size_t lda,
 // B[i][l] -= (T[i][j] / p_j) B[j][l] (l <= j), as masked FMAs on vector-loaded
 // operands (no branches).  Finally L[i][c] = T[i][c] / sqrt(p_c) (c <= i) and
 // L^-1[i][c] = B[i][c] / sqrt(p_i).
+// pre != 0: the block first takes the product of block column k - 1 that the
+// calling sweep left out, A_kk -= L_k,k-1 L_k,k-1^T (FP64 MFMA tile, staged
+// through LDS).
 __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* __restrict__ dinv,
-                                               int* __restrict__ info, int np, int k) {
+                                               int* __restrict__ info, int np, int k, int pre) {
     __shared__ double col[2][64], row[2][64];
     __shared__ double rs[64];                    // 1 / sqrt(p_j)
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    __shared__ double upd[64][65];
     const int w = blockIdx.x, t = threadIdx.x;
     const int i = t >> 2, c0 = (t & 3) * 16;
     const int nblk = np / kNB;
     double* Aw = A + (size_t)w * np * np;
     const size_t off = (size_t)k * kNB * np + (size_t)k * kNB;
+    if (pre) {
+        Tile64 tile;
+        tile.zero();
+        const double* Lk = Aw + (size_t)k * kNB * np;
+        gemm64<false>(Lk, np, Lk, np, (k - 1) * kNB, k * kNB, tile, As, Bs);
+        tile.for_each([&](int r, int c, double v) { upd[r][c] = v; });
+        __syncthreads();
+    }
     double T[16], B[16];
     {
         const double* src = Aw + off + (size_t)i * np + c0;
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-            T[c] = (c0 + c <= i) ? src[c] : 0.0;
+            T[c] = (c0 + c <= i) ? (pre ? src[c] - upd[i][c0 + c] : src[c]) : 0.0;
             B[c] = (c0 + c == i) ? 1.0 : 0.0;
         }
     }
@@ -907,7 +812,7 @@ hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipS
     for (int k = 0; k < nblk; ++k) {
         if (k > 0)
             hipLaunchKernelGGL(k_potrf_update, dim3(nblk - k, nb), dim3(256), 0, s, A, np, k);
-        hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k);
+        hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k, 0);
         const int t = nblk - k - 1;
         if (t > 0) hipLaunchKernelGGL(k_trsm_panel, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k);
     }
@@ -915,8 +820,8 @@ hipError_t launch_potrf(double* A, double* dinv, int* info, int np, int nb, hipS
 }
 
 hipError_t launch_potf2_block(double* A, double* dinv, int* info, int np, int k, int nb,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k);
+                              hipStream_t s, int pre) {
+    hipLaunchKernelGGL(k_potf2, dim3(nb), dim3(256), 0, s, A, dinv, info, np, k, pre);
     return hipGetLastError();
 }
 

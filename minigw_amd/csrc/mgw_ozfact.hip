@@ -8,11 +8,19 @@
 // K block runs as the slice products with i + j < kSlices on
 // v_mfma_i32_32x32x32_i8 (exact in int32), and the K blocks, whose scales
 // differ, are combined in FP64.  The diagonal blocks (factor and inverse in
-// LDS), the panel solves and M_ij = -T_i L_jj^-1 stay FP64 (mgw_kernels.hip).
+// LDS), the panel solves and M_ij = -T_i L_jj^-1 stay FP64.
 // The truncation error per block product is ~2^-35 of the largest element of
 // each row segment; the contraction that follows carries the same order.
+//
+// The sweeps take two block columns per launch; the one block product of the
+// second column that the launch cannot include (kb = k for column k + 1 of
+// the Cholesky, kb = j for column j - 1 of the inverse) runs in FP64 inside
+// the next column's panel kernel, which also cuts its own output blocks into
+// slices: per block column one panel launch instead of a product, a solve and
+// a slicing launch.
 #include "mgw_kernels.h"
 #include "mgw_slices.h"
+#include "mgw_tile64.h"
 
 #include <hip/hip_runtime.h>
 
@@ -27,41 +35,33 @@ __device__ inline size_t blk_index(int w, int nblk, int bi, int bj) {
     return ((size_t)w * nblk + bi) * nblk + bj;
 }
 
-// One workgroup per 64 x 64 block (bi = b0 + blockIdx.x, bj) of A_w: row-scaled
-// slices into RP/RS (when RP != NULL) and, transposed, column-scaled slices
-// into CP/CS (when CP != NULL), both at block (bi, bj).  RS / CS hold, as
-// int32, the exponents e of the inverse scales 2^e (the factor that restores
-// a row from its slices).
-__global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__ A, int np, int b0,
-                                                      int bj, int8_t* __restrict__ RP,
-                                                      double* __restrict__ RS,
-                                                      int8_t* __restrict__ CP,
-                                                      double* __restrict__ CS) {
-    __shared__ double blk[64][65];
-    __shared__ double rsc[64], csc[64];
-    const int w = blockIdx.y, t = threadIdx.x, nblk = np / 64, bi = b0 + blockIdx.x;
-    const double* src = A + (size_t)w * np * np + (size_t)bi * 64 * np + (size_t)bj * 64;
-    for (int e = t; e < 4096; e += 256) blk[e >> 6][e & 63] = src[(size_t)(e >> 6) * np + (e & 63)];
-    __syncthreads();
+// Slices of the 64 x 64 block held in LDS (blk) at block index bix: row-scaled
+// into RP/RS (when RP != NULL) and, transposed, column-scaled into CP/CS
+// (when CP != NULL).  RS / CS hold, as int32, the exponents e of the inverse
+// scales 2^e (the factor that restores a row from its slices).  256 threads;
+// blk is complete (barrier before the call).
+__device__ void slice_block(const double (*blk)[66], double* rsc, double* csc, size_t bix,
+                            int8_t* __restrict__ RP, double* __restrict__ RS,
+                            int8_t* __restrict__ CP, double* __restrict__ CS) {
+    const int t = threadIdx.x;
     if (t < 64) {
         double m = 0.0;
         for (int c = 0; c < 64; ++c) m = fmax(m, fabs(blk[t][c]));
         rsc[t] = slice_scale(m);
-    } else if (t < 128) {
+    } else if (t < 128 && CP) {
         double m = 0.0;
         for (int r = 0; r < 64; ++r) m = fmax(m, fabs(blk[r][t - 64]));
         csc[t - 64] = slice_scale(m);
     }
     __syncthreads();
-    const size_t bix = blk_index(w, nblk, bi, bj);
     if (RP) {
         int8_t* out = RP + bix * kBlkBytes;
         for (int e = t; e < 4096; e += 256) {
             const int r = e >> 6, c = e & 63;
-            int8_t s[kS];
-            slice_value(blk[r][c] * rsc[r], s);
+            int8_t sv[kS];
+            slice_value(blk[r][c] * rsc[r], sv);
 #pragma unroll
-            for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = s[i];
+            for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = sv[i];
         }
         if (t < 64) reinterpret_cast<int*>(RS)[bix * 64 + t] = ilogb(1.0 / rsc[t]);   // e of 2^e
     }
@@ -69,13 +69,103 @@ __global__ __launch_bounds__(256) void k_slice_blocks(const double* __restrict__
         int8_t* out = CP + bix * kBlkBytes;
         for (int e = t; e < 4096; e += 256) {
             const int c = e >> 6, r = e & 63;       // transposed: row c of the output
-            int8_t s[kS];
-            slice_value(blk[r][c] * csc[c], s);
+            int8_t sv[kS];
+            slice_value(blk[r][c] * csc[c], sv);
 #pragma unroll
-            for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = s[i];
+            for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = sv[i];
         }
         if (t < 64) reinterpret_cast<int*>(CS)[bix * 64 + t] = ilogb(1.0 / csc[t]);
     }
+}
+
+// Panel of Cholesky block column k, block i = k + 1 + blockIdx.x of A_w:
+// with pre != 0 first A_ik -= L_i,k-1 L_k,k-1^T (FP64, the product the
+// two-column sweep left out), then L_ik = A_ik Dinv_k^T written back and cut
+// into row slices (RP/RS) and transposed column slices (CP/CS).  Only block
+// column k is written; column k - 1 is final.
+__global__ __launch_bounds__(256) void k_trsm_slice(double* __restrict__ A,
+                                                    const double* __restrict__ dinv, int np, int k,
+                                                    int pre, int8_t* __restrict__ RP,
+                                                    double* __restrict__ RS,
+                                                    int8_t* __restrict__ CP,
+                                                    double* __restrict__ CS) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    __shared__ __attribute__((aligned(16))) double blk[64][66];
+    __shared__ double rsc[64], csc[64];
+    const int w = blockIdx.y, nblk = np / kNB, i = k + 1 + blockIdx.x;
+    double* Aw = A + (size_t)w * np * np;
+    double* Aik = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
+    Tile64 tile;
+    tile.zero();
+    if (pre)
+        gemm64<false>(Aw + (size_t)i * kNB * np, np, Aw + (size_t)k * kNB * np, np, (k - 1) * kNB,
+                      k * kNB, tile, As, Bs);
+    tile.for_each([&](int r, int c, double v) { blk[r][c] = Aik[(size_t)r * np + c] - v; });
+    __syncthreads();
+    tile.zero();
+    gemm64<false>(&blk[0][0], 66, dinv + ((size_t)w * nblk + k) * 4096, 64, 0, kNB, tile, As, Bs);
+    tile.for_each([&](int r, int c, double v) {
+        Aik[(size_t)r * np + c] = v;
+        blk[r][c] = v;
+    });
+    __syncthreads();
+    slice_block(blk, rsc, csc, blk_index(w, nblk, i, k), RP, RS, CP, CS);
+}
+
+// Column j of the inverse factor M = L^-1, block i = j + blockIdx.x of A_w:
+// M_jj = Dinv_j, M_ij = -(T_i + [pre] M_i,j+1 L_j+1,j) Dinv_j for i > j,
+// written back and, with slice != 0, cut into row slices (RP/RS).  T_i is
+// block i of tbuf (this column's slot); the FP64 term is the product the
+// two-column sweep left out, its L_j+1,j read from the slot's block j (T is
+// only defined below the diagonal), where the previous column's launch saved
+// it (save != 0: block (j, j-1) of L into block j-1 of slot `next`) before
+// this column's blocks overwrote it.
+__global__ __launch_bounds__(256) void k_trtri_m_slice(double* __restrict__ A,
+                                                       const double* __restrict__ dinv,
+                                                       const double* __restrict__ tbuf,
+                                                       double* __restrict__ next, int np, int j,
+                                                       int pre, int save, int slice,
+                                                       int8_t* __restrict__ RP,
+                                                       double* __restrict__ RS) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    __shared__ __attribute__((aligned(16))) double blk[64][66];
+    __shared__ double rsc[64];
+    const int w = blockIdx.y, nblk = np / kNB, i = j + blockIdx.x, t = threadIdx.x;
+    double* Aw = A + (size_t)w * np * np;
+    const double* D = dinv + ((size_t)w * nblk + j) * 4096;
+    double* Aij = Aw + (size_t)i * kNB * np + (size_t)j * kNB;
+    const double* Tw = tbuf + (size_t)w * np * kNB;
+    if (blockIdx.x == 0) {
+        if (save) {                 // L_j,j-1 for the next column's launch
+            const double* src = Aw + (size_t)j * kNB * np + (size_t)(j - 1) * kNB;
+            double* dst = next + (size_t)w * np * kNB + (size_t)(j - 1) * 4096;
+            for (int e = t; e < 4096; e += 256) dst[e] = src[(size_t)(e >> 6) * np + (e & 63)];
+        }
+        for (int e = t; e < 4096; e += 256) {
+            const double v = D[e];
+            Aij[(size_t)(e >> 6) * np + (e & 63)] = v;
+            blk[e >> 6][e & 63] = v;
+        }
+    } else {
+        const double* T = Tw + (size_t)i * 4096;
+        Tile64 tile;
+        tile.zero();
+        if (pre)      // M_i,j+1 (row block i, column block j+1) times the saved L_j+1,j
+            gemm64<true>(Aw + (size_t)i * kNB * np + (size_t)(j + 1) * kNB, np,
+                         Tw + (size_t)j * 4096, 64, 0, kNB, tile, As, Bs);
+        tile.for_each([&](int r, int c, double v) { blk[r][c] = T[r * 64 + c] + v; });
+        __syncthreads();
+        tile.zero();
+        gemm64<true>(&blk[0][0], 66, D, 64, 0, kNB, tile, As, Bs);
+        tile.for_each([&](int r, int c, double v) {
+            Aij[(size_t)r * np + c] = -v;
+            blk[r][c] = -v;
+        });
+    }
+    __syncthreads();
+    if (slice) slice_block(blk, rsc, nullptr, blk_index(w, nblk, i, j), RP, RS, nullptr, nullptr);
 }
 
 // LDS byte offset of 16 B chunk `ch` (0..3) of row `row` in a 64 x 64 byte
@@ -307,20 +397,20 @@ size_t fact_slice_bytes(int np) {
 
 // Left-looking blocked Cholesky, two block columns per sweep: the update of
 // columns k and k+1 over the finished columns < k in one launch (they share
-// the L_i,kb operands), then column k factored, solved and sliced, then column
-// k+1's one remaining block product (kb = k), then column k+1.
+// the L_i,kb operands), then column k factored and its panel solved and
+// sliced, then column k+1 factored and solved with its remaining product
+// (kb = k) taken first in FP64.
 hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
                            double* CS, int np, int nb, hipStream_t s) {
     if (np % 64 != 0) return hipErrorInvalidValue;
     const int nblk = np / 64;
-    auto finish = [&](int k) -> hipError_t {
-        if (hipError_t e = launch_potf2_block(A, dinv, info, np, k, nb, s); e != hipSuccess) return e;
+    auto finish = [&](int k, int pre) -> hipError_t {
+        if (hipError_t e = launch_potf2_block(A, dinv, info, np, k, nb, s, pre); e != hipSuccess)
+            return e;
         const int t = nblk - k - 1;
-        if (t > 0) {
-            if (hipError_t e = launch_trsm_block(A, dinv, np, k, nb, s); e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_slice_blocks, dim3(t, nb), dim3(256), 0, s, A, np, k + 1, k, RP, RS,
-                               CP, CS);
-        }
+        if (t > 0)
+            hipLaunchKernelGGL(k_trsm_slice, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k, pre, RP,
+                               RS, CP, CS);
         return hipGetLastError();
     };
     for (int k = 0; k < nblk; k += 2) {
@@ -331,32 +421,29 @@ hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, doubl
             else
                 gemm_i8blk<0, 1>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
         }
-        if (hipError_t e = finish(k); e != hipSuccess) return e;
-        if (two) {
-            gemm_i8blk<0, 1>(RP, RS, CP, CS, A, nullptr, np, k + 1, k + 1, nblk - k - 1, k, k + 1, 0,
-                             nb, s);
-            if (hipError_t e = finish(k + 1); e != hipSuccess) return e;
-        }
+        if (hipError_t e = finish(k, 0); e != hipSuccess) return e;
+        if (two)
+            if (hipError_t e = finish(k + 1, 1); e != hipSuccess) return e;
     }
     return hipGetLastError();
 }
 
 // Inverse sweep, two block columns per step: T for columns j and j-1 over the
 // inverted columns > j in one launch (shared M_i,kb operands), column j
-// inverted and sliced, then column j-1's remaining product (kb = j) added and
-// column j-1 inverted.  tbuf holds two slots of nb x np x 64 (column parity).
+// inverted and sliced (saving L_j,j-1), then column j-1 inverted with its
+// remaining product (kb = j) added first in FP64.  tbuf holds two slots of
+// nb x np x 64 (column parity).
 hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
                            const int8_t* CP, const double* CS, int np, int nb, hipStream_t s) {
     if (np % 64 != 0) return hipErrorInvalidValue;
     const int nblk = np / 64;
     const size_t slot = (size_t)nb * np * 64;
-    auto finish = [&](int j) -> hipError_t {
-        if (hipError_t e = launch_trtri_block(A, dinv, tbuf + (size_t)(j & 1) * slot, np, j, nb, s);
-            e != hipSuccess)
-            return e;
-        if (j > 0)      // row slices of the new inverse blocks (i, j), i >= j
-            hipLaunchKernelGGL(k_slice_blocks, dim3(nblk - j, nb), dim3(256), 0, s, A, np, j, j, RP,
-                               RS, nullptr, nullptr);
+    auto column = [&](int j, int pre, int save) -> hipError_t {
+        double* tj = tbuf + (size_t)(j & 1) * slot;
+        double* tn = tbuf + (size_t)((j + 1) & 1) * slot;     // slot of column j - 1
+        // column 0's blocks are never an operand of a later product: no slices
+        hipLaunchKernelGGL(k_trtri_m_slice, dim3(nblk - j, nb), dim3(256), 0, s, A, dinv, tj, tn, np,
+                           j, pre, save, j > 0 ? 1 : 0, RP, RS);
         return hipGetLastError();
     };
     for (int j = nblk - 1; j >= 0; j -= 2) {
@@ -366,11 +453,9 @@ hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* 
         else
             gemm_i8blk<1, 1>(RP, RS, CP, CS, A, tbuf, np, j, j + 1, nblk - 1 - j, j + 1, nblk, 0, nb,
                              s);
-        if (hipError_t e = finish(j); e != hipSuccess) return e;
-        if (two) {
-            gemm_i8blk<1, 1>(RP, RS, CP, CS, A, tbuf, np, j - 1, j, nblk - j, j, j + 1, 1, nb, s);
-            if (hipError_t e = finish(j - 1); e != hipSuccess) return e;
-        }
+        if (hipError_t e = column(j, 0, two ? 1 : 0); e != hipSuccess) return e;
+        if (two)
+            if (hipError_t e = column(j - 1, 1, 0); e != hipSuccess) return e;
     }
     return hipGetLastError();
 }

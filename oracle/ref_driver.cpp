@@ -21,9 +21,13 @@
 //
 // Input file (little endian):
 //   int64 nmo, nocc, nfreq, npar, state (-1 = all states, else 0-based),
-//         threads (OpenMP frequency parallelism if > 1)
+//         threads (OpenMP frequency parallelism if > 1), kdim
 //   double fermi_energy, eta
-//   double mo_energy[nmo], vxc[nmo*nmo], eri[nmo^4]  (C order)
+//   double mo_energy[nmo], vxc[nmo*nmo], then
+//     kdim == 0: eri[nmo^4]  (C order)
+//     kdim > 0:  B[nmo^2 * kdim] (row-major), eri[pq][rs] = sum_K B[pq][K] B[rs][K]
+//                formed here (the seeded synthetic molecules ship as their
+//                factor: 192 MB instead of a 12.8 GB ERI at nmo = 200)
 // Output file:
 //   int64 nmo, nfreq
 //   double omega[nfreq], weight[nfreq], sigma_x_diag[nmo],
@@ -41,6 +45,13 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#ifdef GW_HAS_BLAS_LAPACK_BACKEND
+extern "C" void dgemm_(const char* transa, const char* transb, const int* m, const int* n,
+                       const int* k, const double* alpha, const double* a, const int* lda,
+                       const double* b, const int* ldb, const double* beta, double* c,
+                       const int* ldc);
+#endif
 
 namespace {
 
@@ -65,8 +76,8 @@ int main(int argc, char** argv) {
     try {
         std::ifstream in(argv[1], std::ios::binary);
         if (!in) throw std::runtime_error("cannot open input");
-        std::int64_t hdr[6];
-        read_into(in, hdr, 6);
+        std::int64_t hdr[7];
+        read_into(in, hdr, 7);
         double fe[2];
         read_into(in, fe, 2);
         const std::size_t nmo = static_cast<std::size_t>(hdr[0]);
@@ -79,7 +90,31 @@ int main(int argc, char** argv) {
         read_into(in, vxc.data(), vxc.size());
         input.vxc_mo = gw::MatrixReal(nmo, nmo, std::move(vxc));
         std::vector<double> eri(nmo * nmo * nmo * nmo);
-        read_into(in, eri.data(), eri.size());
+        if (hdr[6] == 0) {
+            read_into(in, eri.data(), eri.size());
+        } else {
+            const std::size_t n2 = nmo * nmo, kd = static_cast<std::size_t>(hdr[6]);
+            std::vector<double> b(n2 * kd);
+            read_into(in, b.data(), b.size());
+#ifdef GW_HAS_BLAS_LAPACK_BACKEND
+            // the product on the linked BLAS (column-major view: B is kd x n2)
+            const int ni = static_cast<int>(n2), ki = static_cast<int>(kd);
+            const double one = 1.0, zero = 0.0;
+            dgemm_("T", "N", &ni, &ni, &ki, &one, b.data(), &ki, b.data(), &ki, &zero, eri.data(), &ni);
+#else
+#pragma omp parallel for schedule(dynamic, 16)
+            for (std::size_t r = 0; r < n2; ++r) {
+                const double* br = b.data() + r * kd;
+                for (std::size_t c = r; c < n2; ++c) {
+                    const double* bc = b.data() + c * kd;
+                    double acc = 0.0;
+                    for (std::size_t k = 0; k < kd; ++k) acc += br[k] * bc[k];
+                    eri[r * n2 + c] = acc;
+                    eri[c * n2 + r] = acc;
+                }
+            }
+#endif
+        }
         input.eri_mo = gw::Tensor4Real(nmo, nmo, nmo, nmo, std::move(eri));
 
         gw::GwSettings settings;

@@ -116,9 +116,9 @@ def write_h2o_fixtures() -> None:
 
 def write_reference_input(path: str, inp: GwInput, nfreq: int, npar: int, state: int,
                           eta: float, threads: int) -> None:
-    """The raw input file oracle/ref_driver.cpp reads."""
+    """The raw input file oracle/ref_driver.cpp reads (full ERI, kdim = 0)."""
     with open(path, "wb") as fh:
-        np.array([inp.nmo, inp.nocc, nfreq, npar, state, threads], dtype="<i8").tofile(fh)
+        np.array([inp.nmo, inp.nocc, nfreq, npar, state, threads, 0], dtype="<i8").tofile(fh)
         np.array([inp.fermi_energy, eta], dtype="<f8").tofile(fh)
         np.ascontiguousarray(inp.mo_energy, dtype="<f8").tofile(fh)
         np.ascontiguousarray(inp.vxc, dtype="<f8").tofile(fh)
