@@ -156,6 +156,21 @@ int mgw_hip_build_vph_from_eri(mgw_hip_ctx* ctx);
 int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width,
                             double* q_re, double* q_im);
 
+/* mgw_hip_quadratic_forms without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * panel and every intermediate live in a fixed device scratch array of the
+ * library (about 16.2 MiB per device, held until the process ends), which
+ * limits width to 256 and the padded nph (nph rounded up to 256) to 8192;
+ * larger requests fail with MGW_ERR_ARG.  q_dev: width doubles in device
+ * memory, written with the real part of q (eta == 0 only; a complex W_c is
+ * MGW_ERR_UNSUPPORTED).  Needs the resident ERI (mgw_hip_upload_eri) and
+ * W_c (mgw_hip_compute_wc), returns after the results are written.  Every
+ * call on one device shares that device's scratch: calls that overlap on one
+ * device (from two contexts or threads) race. */
+int mgw_hip_quadratic_forms_noalloc(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width,
+                                    double* q_dev);
+
 /* ---- hoisted G0W0 on the resident ERI (compute_sigma_c_with_backend's
  *      device branch, src/gw/sigma_c_driver.cpp:471-514, and run_g0w0,
  *      src/gw/gw.cpp:125-191) */

@@ -48,6 +48,35 @@ void hip_check(hipError_t e, const char* what) {
 
 }  // namespace
 
+// ------------------------------------------------------- fixed device scratch
+// Scratch of mgw_hip_quadratic_forms_noalloc: the panel's ERI row offsets, the
+// packed panel X (np x mpad doubles), the contraction's per-row-tile partial
+// sums and the batch offset, for np <= kNoAllocMaxNph and width <= 256.  Each
+// device holds one copy for the life of the process; host code uses only the
+// device address that hipGetSymbolAddress returned for it (noalloc_base).
+constexpr int kNoAllocMaxNph = 8192, kNoAllocMaxWidth = 256;
+constexpr size_t kNoAllocPk = (size_t)kNoAllocMaxWidth * 8;
+constexpr size_t kNoAllocX = (size_t)kNoAllocMaxNph * kNoAllocMaxWidth * 8;
+constexpr size_t kNoAllocQpart = (size_t)(kNoAllocMaxNph / mgw::kCT) * kNoAllocMaxWidth * 8;
+constexpr size_t kNoAllocBytes = kNoAllocPk + kNoAllocX + kNoAllocQpart + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch;        // device -> address of its scratch
+
+// The scratch array's address on the current device (looked up once per device).
+void* noalloc_base(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) return it->second;
+    void* ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = ptr;
+    return ptr;
+}
+}  // namespace
+
 // ------------------------------------------------------------------ context
 // Everything a captured frequency batch bakes into its kernel arguments: the
 // true particle-hole count n (rows past it get S = 0), the ph padded order
@@ -573,6 +602,7 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         need(device >= 0 && device < count, "mgw_hip_create: device %d out of range (%d visible)",
              device, count);
         hip_check(hipSetDevice(device), "hipSetDevice");
+        (void)noalloc_base(device);          // the device's scratch address, once per device
         auto* ctx = new mgw_hip_ctx();
         ctx->device = device;
         if (const char* env = std::getenv("MGW_INT8_FACTOR")) ctx->int8_factor = env[0] != '0';
@@ -968,6 +998,49 @@ int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t wid
         for (int c = 0; c < width; ++c) pk[c] = ((int64_t)p * nmo + k0 + c) * n2;
         contract_single(ctx, as<double>(ctx->eri.p), pk.data(), width, L.colidx.data(), q_re,
                         q_im);
+    });
+}
+
+int mgw_hip_quadratic_forms_noalloc(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width,
+                                    double* q_dev) {
+    return guarded(ctx, [&] {
+        need(ctx && q_dev, "mgw_hip_quadratic_forms_noalloc: NULL argument");
+        if (!ctx->nmo) fail(MGW_ERR_STATE, "fill_panel before the ERI was uploaded");
+        if (!ctx->wc_ready) fail(MGW_ERR_STATE, "quadratic_forms_panel before compute_w_c");
+        if (ctx->wc_complex)
+            fail(MGW_ERR_UNSUPPORTED,
+                 "mgw_hip_quadratic_forms_noalloc: complex W_c (eta != 0) is not supported");
+        const int nmo = ctx->nmo, np = ctx->np;
+        need(p >= 0 && p < nmo, "DevicePqPhPanelView::fill_panel: p index out of range");
+        need(k0 >= 0 && width >= 0 && k0 <= nmo && width <= nmo - k0,
+             "DevicePqPhPanelView::fill_panel: k panel exceeds nmo");
+        need(ctx->n == ctx->nocc * (nmo - ctx->nocc),
+             "quadratic forms: V_ph dimension %d does not match the ERI", ctx->n);
+        need(width <= kNoAllocMaxWidth,
+             "mgw_hip_quadratic_forms_noalloc: width %d exceeds the scratch limit %d", width,
+             kNoAllocMaxWidth);
+        need(np <= kNoAllocMaxNph,
+             "mgw_hip_quadratic_forms_noalloc: padded nph %d exceeds the scratch limit %d", np,
+             kNoAllocMaxNph);
+        if (width == 0) return;
+        const int mpad = mgw::round_up(width, mgw::kPad);
+        char* base = static_cast<char*>(noalloc_base(ctx->device));
+        int64_t* dpk = reinterpret_cast<int64_t*>(base);
+        double* xp = reinterpret_cast<double*>(base + kNoAllocPk);
+        double* qpart = reinterpret_cast<double*>(base + kNoAllocPk + kNoAllocX);
+        int* f0p = reinterpret_cast<int*>(base + kNoAllocPk + kNoAllocX + kNoAllocQpart);
+        hipStream_t st = ctx->stream;
+        hip_check(mgw::launch_pk_rows(dpk, p, k0, width, nmo, mpad, st), "panel rows");
+        hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, ctx->n,
+                                     np, mpad, st),
+                  "pack X");
+        hip_check(mgw::launch_set_int(f0p, 0, st), "set offset");
+        hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), xp, qpart,
+                                           np, mpad, 1, st),
+                  "contract");
+        hip_check(mgw::launch_reduce_q(qpart, nullptr, q_dev, np / mgw::kCT, width, mpad, f0p, 1, st),
+                  "reduce q");
+        ctx->sync();
     });
 }
 

@@ -100,6 +100,9 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* x
 
 // nrt = row tiles of the contraction that wrote qpart (np / 128 for FP64,
 // contract_split_row_tiles(np) for the split form).
+// pkrow[c] = ((p nmo + k0 + c) nmo^2) for c < width, 0 for width <= c < mpad
+hipError_t launch_pk_rows(int64_t* pkrow, int p, int k0, int width, int nmo, int mpad,
+                          hipStream_t s);
 hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
                            int mpad, const int* f0p, int nb, hipStream_t s);
 

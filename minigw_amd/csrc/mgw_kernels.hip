@@ -359,6 +359,14 @@ __global__ void k_pack_x(const double* __restrict__ src, const int64_t* __restri
     for (int k = lane; k < np; k += 64) o[k] = k < n ? src[base + colidx[k]] : 0.0;
 }
 
+// pkrow[c] = ERI row offset of column (p, k0 + c) for c < width, 0 in the padding
+__global__ void k_pk_rows(int64_t* __restrict__ pkrow, int p, int k0, int width, int nmo, int mpad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= mpad) return;
+    const int64_t n2 = (int64_t)nmo * nmo;
+    pkrow[c] = c < width ? ((int64_t)p * nmo + k0 + c) * n2 : 0;
+}
+
 __global__ void k_reduce_q(const double* __restrict__ qpart, const int* __restrict__ colmap,
                            double* __restrict__ q, int nrt, int m, int mpad,
                            const int* __restrict__ f0p) {
@@ -860,6 +868,13 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* x
                                int np, int mpad, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_contract_f64, dim3(mpad / kCT, np / kCT, nb), dim3(256), 0, s, M, S, xp,
                        qpart, np, mpad);
+    return hipGetLastError();
+}
+
+hipError_t launch_pk_rows(int64_t* pkrow, int p, int k0, int width, int nmo, int mpad,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_pk_rows, dim3((mpad + 255) / 256), dim3(256), 0, s, pkrow, p, k0, width,
+                       nmo, mpad);
     return hipGetLastError();
 }
 
