@@ -114,7 +114,10 @@ int mgw_hip_upload_vph(mgw_hip_ctx* ctx, const double* v_rowmajor, int32_t n);
  * (src/workspace/device_screening_workspace.cu:337-374).  pi0_im may be NULL
  * (all zero, eta == 0: real FP64 MFMA path); a non-zero imaginary part
  * (eta != 0) takes the complex FP64 path.  Re(pi0) > 0 (a non-positive ph
- * gap) is MGW_ERR_UNSUPPORTED; a zero pivot is MGW_ERR_NUMERIC. */
+ * gap) is MGW_ERR_UNSUPPORTED.  At eta == 0 a matrix I + S V S that is not
+ * positive definite (V_ph not positive semidefinite) is factored by LU with
+ * partial pivoting instead and kept as its dense inverse, as the reference's
+ * getrf/getrs do; a singular one is MGW_ERR_NUMERIC. */
 int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0_im, int32_t n);
 
 /* Copy the resident W_c (n x n, row-major) to the host.  wc_im may be NULL. */
@@ -325,8 +328,9 @@ int mgw_hip_server_stats(mgw_hip_server* server, int64_t* completed, int32_t* ru
  * reduction, [10] contraction columns (unordered (p,k) pairs when the ERI is
  * exactly pk-symmetric), [11] time in replayed batch graphs (phases 1-3 and 9
  * are then 0), [12] rank of the ERI decomposition (MGW_PREC_FP64_CD, else 0),
- * [13] its host wall time.  n is the number of entries the caller provides
- * (<= 14). */
+ * [13] its host wall time, [14] frequencies whose symmetrised dielectric
+ * matrix was not positive definite and was screened by pivoted LU instead
+ * (MGW_PREC_FP64).  n is the number of entries the caller provides (<= 15). */
 int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n);
 
 /* Autotuner (no counterpart in the reference): times mgw_hip_g0w0 on the

@@ -10,7 +10,7 @@ REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib", "libmgw_hip.so")
 SOURCES = ["mgw_kernels.hip", "mgw_split.hip", "mgw_ozaki.hip", "mgw_ozfact.hip", "mgw_complex.hip",
-           "mgw_cd.hip", "mgw_peak.hip", "mgw_capi.hip", "mgw_serve.cpp", "mgw_io.cpp"]
+           "mgw_cd.hip", "mgw_lu.hip", "mgw_peak.hip", "mgw_capi.hip", "mgw_serve.cpp", "mgw_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

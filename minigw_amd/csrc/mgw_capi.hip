@@ -106,7 +106,7 @@ struct mgw_hip_ctx {
     std::string error;
     size_t budget = 0;     // 0 = no cap
     size_t held = 0;
-    double timings[14] = {0};
+    double timings[15] = {0};
 
     // grow-only device buffer with budget accounting
     struct Buf {
@@ -205,13 +205,18 @@ struct mgw_hip_ctx {
     int judge_used = 0;               // frequencies the last run judged
     int fell_back = 0;                // the last mgw_hip_g0w0 reran in FP64 (1: judge, 2: numeric)
     Buf hqj;                          // the judge's q row
+    // pivoted-LU screening of a dielectric matrix that is not positive definite
+    Buf luz, ludl, ludu, lupiv, wz;   // W = E^-1 (batch path / workspace), block inverses, pivots
+    bool wc_dense = false;            // the workspace's W_c is held as dense E^-1 (wz)
+    int lu_freqs = 0;                 // frequencies of the last mgw_hip_g0w0 screened by the LU
 
     mgw_hip_ctx() {
         all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
                &wqpart, &wq, &wsrc, &wcol, &hA, &hS, &hdinv, &htbuf, &hinfo, &hpkrow, &hqpart,
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
-               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj};
+               &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj, &luz, &ludl, &ludu,
+               &lupiv, &wz};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -424,6 +429,41 @@ void check_info(mgw_hip_ctx* ctx, const int* dinfo, int count, const char* what)
             fail(MGW_ERR_NUMERIC,
                  "%s: symmetrised dielectric matrix %d is not positive definite (pivot %d)", what, i,
                  info[i]);
+}
+
+// E = I + S V S of `count` matrices (S: count x np, V: np x np) factored by
+// pivoted LU and inverted into W (count x np x np) -- the screening of a
+// dielectric matrix the Cholesky path cannot factor.  A is the matrices'
+// workspace (count x np x np, overwritten).  A singular E is MGW_ERR_NUMERIC.
+void lu_screen(mgw_hip_ctx* ctx, const double* V, const double* S, double* A, double* W, int np,
+               int count) {
+    hipStream_t st = ctx->stream;
+    const size_t nblk = np / mgw::kNB;
+    double* dl = as<double>(ctx->get(ctx->ludl, (size_t)count * nblk * 4096 * 8, "LU block inverses"));
+    double* du = as<double>(ctx->get(ctx->ludu, (size_t)count * nblk * 4096 * 8, "LU block inverses"));
+    int* piv = as<int>(ctx->get(ctx->lupiv, ((size_t)2 * count * np + count) * 4, "LU pivots"));
+    int* perm = piv + (size_t)count * np;
+    int* info = perm + (size_t)count * np;
+    std::vector<int> zero(count, 0);
+    ctx->h2d(info, zero.data(), count);
+    hip_check(mgw::launch_dielectric_full(V, S, A, np, count, st), "dielectric (full)");
+    hip_check(mgw::launch_lu_factor(A, piv, info, dl, du, np, count, st), "LU");
+    std::vector<int> hp((size_t)count * np), hi(count);
+    ctx->d2h(hp.data(), piv, hp.size());
+    ctx->d2h(hi.data(), info, count);
+    ctx->sync();
+    for (int w = 0; w < count; ++w)
+        if (hi[w] != 0)
+            fail(MGW_ERR_NUMERIC, "getrf: the dielectric matrix is singular (zero pivot %d)", hi[w]);
+    // rows of P^T I: the interchanges applied in order to the identity's rows
+    std::vector<int> hperm((size_t)count * np);
+    for (int w = 0; w < count; ++w) {
+        int* pm = hperm.data() + (size_t)w * np;
+        std::iota(pm, pm + np, 0);
+        for (int c = 0; c < np; ++c) std::swap(pm[c], pm[hp[(size_t)w * np + c]]);
+    }
+    ctx->h2d(perm, hperm.data(), hperm.size());
+    hip_check(mgw::launch_lu_inverse(A, dl, du, perm, W, np, count, st), "LU inverse");
 }
 
 struct CdResult {
@@ -781,6 +821,7 @@ int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0
             hip_check(mgw::launch_potrf_c(A, dinv, info, np, 1, ctx->stream), "potrf (complex)");
             check_info(ctx, info, 1, "compute_w_c");
             ctx->wc_complex = true;
+            ctx->wc_dense = false;
             ctx->wc_ready = true;
             return;
         }
@@ -796,7 +837,14 @@ int mgw_hip_compute_wc(mgw_hip_ctx* ctx, const double* pi0_re, const double* pi0
         hip_check(mgw::launch_dielectric(as<double>(ctx->v.p), S, A, np, 1, ctx->stream),
                   "dielectric");
         factor_batch(ctx, A, dinv, info, tbuf, np, 1);
-        check_info(ctx, info, 1, "compute_w_c");
+        int bad = 0;
+        ctx->d2h(&bad, info, 1);
+        ctx->sync();
+        ctx->wc_dense = bad != 0;
+        if (bad) {      // not positive definite: pivoted LU, W = E^-1 kept dense
+            double* W = as<double>(ctx->get(ctx->wz, (size_t)np * np * 8, "E^-1"));
+            lu_screen(ctx, as<double>(ctx->v.p), S, A, W, np, 1);
+        }
         ctx->wc_complex = false;
         ctx->wc_ready = true;
     });
@@ -824,9 +872,14 @@ int mgw_hip_download_wc(mgw_hip_ctx* ctx, double* wc_re, double* wc_im, int32_t 
             return;
         }
         double* wc = as<double>(ctx->get(ctx->wsrc, (size_t)n * n * 8, "W_c staging"));
-        hip_check(mgw::launch_wc_explicit(as<double>(ctx->wA.p), as<double>(ctx->wS.p), wc, n,
-                                          ctx->np, ctx->stream),
-                  "W_c");
+        if (ctx->wc_dense)
+            hip_check(mgw::launch_wc_from_dense(as<double>(ctx->wz.p), as<double>(ctx->wS.p), wc, n,
+                                                ctx->np, ctx->stream),
+                      "W_c (dense)");
+        else
+            hip_check(mgw::launch_wc_explicit(as<double>(ctx->wA.p), as<double>(ctx->wS.p), wc, n,
+                                              ctx->np, ctx->stream),
+                      "W_c");
         ctx->d2h(wc_re, wc, (size_t)n * n);
         ctx->sync();
         if (wc_im) std::fill(wc_im, wc_im + (size_t)n * n, 0.0);
@@ -865,9 +918,14 @@ void contract_single(mgw_hip_ctx* ctx, const double* src, const int64_t* hpkrow,
         ctx->sync();
         return;
     }
-    hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), xp, qpart, np,
-                                       mpad, 1, ctx->stream),
-              "contract");
+    if (ctx->wc_dense)
+        hip_check(mgw::launch_contract_dense(as<double>(ctx->wz.p), as<double>(ctx->wS.p), xp, qpart,
+                                             np, mpad, 1, ctx->stream),
+                  "contract (dense)");
+    else
+        hip_check(mgw::launch_contract_f64(as<double>(ctx->wA.p), as<double>(ctx->wS.p), xp, qpart,
+                                           np, mpad, 1, ctx->stream),
+                  "contract");
     hip_check(mgw::launch_reduce_q(qpart, nullptr, q, np / mgw::kCT, m, mpad, f0p, 1, ctx->stream),
               "reduce q");
     ctx->d2h(q_re, q, m);
@@ -1007,9 +1065,10 @@ int mgw_hip_quadratic_forms_noalloc(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int
         need(ctx && q_dev, "mgw_hip_quadratic_forms_noalloc: NULL argument");
         if (!ctx->nmo) fail(MGW_ERR_STATE, "fill_panel before the ERI was uploaded");
         if (!ctx->wc_ready) fail(MGW_ERR_STATE, "quadratic_forms_panel before compute_w_c");
-        if (ctx->wc_complex)
+        if (ctx->wc_complex || ctx->wc_dense)
             fail(MGW_ERR_UNSUPPORTED,
-                 "mgw_hip_quadratic_forms_noalloc: complex W_c (eta != 0) is not supported");
+                 "mgw_hip_quadratic_forms_noalloc: needs the Cholesky form of a real W_c (eta == 0, "
+                 "positive definite dielectric matrix)");
         const int nmo = ctx->nmo, np = ctx->np;
         need(p >= 0 && p < nmo, "DevicePqPhPanelView::fill_panel: p index out of range");
         need(k0 >= 0 && width >= 0 && k0 <= nmo && width <= nmo - k0,
@@ -1404,7 +1463,36 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             t_aux += d + e;
             ++n_contr;
         }
-        check_info(ctx, info, nfreq, "potrf");
+        // Frequencies whose symmetrised dielectric matrix is not positive
+        // definite: the plain FP64 form screens them by pivoted LU (their q rows
+        // are replaced); the other forms report the breakdown (the mixed forms'
+        // judge then reruns the job in FP64).
+        ctx->lu_freqs = 0;
+        {
+            std::vector<int> hinfo(nfreq);
+            ctx->d2h(hinfo.data(), info, nfreq);
+            ctx->sync();
+            std::vector<int> failed;
+            for (int f = 0; f < nfreq; ++f)
+                if (hinfo[f] != 0) failed.push_back(f);
+            if (!failed.empty()) {
+                if (cplx || cd || mixed || i8)
+                    fail(MGW_ERR_NUMERIC,
+                         "potrf: symmetrised dielectric matrix %d is not positive definite (pivot %d)",
+                         failed[0], hinfo[failed[0]]);
+                double* W = as<double>(ctx->get(ctx->luz, (size_t)np * np * 8, "E^-1"));
+                for (int f : failed) {
+                    hip_check(mgw::launch_set_int(f0p, f, st), "set offset");
+                    hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, 1, st), "scale");
+                    lu_screen(ctx, dV, S, A, W, np, 1);
+                    hip_check(mgw::launch_contract_dense(W, S, xp, qpart, np, mpad, 1, st),
+                              "contract (dense)");
+                    hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0p, 1, st),
+                              "reduce q");
+                }
+                ctx->lu_freqs = (int)failed.size();
+            }
+        }
         ctx->wc_ready = false;
 
         // Error judge of the mixed forms: a sample of the frequencies recomputed
@@ -1484,6 +1572,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->timings[11] = t_graph;
         ctx->timings[12] = cd_rank;
         ctx->timings[13] = cd_ms;
+        ctx->timings[14] = ctx->lu_freqs;
         return true;
 }
 }  // namespace
@@ -1501,7 +1590,8 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             done = g0w0_body(ctx, job, out);
         } catch (const MgwError& e) {
             // a mixed form's factorization breakdown falls back like a failed judge
-            if (!(mixed_form && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC)) throw;
+            if (!(mixed_form && ctx->judge_n > 0 && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC))
+                throw;
             ctx->fell_back = 2;
         }
         if (done) return;
@@ -1778,7 +1868,7 @@ int mgw_hip_last_timings(const mgw_hip_ctx* ctx, double* ms, int32_t n) {
         g_thread_error = "mgw_hip_last_timings: bad argument";
         return MGW_ERR_ARG;
     }
-    for (int i = 0; i < n && i < 14; ++i) ms[i] = ctx->timings[i];
+    for (int i = 0; i < n && i < 15; ++i) ms[i] = ctx->timings[i];
     return MGW_OK;
 }
 

@@ -68,6 +68,24 @@ hipError_t launch_trsm_block(double* A, const double* dinv, int np, int k, int n
 hipError_t launch_trtri_block(double* A, const double* dinv, const double* tbuf, int np, int j,
                               int nb, hipStream_t s);
 
+// Pivoted-LU screening (mgw_lu.hip) for a symmetrised dielectric matrix that
+// is not positive definite: E = I + S V S in full, E = P L U in place
+// (piv[w][c] = row swapped with row c, info[w] = first zero pivot, dl / du =
+// inverses of the diagonal blocks' unit lower / upper factors, nb x (np/64) x
+// 64 x 64 each), W = E^-1 into Z from perm (perm[w][r]: row r of P^T I is the
+// unit row perm[r]), and the q partials of a dense W (as launch_contract_f64's,
+// (S x)^T W (S x) summed per 128-row tile) and W_c = -S W S.
+hipError_t launch_dielectric_full(const double* v, const double* S, double* A, int np, int nb,
+                                  hipStream_t s);
+hipError_t launch_lu_factor(double* A, int* piv, int* info, double* dl, double* du, int np, int nb,
+                            hipStream_t s);
+hipError_t launch_lu_inverse(const double* A, const double* dl, const double* du, const int* perm,
+                             double* Z, int np, int nb, hipStream_t s);
+hipError_t launch_contract_dense(const double* W, const double* S, const double* xp, double* qpart,
+                                 int np, int mpad, int nb, hipStream_t s);
+hipError_t launch_wc_from_dense(const double* W, const double* S, double* wc, int n, int np,
+                                hipStream_t s);
+
 // Factorization of the mixed forms (mgw_ozfact.hip): the same blocked
 // Cholesky and inverse sweep with the block products on int8 MFMA.  RP/RS and
 // CP/CS hold the row- and column-scaled slices of every 64 x 64 block

@@ -244,11 +244,11 @@ class HipContext:
         self.call("mgw_hip_release_buffers")
 
     def last_timings(self) -> dict:
-        t = np.zeros(14)
-        self.call("mgw_hip_last_timings", _ptr(t), 14)
+        t = np.zeros(15)
+        self.call("mgw_hip_last_timings", _ptr(t), 15)
         keys = ["build_v", "dielectric", "factorize", "contract", "sigma_c", "pade_qp", "total",
                 "freq_batch", "contract_launches", "split_reduce", "contract_columns",
-                "graph_batches", "cd_rank", "cd_wall_ms"]
+                "graph_batches", "cd_rank", "cd_wall_ms", "lu_freqs"]
         return dict(zip(keys, t.tolist()))
 
     def probe_mfma(self, kind: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:

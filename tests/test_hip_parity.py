@@ -447,3 +447,63 @@ def test_fp64_cd_tolerance_and_errors(hip, ctx):
     assert e.value.code == hip.MGW_ERR_UNSUPPORTED
     ok = hip.run_g0w0(inp, 16, 8, ctx=ctx, precision=hip.PREC_FP64_CD)
     assert np.abs(ok.sigma_c - tight.sigma_c).max() == 0.0
+
+
+# ------------------------------------------------------------ not positive definite
+def _indefinite_input(nmo, nocc, seed, alpha):
+    """The seeded molecule with a rank-1 pair term subtracted from its ERI,
+    eri[p,q,r,s] -= alpha U[p,q] U[r,s] (U symmetric; 8-fold symmetry kept):
+    V_ph gets a negative eigenvalue, so I + S V S is indefinite at the low
+    imaginary frequencies (S^2 = 4 D / (w^2 + D^2) large) and positive definite
+    at the high ones."""
+    inp = synthetic_input(nmo, nocc, seed)
+    rng = np.random.default_rng(seed)
+    u = rng.normal(0.0, 1.0, (nmo, nmo))
+    u = 0.5 * (u + u.T)
+    eri = inp.eri - alpha * np.einsum("pq,rs->pqrs", u, u)
+    return GwInput(mo_energy=inp.mo_energy, eri=np.ascontiguousarray(eri), vxc=inp.vxc,
+                   nocc=inp.nocc, fermi_energy=inp.fermi_energy)
+
+
+@pytest.mark.parametrize("nmo,nocc", [(30, 6), (48, 10)])
+def test_indefinite_dielectric_takes_pivoted_lu(hip, ctx, nmo, nocc):
+    """V_ph with a negative eigenvalue: where I + S V S is not positive
+    definite the FP64 path screens by LU with partial pivoting (the reference's
+    getrf/getrs) instead of stopping; W_c and q against the oracle's LU, the
+    whole chain against the oracle, and the int8 form, whose factorization
+    breaks down there, rerun in FP64 by its judge's fallback."""
+    say(f"case: indefinite dielectric matrix, nmo={nmo} nocc={nocc}")
+    inp = _indefinite_input(nmo, nocc, 7, 0.05)
+    ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+    v = orc.v_ph_matrix(inp.eri, ph)
+    assert np.linalg.eigvalsh(0.5 * (v + v.T)).min() < -0.5
+    ws = hip.HipScreeningWorkspace(v, ctx)
+    view = hip.HipPqPhPanelView(inp.eri, inp.nocc, ctx)
+    for w in (1e-3, 0.5, 40.0):
+        pi0 = orc.pi0_diag_real(w, ph.delta)
+        s = np.sqrt(-pi0)
+        spd = np.linalg.eigvalsh(np.eye(ph.size) + s[:, None] * v * s[None, :]).min() > 0
+        ws.compute_w_c(pi0)
+        want = orc.compute_w_c(v, pi0)
+        err = np.abs(ws.w_c().real - want).max() / np.abs(want).max()
+        q = view.quadratic_forms(2, 0, nmo)
+        qw = orc.quadratic_forms_panel(want, orc.pq_panel(inp.eri, ph, 2, 0, nmo))
+        eq = np.abs(q.real - qw).max() / np.abs(qw).max()
+        say(f"   w={w}: positive definite {spd}, W_c rel err {err:.2e}, q rel err {eq:.2e}")
+        assert err < 1e-11 and eq < 1e-11
+    res = hip.run_g0w0(inp, 40, 12, ctx=ctx)
+    lu = int(res.timings["lu_freqs"])
+    want = orc.run_g0w0(inp, 40, 12)
+    d = np.abs(res.sigma_c - want.sigma_c).max() / np.abs(want.sigma_c).max()
+    say(f"   G0W0: {lu} of 40 frequencies by pivoted LU, rel |dSigma_c| {d:.2e}")
+    assert 0 < lu < 40 and d < 1e-11
+    with pytest.raises(hip.HipError) as e:
+        hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+    assert e.value.code == hip.MGW_ERR_NUMERIC
+    ctx.set_judge(2)
+    try:
+        fb = hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        assert ctx.last_judge()["fell_back"] == 2
+    finally:
+        ctx.set_judge(0)
+    assert np.array_equal(fb.sigma_c, res.sigma_c)
