@@ -6,6 +6,7 @@
 
 #include "mgw_kernels.h"
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +19,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -45,6 +47,46 @@ struct MgwError : std::runtime_error {
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) fail(MGW_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
 }
+
+// Named host ranges for rocprofv3 --marker-trace, with the phase names of the
+// reference's profiling table (src/profiling.cpp:12-32): "screening setup",
+// "build W_c", "sigma_c", "sigma_c wall", "exchange", "pade/output pade".
+// The ROCTx library is loaded at run time (rocprofiler-sdk's, else the
+// roctracer one); without it the ranges cost a branch.  A range spans the
+// host's enqueue of a phase and, where the driver waits (after each frequency
+// batch and at the end of a molecule), the device work too.
+struct Roctx {
+    int (*push)(const char*) = nullptr;
+    int (*pop)() = nullptr;
+    Roctx() {
+        for (const char* lib : {"librocprofiler-sdk-roctx.so.1", "libroctx64.so.4",
+                                "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"}) {
+            void* h = dlopen(lib, RTLD_NOW | RTLD_LOCAL);
+            if (!h) continue;
+            push = reinterpret_cast<int (*)(const char*)>(dlsym(h, "roctxRangePushA"));
+            pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+            if (push && pop) return;
+            push = nullptr;
+            pop = nullptr;
+        }
+    }
+};
+
+Roctx& roctx() {
+    static Roctx r;
+    return r;
+}
+
+struct Range {
+    explicit Range(const char* name) {
+        if (roctx().push) roctx().push(name);
+    }
+    ~Range() {
+        if (roctx().pop) roctx().pop();
+    }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
 
 }  // namespace
 
@@ -1190,6 +1232,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             }
         } evg{ev};
         hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
+        auto setup_range = std::make_unique<Range>("screening setup");
 
         // resident per-molecule data (the decomposed form needs no V_ph)
         double* dV = cd ? nullptr : as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
@@ -1279,6 +1322,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                       "pack X (judge)");
         }
         hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
+        setup_range.reset();
 
         // per-molecule outputs before the batch, which takes the room that is left
         double* sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
@@ -1449,8 +1493,11 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 ++n_contr;
                 continue;
             }
-            enqueue_batch(b, true);
-            hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
+            {
+                Range batch_range("build W_c + sigma_c (frequency batch)");
+                enqueue_batch(b, true);
+                hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
+            }
             float a = 0, bb = 0, c = 0, d = 0, e = 0;
             hip_check(hipEventElapsedTime(&a, ev[2], ev[3]), "hipEventElapsedTime");
             hip_check(hipEventElapsedTime(&bb, ev[3], ev[4]), "hipEventElapsedTime");
@@ -1535,12 +1582,19 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         if (out->q) ctx->d2h(out->q, dq, (size_t)nfreq * m);
 
         hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-        hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq, gq,
-                                      sre, sim, st),
-                  "sigma_c");
-        hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-        hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), sx, nmo, nocc, st), "sigma_x");
         {
+            Range r("sigma_c wall");
+            hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq,
+                                          gq, sre, sim, st),
+                      "sigma_c");
+        }
+        hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+        {
+            Range r("exchange");
+            hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), sx, nmo, nocc, st), "sigma_x");
+        }
+        {
+            Range r("pade/output pade");
             std::vector<double> nan(nmo, std::numeric_limits<double>::quiet_NaN());
             std::vector<int> zero(nmo, 0);
             ctx->h2d(qp, nan.data(), nmo);
