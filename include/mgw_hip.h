@@ -159,6 +159,26 @@ int mgw_hip_build_vph_from_eri(mgw_hip_ctx* ctx);
 int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width,
                             double* q_re, double* q_im);
 
+/* The two halves of mgw_hip_quadratic_forms for callers that keep the
+ * reference's split between the panel source and the screening workspace
+ * (two contexts on one device).  mgw_hip_fill_panel replaces
+ * DevicePqPhPanelView::fill_panel (src/workspace/device_pq_ph_panel.cu:284-314):
+ * it writes X(ia, kk) = (p, k0 + kk | i a) into a device buffer owned by ctx,
+ * column-major with leading dimension *ld (nph rounded up to 256, rows past
+ * nph zero), and returns it in *x_dev; the panel stays valid until the next
+ * fill on ctx or its destruction, and is complete when the call returns.
+ * The values are real (the ERI is real), not the reference's complex copy. */
+int mgw_hip_fill_panel(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width,
+                       const double** x_dev, int32_t* ld);
+
+/* q[c] = x_c^T W_c x_c for the nvec columns of a device panel x_dev (column c
+ * at x_dev + c * ld, n = nph rows read, ld >= n) on ctx's device; replaces
+ * DeviceScreeningWorkspace::quadratic_forms_panel(const DeviceComplexPanelView&)
+ * (src/workspace/device_screening_workspace.cu:376-417).  A pointer that is
+ * not device memory of ctx's device is MGW_ERR_ARG. */
+int mgw_hip_quadratic_forms_device(mgw_hip_ctx* ctx, const double* x_dev, int32_t n, int32_t ld,
+                                   int32_t nvec, double* q_re, double* q_im);
+
 /* mgw_hip_quadratic_forms without any device allocation, for callers that
  * cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The

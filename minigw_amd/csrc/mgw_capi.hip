@@ -251,6 +251,7 @@ struct mgw_hip_ctx {
     Buf luz, ludl, ludu, lupiv, wz;   // W = E^-1 (batch path / workspace), block inverses, pivots
     bool wc_dense = false;            // the workspace's W_c is held as dense E^-1 (wz)
     int lu_freqs = 0;                 // frequencies of the last mgw_hip_g0w0 screened by the LU
+    Buf ppk, panel;                   // the last filled device panel and its row offsets
 
     mgw_hip_ctx() {
         all = {&flag, &hcolmap, &wxp, &hxp, &f0buf, &eri, &v, &colidx, &rowbase, &delta, &wA, &wS, &wdinv, &wtbuf, &winfo, &wpkrow,
@@ -258,7 +259,7 @@ struct mgw_hip_ctx {
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
                &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj, &luz, &ludl, &ludu,
-               &lupiv, &wz};
+               &lupiv, &wz, &ppk, &panel};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -1142,6 +1143,51 @@ int mgw_hip_quadratic_forms_noalloc(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int
         hip_check(mgw::launch_reduce_q(qpart, nullptr, q_dev, np / mgw::kCT, width, mpad, f0p, 1, st),
                   "reduce q");
         ctx->sync();
+    });
+}
+
+int mgw_hip_fill_panel(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width, const double** x_dev,
+                       int32_t* ld) {
+    return guarded(ctx, [&] {
+        need(ctx && x_dev && ld, "mgw_hip_fill_panel: NULL argument");
+        if (!ctx->nmo) fail(MGW_ERR_STATE, "fill_panel before the ERI was uploaded");
+        const int nmo = ctx->nmo;
+        need(p >= 0 && p < nmo, "DevicePqPhPanelView::fill_panel: p index out of range");
+        need(k0 >= 0 && width >= 0 && k0 <= nmo && width <= nmo - k0,
+             "DevicePqPhPanelView::fill_panel: k panel exceeds nmo");
+        const PhLayout L = ph_layout(nmo, ctx->nocc);
+        const int mpad = mgw::round_up(std::max(width, 1), mgw::kPad);
+        int64_t* dpk = as<int64_t>(ctx->get(ctx->ppk, (size_t)mpad * 8, "panel rows"));
+        double* xp = as<double>(ctx->get(ctx->panel, (size_t)mpad * L.np * 8, "device panel"));
+        hip_check(mgw::launch_pk_rows(dpk, p, k0, width, nmo, mpad, ctx->stream), "panel rows");
+        hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, L.n,
+                                     L.np, mpad, ctx->stream),
+                  "pack X");
+        ctx->sync();   // the panel may be read on another context's stream
+        *x_dev = xp;
+        *ld = L.np;
+    });
+}
+
+int mgw_hip_quadratic_forms_device(mgw_hip_ctx* ctx, const double* x_dev, int32_t n, int32_t ld,
+                                   int32_t nvec, double* q_re, double* q_im) {
+    return guarded(ctx, [&] {
+        need(ctx && x_dev && q_re, "mgw_hip_quadratic_forms_device: NULL argument");
+        if (!ctx->wc_ready) fail(MGW_ERR_STATE, "quadratic_forms_panel before compute_w_c");
+        need(n == ctx->n && nvec >= 0 && ld >= n,
+             "DeviceScreeningWorkspace::quadratic_forms_panel: inconsistent dimensions");
+        hipPointerAttribute_t attr{};
+        const hipError_t e = hipPointerGetAttributes(&attr, x_dev);
+        if (e != hipSuccess) (void)hipGetLastError();
+        need(e == hipSuccess && attr.type == hipMemoryTypeDevice && attr.device == ctx->device,
+             "mgw_hip_quadratic_forms_device: the panel is not device memory of device %d",
+             ctx->device);
+        if (nvec == 0) return;
+        std::vector<int64_t> pk(nvec);
+        for (int c = 0; c < nvec; ++c) pk[c] = (int64_t)c * ld;
+        std::vector<int> col(ctx->np);
+        std::iota(col.begin(), col.end(), 0);
+        contract_single(ctx, x_dev, pk.data(), nvec, col.data(), q_re, q_im);
     });
 }
 

@@ -81,6 +81,10 @@ def _load():
         "mgw_hip_release_buffers": (C.c_int, [vp]),
         "mgw_hip_build_vph_from_eri": (C.c_int, [vp]),
         "mgw_hip_quadratic_forms": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
+        "mgw_hip_fill_panel": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp),
+                                         C.POINTER(C.c_int32)]),
+        "mgw_hip_quadratic_forms_device": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, _dp,
+                                                     _dp]),
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
         "mgw_hip_pade_qp": (C.c_int, [vp, C.c_int32, C.c_int32, _ip, C.c_int32, C.c_int32, _dp,
                                       _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _ip]),
@@ -149,6 +153,12 @@ def transformed_legendre_grid(nfreq: int):
 def estimate_bytes(nmo: int, nocc: int, nfreq: int, nstates: int = 0, nbatch: int = 0,
                    precision: int = PREC_FP64) -> int:
     return int(lib.mgw_hip_estimate_bytes(nmo, nocc, nfreq, nstates, nbatch, precision))
+
+
+def device_count() -> int:
+    """Visible HIP devices (0 where the runtime finds none)."""
+    n = C.c_int(0)
+    return n.value if lib.mgw_hip_device_count(C.byref(n)) == MGW_OK else 0
 
 
 def device_for_local_rank(local_rank: int, tasks_per_gpu: int) -> int:
@@ -318,12 +328,29 @@ class HipScreeningWorkspace:
         self.ctx.call("mgw_hip_download_wc", _ptr(re), _ptr(im), self._n)
         return re + 1j * im
 
-    def quadratic_forms_panel(self, x_panel: np.ndarray) -> np.ndarray:
+    def quadratic_forms_panel(self, x_panel) -> np.ndarray:
+        """q for a host panel (n_ph x nvec, row-major) or a HipDevicePanel."""
+        if isinstance(x_panel, HipDevicePanel):
+            q, qi = np.zeros(x_panel.cols), np.zeros(x_panel.cols)
+            self.ctx.call("mgw_hip_quadratic_forms_device", x_panel.ptr, x_panel.rows, x_panel.ld,
+                          x_panel.cols, _ptr(q), _ptr(qi))
+            return q + 1j * qi
         x = _f64(x_panel)
         n, nvec = x.shape
         q, qi = np.zeros(nvec), np.zeros(nvec)
         self.ctx.call("mgw_hip_quadratic_forms_host", _ptr(x), n, nvec, _ptr(q), _ptr(qi))
         return q + 1j * qi
+
+
+@dataclass
+class HipDevicePanel:
+    """DeviceComplexPanelView: a device panel (real FP64, column-major, column
+    c at ptr + c * ld) owned by the HipPqPhPanelView that filled it, valid until
+    its next fill_panel."""
+    ptr: int
+    rows: int
+    ld: int
+    cols: int
 
 
 class HipPqPhPanelView:
@@ -346,6 +373,12 @@ class HipPqPhPanelView:
         q, qi = np.zeros(max(width, 0)), np.zeros(max(width, 0))
         self.ctx.call("mgw_hip_quadratic_forms", p, k0, width, _ptr(q), _ptr(qi))
         return q + 1j * qi
+
+    def fill_panel(self, p: int, k0: int, width: int) -> HipDevicePanel:
+        """X(ia, kk) = (p, k0 + kk | i a) in device memory of this view's context."""
+        ptr, ld = C.c_void_p(), C.c_int32()
+        self.ctx.call("mgw_hip_fill_panel", p, k0, width, C.byref(ptr), C.byref(ld))
+        return HipDevicePanel(ptr.value or 0, self.nph, ld.value, width)
 
 
 @dataclass

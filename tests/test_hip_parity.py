@@ -137,6 +137,41 @@ def test_eri_panels_match_oracle(hip, ctx):
         view.quadratic_forms(0, 20, 5)
 
 
+def test_device_panels_across_contexts(hip, ctx):
+    """fill_panel on the panel source's context, quadratic_forms_panel on the
+    workspace's: the reference's two device objects, agreeing with the fused
+    entry point and with the oracle."""
+    say("case: device panels, panel source and workspace on two contexts, nmo=24")
+    inp = synthetic_input(24, 5, 11)
+    ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
+    v = orc.v_ph_matrix(inp.eri, ph)
+    ws = hip.HipScreeningWorkspace(v, ctx)
+    pi0 = orc.pi0_diag_real(0.8, ph.delta)
+    ws.compute_w_c(pi0)
+    wc = orc.compute_w_c(v, pi0)
+    src_ctx = hip.HipContext(0)
+    try:
+        view = hip.HipPqPhPanelView(inp.eri, inp.nocc, src_ctx)
+        fused = hip.HipPqPhPanelView(inp.eri, inp.nocc, ctx)
+        for p, k0, width in [(0, 0, 24), (23, 20, 4), (7, 3, 1), (5, 0, 0)]:
+            panel = view.fill_panel(p, k0, width)
+            assert panel.ld >= ph.size and panel.ld % 256 == 0 and panel.cols == width
+            q = ws.quadratic_forms_panel(panel)
+            want = orc.quadratic_forms_panel(wc, orc.pq_panel(inp.eri, ph, p, k0, width))
+            assert np.allclose(q.real, want, rtol=1e-11, atol=1e-13), (p, k0, width)
+            assert np.array_equal(q, fused.quadratic_forms(p, k0, width)), (p, k0, width)
+        with pytest.raises(hip.HipError) as e:
+            view.fill_panel(24, 0, 1)
+        assert "p index out of range" in str(e.value)
+        host = np.zeros(ph.size)
+        bad = hip.HipDevicePanel(host.ctypes.data, ph.size, ph.size, 1)
+        with pytest.raises(hip.HipError) as e:
+            ws.quadratic_forms_panel(bad)             # host memory is refused, not read
+        assert e.value.code == hip.MGW_ERR_ARG
+    finally:
+        src_ctx.close()
+
+
 def test_boundary_errors(hip, ctx):
     say("case: boundary error paths")
     with pytest.raises(hip.HipError) as e:

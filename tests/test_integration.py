@@ -1,34 +1,104 @@
 """The reference-side binding of the integration guide (its ```cpp file=...```
-blocks) compiles against the reference's own headers (syntax check; skipped
-where the tree is absent)."""
+blocks and ```diff``` patches), linked for real: oracle/build_integration.py
+compiles them with the reference's own sources into the reference's `gw`
+executable against libmgw_hip.so.  Here, without a GPU, the executable must
+run its CPU paths, refuse unknown hip forms, and fail loudly (library error,
+non-zero exit) when the hip backend finds no device.  Skipped where the
+reference tree or the built library is absent."""
 import os
-import re
 import subprocess
-import tempfile
+import sys
 
+import numpy as np
 import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = "/root/reference"
+GOLD = os.path.join(REPO, "tests", "golden")
+LIB = os.path.join(REPO, "minigw_amd", "lib", "libmgw_hip.so")
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+import build_integration as bi  # noqa: E402
+
+needs_tree = pytest.mark.skipif(not (os.path.isdir(os.path.join(REF, "src")) and os.path.exists(LIB)),
+                                reason="reference tree or built library absent")
 
 
-def cpp_blocks():
-    txt = open(os.path.join(REPO, "INTEGRATION.md")).read()
-    return re.findall(r"```cpp file=(\S+)\n(.*?)```", txt, flags=re.S)
+def test_guide_has_binding_files_and_patches():
+    blocks = bi.guide_blocks()
+    files = {n for k, n, _ in blocks if k == "cpp"}
+    patches = {n for k, n, _ in blocks if k == "diff"}
+    assert {"include/linalg/linalg_hip.hpp", "src/linalg/linalg_hip.cpp", "include/gw/sigma_c_hip.hpp",
+            "src/gw/sigma_c_hip.cpp", "include/workspace/hip_screening_workspace.hpp",
+            "src/workspace/hip_screening_workspace.cpp", "src/hdf5_input_hip.cpp"} <= files
+    assert patches == {"src/linalg/backend_factory.cpp", "src/gw/sigma_c_driver.cpp"}
 
 
-def test_integration_has_binding_files():
-    names = [n for n, _ in cpp_blocks()]
-    assert "src/linalg/linalg_hip.cpp" in names and "src/gw/sigma_c_hip.cpp" in names
+@pytest.fixture(scope="module")
+def gw_exe(tmp_path_factory):
+    if not (os.path.isdir(os.path.join(REF, "src")) and os.path.exists(LIB)):
+        pytest.skip("reference tree or built library absent")
+    out = str(tmp_path_factory.mktemp("gw") / "gw_hip")
+    return bi.build(out, REF, jobs=8, log=lambda *a: None)
 
 
-@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "include")), reason="reference tree absent")
-@pytest.mark.parametrize("name,code", cpp_blocks())
-def test_binding_compiles_against_reference_headers(name, code):
-    with tempfile.TemporaryDirectory() as td:
-        src = os.path.join(td, os.path.basename(name))
-        open(src, "w").write(code)
-        res = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall",
-                              "-I" + os.path.join(REF, "include"), "-I" + os.path.join(REPO, "include"),
-                              src], capture_output=True, text=True)
-        assert res.returncode == 0, res.stderr
+@pytest.fixture(scope="module")
+def h2o_dir(tmp_path_factory):
+    """The H2O regression input in the library's raw form, under the file name
+    the reference's main reads (gw_input.h5)."""
+    from minigw_amd import gwio
+    from minigw_amd.synthetic import GwInput
+    d = np.load(os.path.join(GOLD, "h2o_input.npz"))
+    inp = GwInput(mo_energy=d["mo_energy"], eri=d["eri"], vxc=d["vxc"], nocc=int(d["nocc"]),
+                  fermi_energy=float(d["fermi_energy"]))
+    path = tmp_path_factory.mktemp("h2o")
+    gwio.write_gw_input_raw(str(path / "gw_input.h5"), inp)
+    return path
+
+
+def run(exe, *args, timeout=300):
+    return subprocess.run([exe] + list(args), capture_output=True, text=True, timeout=timeout)
+
+
+@needs_tree
+def test_gw_links_against_the_library(gw_exe):
+    dyn = subprocess.run(["readelf", "-d", gw_exe], capture_output=True, text=True).stdout
+    assert "libmgw_hip.so" in dyn
+    res = run(gw_exe, "--help")
+    assert res.returncode == 0 and "--linalg-backend" in res.stdout + res.stderr
+
+
+@needs_tree
+def test_reference_backend_runs_in_the_linked_executable(gw_exe, h2o_dir, tmp_path):
+    """The patched factory and driver leave the reference's own CPU path intact."""
+    out = tmp_path / "out"
+    out.mkdir()
+    res = run(gw_exe, "--input-dir", str(h2o_dir), "--output-dir", str(out), "--linalg-backend",
+              "reference", "--freq-points", "12", "--state", "5")
+    assert res.returncode == 0, res.stderr
+    lines = (out / "E_c_before_Pade.out").read_text().splitlines()
+    assert len(lines) == 12
+    assert "G0W0 Calculation Summary" in (out / "gw.out").read_text()
+
+
+@needs_tree
+def test_unknown_hip_form_is_refused(gw_exe, h2o_dir, tmp_path):
+    out = tmp_path / "out"
+    out.mkdir()
+    res = run(gw_exe, "--input-dir", str(h2o_dir), "--output-dir", str(out), "--linalg-backend",
+              "hip-bogus")
+    assert res.returncode != 0 and "Unknown hip backend: hip-bogus" in res.stderr
+
+
+@needs_tree
+@pytest.mark.parametrize("form", ["hip", "hip-int8", "hip-workspace"])
+def test_hip_backend_fails_loudly_without_a_device(gw_exe, h2o_dir, tmp_path, form):
+    from minigw_amd import hip
+    if hip.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    out = tmp_path / "out"
+    out.mkdir()
+    res = run(gw_exe, "--input-dir", str(h2o_dir), "--output-dir", str(out), "--linalg-backend", form)
+    assert res.returncode != 0
+    assert "mgw_hip_device_for_local_rank failed" in res.stderr
+    assert not (out / "E_c.out").exists()
