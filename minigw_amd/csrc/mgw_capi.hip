@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <limits>
 #include <map>
 #include <memory>
@@ -144,6 +145,9 @@ struct mgw_hip_ctx {
     hipStream_t side[kSides] = {nullptr, nullptr, nullptr};
     hipEvent_t fork_ev = nullptr, join_ev[kSides] = {nullptr, nullptr, nullptr};
     int factor_streams = 2;           // MGW_FACTOR_STREAMS: 1, 2 or 4 parts
+    // most frequencies factored at once (MGW_FREQ_BATCH_CAP); at nmo = 200 a
+    // batch of 100 takes 97 GB in the int8 form
+    int batch_cap = 100;
     int split_order = 4096;           // MGW_FACTOR_SPLIT_ORDER: smallest order that splits
     std::string error;
     size_t budget = 0;     // 0 = no cap
@@ -696,6 +700,8 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         if (const char* env = std::getenv("MGW_FACTOR_STREAMS"))
             ctx->factor_streams = env[0] == '1' ? 1 : env[0] == '4' ? 4 : 2;
         if (const char* env = std::getenv("MGW_FACTOR_SPLIT_ORDER")) ctx->split_order = std::max(0, std::atoi(env));
+        if (const char* env = std::getenv("MGW_FREQ_BATCH_CAP"))
+            ctx->batch_cap = std::min(1024, std::max(1, std::atoi(env)));
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
         // only the side streams the split uses: idle streams still take hardware queues,
@@ -1384,16 +1390,30 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         // frequency batch size from the device budget
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-        size_t room = free_b > (size_t)(2ull << 30) ? free_b - (size_t)(2ull << 30) : free_b / 2;
-        if (ctx->budget) room = std::min(room, ctx->budget > ctx->held ? ctx->budget - ctx->held : 0);
+        // the batch buffers this context already holds are reused or released
+        // before a larger allocation: they count as room
+        size_t reusable = 0;
+        for (const mgw_hip_ctx::Buf* b : {&ctx->hA, &ctx->hS, &ctx->hdinv, &ctx->hY, &ctx->hfr,
+                                          &ctx->hfc, &ctx->htbuf, &ctx->hqpart, &ctx->mhi,
+                                          &ctx->mlo, &ctx->mscale})
+            reusable += b->bytes;
+        const size_t avail = free_b + reusable;
+        size_t room = avail > (size_t)(2ull << 30) ? avail - (size_t)(2ull << 30) : avail / 2;
+        if (ctx->budget) {
+            const size_t fixed = ctx->held - reusable;
+            room = std::min(room, ctx->budget > fixed ? ctx->budget - fixed : 0);
+        }
         const size_t per = cd ? batch_bytes(fp, mpad, MGW_PREC_FP64) + (size_t)np * 8
                               : batch_bytes(np, mpad, job->precision, cplx);
         int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
-        nb = std::min(nb, cd ? 256 : 64);
+        nb = std::min(nb, cd ? std::max(256, ctx->batch_cap) : ctx->batch_cap);
         const Tuned tn = tuned(nmo, nocc, ns, job->precision);
         const int pick = ctx->forced_nb ? ctx->forced_nb : tn.nb;
         if (pick > 0) nb = std::min(nb, pick);
         if (ctx->max_nb > 0) nb = std::min(nb, ctx->max_nb);
+        // equal batches: the factorization's per-batch latency is paid once per
+        // batch whatever its size, so no short last batch
+        nb = (nfreq + (nfreq + nb - 1) / nb - 1) / ((nfreq + nb - 1) / nb);
         // untuned unit shape: measured at nmo=200, 16 frequencies per unit for
         // the int8 contraction (1-1.5 % faster than 4), 4 for the FP16 split
         const int uf = ctx->forced_uf ? ctx->forced_uf

@@ -111,7 +111,8 @@ __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* _
         Tile64 tile;
         tile.zero();
         const double* Lk = Aw + (size_t)k * kNB * np;
-        gemm64<false>(Lk, np, Lk, np, (k - 1) * kNB, k * kNB, tile, As, Bs);
+        gemm64_k64<false>(Lk + (size_t)(k - 1) * kNB, np, Lk + (size_t)(k - 1) * kNB, np, tile, As,
+                          Bs);
         tile.for_each([&](int r, int c, double v) { upd[r][c] = v; });
         __syncthreads();
     }
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void k_trsm_panel(double* __restrict__ A,
     const double* Aik = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
     Tile64 tile;
     tile.zero();
-    gemm64<false>(Aik, np, D, 64, 0, kNB, tile, As, Bs);
+    gemm64_k64<false>(Aik, np, D, 64, tile, As, Bs);
     double* out = Aw + (size_t)i * kNB * np + (size_t)k * kNB;
     tile.for_each([&](int r, int c, double v) { out[(size_t)r * np + c] = v; });
 }

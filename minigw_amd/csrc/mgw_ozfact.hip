@@ -44,14 +44,24 @@ __device__ void slice_block(const double (*blk)[66], double* rsc, double* csc, s
                             int8_t* __restrict__ RP, double* __restrict__ RS,
                             int8_t* __restrict__ CP, double* __restrict__ CS) {
     const int t = threadIdx.x;
-    if (t < 64) {
-        double m = 0.0;
-        for (int c = 0; c < 64; ++c) m = fmax(m, fabs(blk[t][c]));
-        rsc[t] = slice_scale(m);
-    } else if (t < 128 && CP) {
-        double m = 0.0;
-        for (int r = 0; r < 64; ++r) m = fmax(m, fabs(blk[r][t - 64]));
-        csc[t - 64] = slice_scale(m);
+    {
+        // row (column) maxima: four consecutive lanes per row (column), 16 entries each
+        const int line = t >> 2, part = (t & 3) * 16;
+        double mr = 0.0, mc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            mr = fmax(mr, fabs(blk[line][part + c]));
+            if (CP) mc = fmax(mc, fabs(blk[part + c][line]));
+        }
+#pragma unroll
+        for (int d = 1; d < 4; d <<= 1) {
+            mr = fmax(mr, __shfl_xor(mr, d));
+            if (CP) mc = fmax(mc, __shfl_xor(mc, d));
+        }
+        if ((t & 3) == 0) {
+            rsc[line] = slice_scale(mr);
+            if (CP) csc[line] = slice_scale(mc);
+        }
     }
     __syncthreads();
     if (RP) {
@@ -99,12 +109,12 @@ __global__ __launch_bounds__(256) void k_trsm_slice(double* __restrict__ A,
     Tile64 tile;
     tile.zero();
     if (pre)
-        gemm64<false>(Aw + (size_t)i * kNB * np, np, Aw + (size_t)k * kNB * np, np, (k - 1) * kNB,
-                      k * kNB, tile, As, Bs);
+        gemm64_k64<false>(Aw + (size_t)i * kNB * np + (size_t)(k - 1) * kNB, np,
+                          Aw + (size_t)k * kNB * np + (size_t)(k - 1) * kNB, np, tile, As, Bs);
     tile.for_each([&](int r, int c, double v) { blk[r][c] = Aik[(size_t)r * np + c] - v; });
     __syncthreads();
     tile.zero();
-    gemm64<false>(&blk[0][0], 66, dinv + ((size_t)w * nblk + k) * 4096, 64, 0, kNB, tile, As, Bs);
+    gemm64_k64<false>(&blk[0][0], 66, dinv + ((size_t)w * nblk + k) * 4096, 64, tile, As, Bs);
     tile.for_each([&](int r, int c, double v) {
         Aik[(size_t)r * np + c] = v;
         blk[r][c] = v;
@@ -153,12 +163,12 @@ __global__ __launch_bounds__(256) void k_trtri_m_slice(double* __restrict__ A,
         Tile64 tile;
         tile.zero();
         if (pre)      // M_i,j+1 (row block i, column block j+1) times the saved L_j+1,j
-            gemm64<true>(Aw + (size_t)i * kNB * np + (size_t)(j + 1) * kNB, np,
-                         Tw + (size_t)j * 4096, 64, 0, kNB, tile, As, Bs);
+            gemm64_k64<true>(Aw + (size_t)i * kNB * np + (size_t)(j + 1) * kNB, np,
+                             Tw + (size_t)j * 4096, 64, tile, As, Bs);
         tile.for_each([&](int r, int c, double v) { blk[r][c] = T[r * 64 + c] + v; });
         __syncthreads();
         tile.zero();
-        gemm64<true>(&blk[0][0], 66, D, 64, 0, kNB, tile, As, Bs);
+        gemm64_k64<true>(&blk[0][0], 66, D, 64, tile, As, Bs);
         tile.for_each([&](int r, int c, double v) {
             Aij[(size_t)r * np + c] = -v;
             blk[r][c] = -v;

@@ -125,5 +125,55 @@ __device__ inline void gemm64(const double* __restrict__ a, size_t lda,
     __syncthreads();
 }
 
+// gemm64 over exactly one 64-wide K block (a and b already offset to it): the
+// whole of both operands is fetched into registers at once, so the block
+// waits for memory once instead of once per K slab, then staged through LDS
+// slab by slab.  For the single-block products of the panel kernels, whose
+// time is the latency of these loads.
+template <bool NN>
+__device__ inline void gemm64_k64(const double* __restrict__ a, size_t lda,
+                                  const double* __restrict__ b, size_t ldb, Tile64& tile,
+                                  double (*As)[64 + kLdsPad], double (*Bs)[64 + kLdsPad]) {
+    constexpr int kSlabs = 64 / kBK;
+    const int t = threadIdx.x;
+    const int am = t >> 2, akq = (t & 3) * 4;
+    const int bk = t >> 4, bnq = (t & 15) * 4;
+    double2 ra[kSlabs][2], rb[kSlabs][2];
+#pragma unroll
+    for (int s = 0; s < kSlabs; ++s) {
+        const int k0 = s * kBK;
+        const double2* pa = reinterpret_cast<const double2*>(a + (size_t)am * lda + k0 + akq);
+        ra[s][0] = pa[0];
+        ra[s][1] = pa[1];
+        const double2* pb =
+            NN ? reinterpret_cast<const double2*>(b + (size_t)(k0 + bk) * ldb + bnq)
+               : reinterpret_cast<const double2*>(b + (size_t)am * ldb + k0 + akq);
+        rb[s][0] = pb[0];
+        rb[s][1] = pb[1];
+    }
+#pragma unroll
+    for (int s = 0; s < kSlabs; ++s) {
+        __syncthreads();
+        As[akq + 0][am] = ra[s][0].x;
+        As[akq + 1][am] = ra[s][0].y;
+        As[akq + 2][am] = ra[s][1].x;
+        As[akq + 3][am] = ra[s][1].y;
+        if (NN) {
+            Bs[bk][bnq + 0] = rb[s][0].x;
+            Bs[bk][bnq + 1] = rb[s][0].y;
+            Bs[bk][bnq + 2] = rb[s][1].x;
+            Bs[bk][bnq + 3] = rb[s][1].y;
+        } else {
+            Bs[akq + 0][am] = rb[s][0].x;
+            Bs[akq + 1][am] = rb[s][0].y;
+            Bs[akq + 2][am] = rb[s][1].x;
+            Bs[akq + 3][am] = rb[s][1].y;
+        }
+        __syncthreads();
+        tile.step(As, Bs);
+    }
+    __syncthreads();
+}
+
 }  // namespace
 }  // namespace mgw
