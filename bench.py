@@ -45,7 +45,7 @@ def contraction_bytes(n: int, m: int, nb: float, form: str) -> float:
     operands once each in the form's representation (the triangle of M per
     frequency, the panel X once per launch) and the per-row-tile partial sums
     written (128-row tiles)."""
-    per = {"mixed_int8": 5.0, "mixed": 4.0, "fp64": 8.0, "fp64_cd": 8.0}[form]
+    per = {"mixed_int8": 5.0, "mixed_int8x4": 4.0, "mixed": 4.0, "fp64": 8.0, "fp64_cd": 8.0}[form]
     return per * (n * (n + 1) / 2 * nb + n * m) + 8.0 * (n / 128) * m * nb
 
 
@@ -256,14 +256,17 @@ FORMS = {
     # name: (precision code, contraction kernel, products per element, vendor peak,
     #        measured-peak key, dtype label)
     "mixed": (1, "k_contract_split<FP16>", 3, 2.5e15, "fp16_mfma_32x32", "fp16x3-split+fp64"),
-    "mixed_int8": (2, "k_contract_i8", 15, 5.0e15, "int8_mfma_32x32", "int8x5-slice+fp64"),
+    "mixed_int8": (2, "k_contract_i8<5, 7>", 15, 5.0e15, "int8_mfma_32x32", "int8x5-slice+fp64"),
+    # four balanced 8-bit digits per operand: 10 digit products
+    "mixed_int8x4": (4, "k_contract_i8<4, 8>", 10, 5.0e15, "int8_mfma_32x32", "int8x4-digit+fp64"),
     "fp64": (0, "k_contract_f64", 1, 78.6e12, "fp64_mfma", "fp64"),
     # FP64 with the ERI decomposed on the device (pair-space screening, rank r)
     "fp64_cd": (3, "k_contract_f64", 1, 78.6e12, "fp64_mfma", "fp64"),
 }
 
 
-RANDOM_PEAK = {"mixed_int8": "int8_mfma_32x32_random", "fp64": "fp64_mfma_random",
+RANDOM_PEAK = {"mixed_int8": "int8_mfma_32x32_random", "mixed_int8x4": "int8_mfma_32x32_random",
+               "fp64": "fp64_mfma_random",
                "fp64_cd": "fp64_mfma_random"}
 
 
@@ -310,8 +313,8 @@ def main() -> int:
     ap.add_argument("--config", default="nmo200", choices=["nmo24", "nmo120", "nmo200"])
     ap.add_argument("--nfreq", type=int, default=200)
     ap.add_argument("--npar", type=int, default=16)
-    ap.add_argument("--precision", default="mixed_int8", choices=list(FORMS))
-    ap.add_argument("--alt-precision", default="mixed", choices=list(FORMS) + ["none"],
+    ap.add_argument("--precision", default="mixed_int8x4", choices=list(FORMS))
+    ap.add_argument("--alt-precision", default="mixed_int8", choices=list(FORMS) + ["none"],
                     help="second mixed form, timed and checked against FP64 beside the headline one")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cd-steps", type=int, default=3,

@@ -52,7 +52,7 @@ enum mgw_precision {
     MGW_PREC_MIXED_INT8 = 2, /* contraction on int8 MFMA: 5 slices per operand,   */
                              /* exact int32 products, FP64 epilogue (~1e-10       */
                              /* relative in Sigma_c); factorization as MIXED      */
-    MGW_PREC_FP64_CD = 3     /* FP64 throughout, with the ERI decomposed on the   */
+    MGW_PREC_FP64_CD = 3,    /* FP64 throughout, with the ERI decomposed on the   */
                              /* device (pivoted Cholesky over unique pairs, to    */
                              /* mgw_hip_set_cd_tolerance): the dielectric is      */
                              /* factored in the rank-r pair space (r x r instead  */
@@ -61,6 +61,11 @@ enum mgw_precision {
                              /* UNSUPPORTED) and eta == 0; no counterpart in the  */
                              /* C++ reference (its PySCF companion's auxiliary-   */
                              /* basis form, pyscf_prep/gw_ac.py:169-219)          */
+    MGW_PREC_MIXED_INT8_4 = 4 /* as MIXED_INT8 with four balanced 8-bit digits per */
+                             /* operand (10 int8 products instead of 15) in the  */
+                             /* contraction and the factorization's block        */
+                             /* products (~2^-32 of a row's largest element,     */
+                             /* ~1e-9 relative in Sigma_c); nph <= 32512         */
 };
 
 /* ---- library / context ------------------------------------------------ */
@@ -232,11 +237,12 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
 /* Error judge and FP64 fallback of the mixed forms (no counterpart in the
  * reference, whose only failure check is the factorization's devInfo,
  * src/workspace/device_screening_workspace.cu:324-334).  With nsample > 0,
- * every MGW_PREC_MIXED / MGW_PREC_MIXED_INT8 run of mgw_hip_g0w0 on ctx
- * recomputes nsample of its frequencies (evenly spaced) with every step in
- * FP64 and compares their quadratic forms with the form's: the error is the
- * largest |q_form - q_fp64| over the largest |q_fp64| of a frequency.  When
- * it exceeds tol (tol = 0: 1e-7 for MIXED_INT8, 1e-5 for MIXED) or the
+ * every MGW_PREC_MIXED / MGW_PREC_MIXED_INT8 / MGW_PREC_MIXED_INT8_4 run of
+ * mgw_hip_g0w0 on ctx recomputes nsample of its frequencies (evenly spaced)
+ * with every step in FP64 and compares their quadratic forms with the
+ * form's: the error is the largest |q_form - q_fp64| over the largest
+ * |q_fp64| of a frequency.  When it exceeds tol (tol = 0: 1e-7 for
+ * MIXED_INT8, 1e-6 for MIXED_INT8_4, 1e-5 for MIXED) or the
  * form's factorization breaks down, and fallback != 0, the call reruns the
  * job in MGW_PREC_FP64 and returns that result.  nsample = 0 turns it off. */
 int mgw_hip_set_judge(mgw_hip_ctx* ctx, int32_t nsample, double tol, int32_t fallback);

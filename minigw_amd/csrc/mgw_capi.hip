@@ -397,6 +397,16 @@ PhLayout ph_layout(int nmo, int nocc) {
     return L;
 }
 
+// the forms whose contraction runs on int8 MFMA, and their digit scheme
+bool int8_form(int precision) {
+    return precision == MGW_PREC_MIXED_INT8 || precision == MGW_PREC_MIXED_INT8_4;
+}
+// the forms whose factorization's block products run on int8 MFMA
+bool mixed_form(int precision) { return precision == MGW_PREC_MIXED || int8_form(precision); }
+int digit_scheme(int precision) {
+    return precision == MGW_PREC_MIXED_INT8_4 ? mgw::kDigits4x8 : mgw::kDigits5x7;
+}
+
 size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
     const size_t nblk = np / mgw::kNB;
     if (cplx)                                // E/L, S, dinv, Y/Z complex; qpart unused
@@ -409,9 +419,10 @@ size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
                + (size_t)(np / mgw::kCT) * mpad * 8  // qpart
                + 4;                          // info
     if (precision == MGW_PREC_MIXED) b += (size_t)np * np * 4 + (size_t)np * 8;  // M planes
-    if (precision == MGW_PREC_MIXED_INT8) b += (size_t)np * np * mgw::kSlices + (size_t)np * 8;
-    if (precision == MGW_PREC_MIXED || precision == MGW_PREC_MIXED_INT8)   // factor slices, 2nd trtri slot
-        b += 2 * mgw::fact_slice_bytes(np) + (size_t)np * 64 * 8;
+    if (int8_form(precision))
+        b += (size_t)np * np * mgw::i8_digits(digit_scheme(precision)) + (size_t)np * 8;
+    if (mixed_form(precision))               // factor slices, 2nd trtri slot
+        b += 2 * mgw::fact_slice_bytes(np, digit_scheme(precision)) + (size_t)np * 64 * 8;
     return b;
 }
 
@@ -423,24 +434,26 @@ size_t batch_bytes(int np, int mpad, int precision, bool cplx = false) {
 // slot, 2 slots with int8 products), so the batch splits into parts of
 // consecutive frequencies that run on their own streams.
 void factor_batch(mgw_hip_ctx* ctx, double* A, double* dinv, int* info, double* tbuf, int np,
-                  int nb, char* rslices = nullptr, char* cslices = nullptr) {
+                  int nb, char* rslices = nullptr, char* cslices = nullptr,
+                  int scheme = mgw::kDigits5x7) {
     const size_t nblk = (size_t)(np / mgw::kNB);
     const bool i8 = rslices && cslices;
-    const size_t planes = (size_t)nb * nblk * nblk * mgw::kSlices * 4096;
+    const size_t digits = (size_t)mgw::i8_digits(scheme);
+    const size_t planes = (size_t)nb * nblk * nblk * digits * 4096;
     auto run = [&](int f0, int count, hipStream_t s) {
         double* Af = A + (size_t)f0 * np * np;
         double* Df = dinv + (size_t)f0 * nblk * 4096;
         int* If = info + f0;
         double* Tf = tbuf + (size_t)f0 * (i8 ? 2 : 1) * np * mgw::kNB;
         if (i8) {
-            int8_t* RP = reinterpret_cast<int8_t*>(rslices) + (size_t)f0 * nblk * nblk * mgw::kSlices * 4096;
-            int8_t* CP = reinterpret_cast<int8_t*>(cslices) + (size_t)f0 * nblk * nblk * mgw::kSlices * 4096;
+            int8_t* RP = reinterpret_cast<int8_t*>(rslices) + (size_t)f0 * nblk * nblk * digits * 4096;
+            int8_t* CP = reinterpret_cast<int8_t*>(cslices) + (size_t)f0 * nblk * nblk * digits * 4096;
             // scale exponents: one int per row of each 64 x 64 block
             double* RS = reinterpret_cast<double*>(rslices + planes + (size_t)f0 * nblk * nblk * 64 * 4);
             double* CS = reinterpret_cast<double*>(cslices + planes + (size_t)f0 * nblk * nblk * 64 * 4);
-            hip_check(mgw::launch_potrf_i8(Af, Df, If, RP, RS, CP, CS, np, count, s),
+            hip_check(mgw::launch_potrf_i8(Af, Df, If, RP, RS, CP, CS, np, count, scheme, s),
                       "potrf (int8 products) launch");
-            hip_check(mgw::launch_trtri_i8(Af, Df, Tf, RP, RS, CP, CS, np, count, s),
+            hip_check(mgw::launch_trtri_i8(Af, Df, Tf, RP, RS, CP, CS, np, count, scheme, s),
                       "trtri (int8 products) launch");
         } else {
             hip_check(mgw::launch_potrf(Af, Df, If, np, count, s), "potrf launch");
@@ -1218,7 +1231,8 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             fail(MGW_ERR_UNSUPPORTED,
                  "eta != 0 (complex screening) runs in MGW_PREC_FP64 only on the hip backend");
         need(job->precision == MGW_PREC_FP64 || job->precision == MGW_PREC_MIXED ||
-                 job->precision == MGW_PREC_MIXED_INT8 || job->precision == MGW_PREC_FP64_CD,
+                 job->precision == MGW_PREC_MIXED_INT8 || job->precision == MGW_PREC_FP64_CD ||
+                 job->precision == MGW_PREC_MIXED_INT8_4,
              "mgw_hip_g0w0: unknown precision %d", job->precision);
         const bool cd = job->precision == MGW_PREC_FP64_CD;
         std::vector<int> states;
@@ -1333,7 +1347,9 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                         .count();
         }
         const bool mixed = job->precision == MGW_PREC_MIXED;
-        const bool i8 = job->precision == MGW_PREC_MIXED_INT8;
+        const bool i8 = int8_form(job->precision);
+        const int scheme = digit_scheme(job->precision);
+        const size_t digits = (size_t)mgw::i8_digits(scheme);
         const mgw::Split16 fmt = mgw::Split16::FP16;
         uint16_t *xh = nullptr, *xl = nullptr;
         double* xs = nullptr;
@@ -1357,12 +1373,14 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                       "split X");
         }
         if (i8) {
-            need(np <= mgw::kMaxK, "mgw_hip_g0w0: nph %d exceeds the int8 contraction's limit %d", n,
-                 mgw::kMaxK);
-            xh = as<uint16_t>(ctx->get(ctx->xhi, (size_t)mgw::kSlices * mpad * np, "X slices"));
+            need(np <= mgw::i8_max_k(scheme),
+                 "mgw_hip_g0w0: nph %d exceeds the int8 contraction's limit %d", n,
+                 mgw::i8_max_k(scheme));
+            xh = as<uint16_t>(ctx->get(ctx->xhi, digits * mpad * np, "X slices"));
             xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
             hip_check(mgw::launch_slice_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p),
-                                          reinterpret_cast<int8_t*>(xh), xs, n, np, mpad, st),
+                                          reinterpret_cast<int8_t*>(xh), xs, n, np, mpad, scheme,
+                                          st),
                       "slice X");
         }
         // the error judge's FP64 X (mixed forms), before the batch takes the room
@@ -1419,7 +1437,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const int uf = ctx->forced_uf ? ctx->forced_uf
                        : ctx->split_uf ? ctx->split_uf
                        : tn.uf ? tn.uf
-                       : job->precision == MGW_PREC_MIXED_INT8 ? 16
+                       : i8 ? 16
                                                                 : 4;
         const size_t zf = cplx ? 2 : 1;    // complex entries are two doubles
         const int fnblk = fp / mgw::kNB;
@@ -1430,8 +1448,10 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                          : nullptr;
         char *fr = nullptr, *fc = nullptr;
         if ((mixed || i8) && ctx->int8_factor) {       // the mixed forms' int8 block products
-            fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np), "factor row slices"));
-            fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np), "factor column slices"));
+            fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np, scheme),
+                                   "factor row slices"));
+            fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np, scheme),
+                                   "factor column slices"));
         }
         double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)(fr ? 2 : 1) * nb * fp * 64 * 8,
                                            "trtri scratch"));
@@ -1445,7 +1465,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
         }
         if (i8) {
-            mh = as<uint16_t>(ctx->get(ctx->mhi, (size_t)nb * mgw::kSlices * np * np, "M slices"));
+            mh = as<uint16_t>(ctx->get(ctx->mhi, (size_t)nb * digits * np * np, "M slices"));
             ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
         }
         hip_check(hipMemsetAsync(info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
@@ -1493,12 +1513,13 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
             hip_check(mgw::launch_dielectric(dV, S, A, np, b, st), "dielectric");
             if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-            factor_batch(ctx, A, dinv, info, tbuf, np, b, fr, fc);
+            factor_batch(ctx, A, dinv, info, tbuf, np, b, fr, fc, scheme);
             if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
             if (mixed)
                 hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
             if (i8)
-                hip_check(mgw::launch_slice_m(A, S, reinterpret_cast<int8_t*>(mh), ms, np, b, st),
+                hip_check(mgw::launch_slice_m(A, S, reinterpret_cast<int8_t*>(mh), ms, np, b, scheme,
+                                              st),
                           "slice M");
             if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
             if (mixed)
@@ -1508,7 +1529,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             else if (i8)
                 hip_check(mgw::launch_contract_i8(reinterpret_cast<const int8_t*>(mh), ms,
                                                   reinterpret_cast<const int8_t*>(xh), qpart, np,
-                                                  mpad, b, uf, st),
+                                                  mpad, b, uf, scheme, st),
                           "contract int8");
             else
                 hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");
@@ -1614,7 +1635,10 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         // |q_form - q_fp64| over the largest |q_fp64| of each sampled frequency.
         if ((mixed || i8) && ctx->judge_n > 0) {
             const int nj = std::min(ctx->judge_n, nfreq);
-            const double tol = ctx->judge_tol > 0.0 ? ctx->judge_tol : i8 ? 1e-7 : 1e-5;
+            const double tol = ctx->judge_tol > 0.0               ? ctx->judge_tol
+                               : job->precision == MGW_PREC_MIXED_INT8 ? 1e-7
+                               : i8                                   ? 1e-6
+                                                                      : 1e-5;
             double* qj = as<double>(ctx->get(ctx->hqj, (size_t)m * 8, "judge q"));
             std::vector<double> qa(m), qb(m);
             double worst = 0.0;
@@ -1703,14 +1727,13 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         ctx->fell_back = 0;
         ctx->judge_err = -1.0;
         ctx->judge_used = 0;
-        const bool mixed_form =
-            job->precision == MGW_PREC_MIXED || job->precision == MGW_PREC_MIXED_INT8;
+        const bool mixed = mixed_form(job->precision);
         bool done = false;
         try {
             done = g0w0_body(ctx, job, out);
         } catch (const MgwError& e) {
             // a mixed form's factorization breakdown falls back like a failed judge
-            if (!(mixed_form && ctx->judge_n > 0 && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC))
+            if (!(mixed && ctx->judge_n > 0 && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC))
                 throw;
             ctx->fell_back = 2;
         }
@@ -1946,7 +1969,7 @@ int mgw_hip_autotune(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, int32_t* best_ba
         // mixed forms: the contraction's unit shape (frequencies per 32-tile
         // unit sharing an XCD's L2) at the chosen batch
         int best_uf = 0;
-        if ((job->precision == MGW_PREC_MIXED || job->precision == MGW_PREC_MIXED_INT8) && best > 0) {
+        if (mixed_form(job->precision) && best > 0) {
             double best_c = 0.0;
             for (int uf : {4, 2, 8, 16}) {
                 mgw_g0w0_job j = *job;
@@ -2016,7 +2039,8 @@ size_t mgw_hip_estimate_bytes(int32_t nmo, int32_t nocc, int32_t nfreq, int32_t 
     const size_t sg = mgw::sigma_workspace_bytes(nmo, ns, nfreq);     // quadrature factors
     size_t fixed = eri + v + sg + (size_t)nfreq * m * 8 + (size_t)L.np * 12 + (size_t)mpad * 8;
     fixed += precision == MGW_PREC_MIXED        ? (size_t)mpad * L.np * 4 + (size_t)mpad * 8
-             : precision == MGW_PREC_MIXED_INT8 ? (size_t)mpad * L.np * mgw::kSlices + (size_t)mpad * 8
+             : int8_form(precision) ? (size_t)mpad * L.np * mgw::i8_digits(digit_scheme(precision)) +
+                                          (size_t)mpad * 8
                                                 : (size_t)mpad * L.np * 8;
     const int nb = std::max(1, std::min(nbatch > 0 ? nbatch : nfreq, nfreq));
     return fixed + (size_t)nb * batch_bytes(L.np, mpad, precision);

@@ -13,8 +13,14 @@ constexpr int kNB = 64;    // factorization block (potrf / trtri tiles)
 constexpr int kCT = 128;   // FP64 contraction tile: rows of M and columns of X per workgroup
 constexpr int kPad = 256;  // np and the contraction column count are padded to this
 constexpr int kMaxPade = 64;
-constexpr int kSlices = 5;       // int8 slices per operand of the integer-MFMA contraction
-constexpr int kMaxK = 65536;     // its largest np (keeps the int32 group sums exact)
+// Digit schemes of the integer-MFMA kernels (mgw_slices.h): five signed 7-bit
+// digits per operand (15 digit products, ~2^-35 of a row's largest element)
+// or four balanced 8-bit digits (10 products, ~2^-32).
+enum I8Digits : int { kDigits5x7 = 0, kDigits4x8 = 1 };
+inline int i8_digits(int scheme) { return scheme == kDigits4x8 ? 4 : 5; }
+// largest np of the integer contraction: keeps every int32 group sum exact
+// (5 x 2^12 x K < 2^31 for 7-bit digits, 4 x 2^14 x K < 2^31 for 8-bit ones)
+inline int i8_max_k(int scheme) { return scheme == kDigits4x8 ? 32512 : 65536; }
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -89,16 +95,17 @@ hipError_t launch_wc_from_dense(const double* W, const double* S, double* wc, in
 // Factorization of the mixed forms (mgw_ozfact.hip): the same blocked
 // Cholesky and inverse sweep with the block products on int8 MFMA.  RP/RS and
 // CP/CS hold the row- and column-scaled slices of every 64 x 64 block
-// (nb x fact_slice_bytes(np) bytes each: planes first, then nb x (np/64)^2 x
-// 64 scale exponents (int32 in 8-byte slots) at the byte offset
-// nb x (np/64)^2 x kSlices x 4096).
+// (nb x fact_slice_bytes(np, scheme) bytes each: planes first, then
+// nb x (np/64)^2 x 64 scale exponents (int32) at the byte offset
+// nb x (np/64)^2 x i8_digits(scheme) x 4096).
 // Both sweeps update two block columns per launch; launch_trtri_i8's tbuf
 // holds two slots of nb x np x 64 doubles.
-size_t fact_slice_bytes(int np);
+size_t fact_slice_bytes(int np, int scheme);
 hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
-                           double* CS, int np, int nb, hipStream_t s);
+                           double* CS, int np, int nb, int scheme, hipStream_t s);
 hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
-                           const int8_t* CP, const double* CS, int np, int nb, hipStream_t s);
+                           const int8_t* CP, const double* CS, int np, int nb, int scheme,
+                           hipStream_t s);
 
 // In-place inverse of the lower factor: A_w <- L_w^{-1} (lower, zeros above the
 // diagonal inside diagonal blocks).  tbuf: nb x np x 64 scratch.
@@ -200,18 +207,21 @@ hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const doub
 hipError_t launch_mfma_peak(int kind, int blocks, int iters, float* sink, hipStream_t s);
 hipError_t launch_stream_copy(const double* src, double* dst, size_t count, hipStream_t s);
 
-// Integer-MFMA form of the contraction (kSlices int8 slices per operand,
-// exact int32 products, FP64 epilogue).  X columns (as launch_split_x) and
-// the rows of M_w S_w are scaled by powers of two (xscale, mscale) and sliced
-// into kSlices planes of 128 x 32 byte blocks; launch_contract_i8 writes the
-// same qpart as the other contractions with np / 128 row tiles (reduce with
-// launch_reduce_q_scaled).  np and mpad multiples of 128, np <= kMaxK.
+// Integer-MFMA form of the contraction (i8_digits(scheme) int8 digits per
+// operand, exact int32 products, FP64 epilogue).  X columns (as
+// launch_split_x) and the rows of M_w S_w are scaled by powers of two
+// (xscale, mscale) and split into one plane per digit of 128 x 32 byte
+// blocks; launch_contract_i8 writes the same qpart as the other contractions
+// with np / 128 row tiles (reduce with launch_reduce_q_scaled).  np and mpad
+// multiples of 128, np <= i8_max_k(scheme).
 hipError_t launch_slice_x(const double* src, const int64_t* pkrow, const int* colidx,
-                          int8_t* planes, double* xscale, int n, int np, int mpad, hipStream_t s);
+                          int8_t* planes, double* xscale, int n, int np, int mpad, int scheme,
+                          hipStream_t s);
 hipError_t launch_slice_m(const double* M, const double* S, int8_t* planes, double* mscale, int np,
-                          int nb, hipStream_t s);
+                          int nb, int scheme, hipStream_t s);
 hipError_t launch_contract_i8(const int8_t* mplanes, const double* mscale, const int8_t* xplanes,
-                              double* qpart, int np, int mpad, int nb, int uf, hipStream_t s);
+                              double* qpart, int np, int mpad, int nb, int uf, int scheme,
+                              hipStream_t s);
 int contract_i8_row_tiles(int np);
 hipError_t launch_probe_i8(const int8_t* a, const int8_t* b, int* d, int ntests, hipStream_t s);
 

@@ -2,20 +2,18 @@
 //
 // q_c = -|| M x_c ||^2 with M = L^-1 S (lower triangular) and x_c an ERI
 // column.  Each row of M and each column of X is written as a power of two
-// times kSlices signed 7-bit slices,
-//     a = 2^e * sum_i t_i 2^-(6 + 7 i),   t_i integer in [-64, 64],
-// (the first slice takes 6 bits of a value below 2^e, every later one the
-// rounded remainder, so the representation error is below 2^(e - 6 - 7 S)).
-// z = M x is then sum over slice pairs (i, j), i + j <= S - 1 (0-based), of
-// 2^-(12 + 7 (i + j)) (T_i U_j): the products T_i U_j run on
-// v_mfma_i32_32x32x32_i8 and are exact in int32, pairs of equal i + j share
-// one accumulator, and the FP64 epilogue combines the S groups, undoes the row
-// scale and sums the squares of each column.  Unlike the 16-bit hi/lo split
-// (22 bits per element, FP32 accumulation) the only error is the truncation
-// of the slices, ~2^-35 of a row's largest element for 5 slices.
+// times S signed digits of B bits (mgw_slices.h: five 7-bit digits, or four
+// balanced 8-bit ones).  z = M x is then the sum over digit pairs (i, j),
+// i + j <= S - 1 (0-based), of 2^-(2 (B - 1) + B (i + j)) (T_i U_j): the
+// products T_i U_j run on v_mfma_i32_32x32x32_i8 and are exact in int32,
+// pairs of equal i + j share one accumulator, and the FP64 epilogue combines
+// the S groups, undoes the row scale and sums the squares of each column.
+// Unlike the 16-bit hi/lo split (22 bits per element, FP32 accumulation) the
+// only error is the truncation of the digits: ~2^-35 of a row's largest
+// element for 5 x 7 bits (15 products), ~2^-32 for 4 x 8 bits (10 products).
 //
-// Bounds: |t u| <= 2^12, at most kSlices pairs per group and K <= kMaxK
-// rows, so a group sum stays below 5 * 2^12 * 2^16 < 2^31.
+// Bounds: a group holds at most S pairs and K <= i8_max_k rows, so a group
+// sum stays below 5 * 2^12 * 2^16 (7-bit) or 4 * 2^14 * 2^15 (8-bit) <= 2^31.
 #include "mgw_kernels.h"
 #include "mgw_slices.h"
 
@@ -25,7 +23,6 @@ namespace mgw {
 namespace {
 
 
-constexpr int kS = kSlices;
 constexpr int kIBM = 128, kIBN = 128, kIBK = 32;   // workgroup tile and K step
 
 // LDS byte offset of the 16 B half `kh` (0 or 16) of row `row` in a 128 x 32
@@ -49,7 +46,8 @@ __device__ inline double wave_max(double v) {
 }
 
 
-// One wave per column c of X: gather, scale, slice.
+// One wave per column c of X: gather, scale, split into digits.
+template <int kS, int B>
 __global__ __launch_bounds__(256) void k_slice_x(const double* __restrict__ src,
                                                  const int64_t* __restrict__ pkrow,
                                                  const int* __restrict__ colidx,
@@ -61,12 +59,12 @@ __global__ __launch_bounds__(256) void k_slice_x(const double* __restrict__ src,
     const int64_t base = pkrow[c];
     double mx = 0.0;
     for (int k = lane; k < n; k += 64) mx = fmax(mx, fabs(src[base + colidx[k]]));
-    const double scale = slice_scale(wave_max(mx));
+    const double scale = digit_scale<B>(wave_max(mx));
     const size_t plane = (size_t)mpad * np;
     for (int k = lane; k < np; k += 64) {
         const double x = k < n ? src[base + colidx[k]] * scale : 0.0;
         int8_t t[kS];
-        slice_value(x, t);
+        digit_split<kS, B>(x, t);
         const size_t o = slice_index(c, k, np);
 #pragma unroll
         for (int i = 0; i < kS; ++i) planes[i * plane + o] = t[i];
@@ -77,6 +75,7 @@ __global__ __launch_bounds__(256) void k_slice_x(const double* __restrict__ src,
 // One wave per row r of M_w S_w (lower triangle, zeros above it up to the end of
 // the row's 128-row tile; the planes beyond, which the contraction never reads,
 // are left as they are).
+template <int kS, int B>
 __global__ __launch_bounds__(256) void k_slice_m(const double* __restrict__ M,
                                                  const double* __restrict__ S,
                                                  int8_t* __restrict__ planes,
@@ -88,7 +87,7 @@ __global__ __launch_bounds__(256) void k_slice_m(const double* __restrict__ M,
     const double* Sw = S + (size_t)w * np;
     double mx = 0.0;
     for (int k = lane; k <= r; k += 64) mx = fmax(mx, fabs(row[k] * Sw[k]));
-    const double scale = slice_scale(wave_max(mx));
+    const double scale = digit_scale<B>(wave_max(mx));
     const size_t plane = (size_t)np * np;
     int8_t* out = planes + (size_t)w * kS * plane;
     // the contraction reads a row tile's K range up to its own diagonal tile
@@ -96,7 +95,7 @@ __global__ __launch_bounds__(256) void k_slice_m(const double* __restrict__ M,
     for (int k = lane; k < kend; k += 64) {
         const double x = k <= r ? (row[k] * Sw[k]) * scale : 0.0;
         int8_t t[kS];
-        slice_value(x, t);
+        digit_split<kS, B>(x, t);
         const size_t o = slice_index(r, k, np);
 #pragma unroll
         for (int i = 0; i < kS; ++i) out[i * plane + o] = t[i];
@@ -128,11 +127,12 @@ __device__ inline I8Tile i8_tile(int L, int nrow, int ncol, int nb, int uf, int 
 
 // 128 x 128 tile, 512 threads = 8 waves in 4 (rows) x 2 (columns), each wave
 // 32 rows x 64 columns (two 32 x 32 blocks) with kS group accumulators per
-// block.  Per K step of 32 the 2 x kS slice blocks (40 KB) are loaded into
-// registers two steps ahead (5 x 16 B per thread) and stored to the LDS
+// block.  Per K step of 32 the 2 x kS digit blocks (8 kS KB) are loaded into
+// registers two steps ahead (kS x 16 B per thread) and stored to the LDS
 // buffer the next step reads while the current one is computed from the
 // other (two buffers, one barrier per step; rows swizzled by lds_off).  M is
 // lower triangular, so the K loop stops at r0 + 128.
+template <int kS, int B>
 __global__ __launch_bounds__(512) void k_contract_i8(
     const int8_t* __restrict__ mplanes, const double* __restrict__ mscale,
     const int8_t* __restrict__ xplanes, double* __restrict__ qpart, int np, int mpad, int nb,
@@ -145,17 +145,18 @@ __global__ __launch_bounds__(512) void k_contract_i8(
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
 
-    // loader: chunk u of thread t -> global chunk g = t + 512 u of 2560:
-    // operand g / 1280 (0 = M, 1 = X), slice (g % 1280) / 256, row (g % 256) / 2,
-    // k half (g % 2) * 16
+    // loader: chunk u of thread t -> global chunk g = t + 512 u of 512 kS:
+    // operand g / (256 kS) (0 = M, 1 = X), digit (g % (256 kS)) / 256,
+    // row (g % 256) / 2, k half (g % 2) * 16
+    constexpr int kOpChunks = 256 * kS;
     const size_t mplane = (size_t)np * np, xplane = (size_t)mpad * np;
-    const int8_t* src[5];
-    int lop[5], lsl[5], lrow[5], lkh[5], lofs[5];
+    const int8_t* src[kS];
+    int lop[kS], lsl[kS], lrow[kS], lkh[kS], lofs[kS];
 #pragma unroll
-    for (int u = 0; u < 5; ++u) {
+    for (int u = 0; u < kS; ++u) {
         const int g = t + 512 * u;
-        lop[u] = g / 1280;
-        lsl[u] = (g % 1280) / 256;
+        lop[u] = g / kOpChunks;
+        lsl[u] = (g % kOpChunks) / 256;
         lrow[u] = (g % 256) >> 1;
         lkh[u] = (g & 1) * 16;
         lofs[u] = lds_off(lrow[u], lkh[u]);
@@ -165,14 +166,14 @@ __global__ __launch_bounds__(512) void k_contract_i8(
     }
     const int kend = min(np, r0 + kIBM);            // multiple of 32, >= 128
     const int nsteps = kend / kIBK;
-    u32x4 pre[5];
+    u32x4 pre[kS];
 #pragma unroll
-    for (int u = 0; u < 5; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u]);
+    for (int u = 0; u < kS; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u]);
 #pragma unroll
-    for (int u = 0; u < 5; ++u)
+    for (int u = 0; u < kS; ++u)
         *reinterpret_cast<u32x4*>(&lds[0][lop[u]][lsl[u]][lofs[u]]) = pre[u];
 #pragma unroll
-    for (int u = 0; u < 5; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u] + 4096);
+    for (int u = 0; u < kS; ++u) pre[u] = *reinterpret_cast<const u32x4*>(src[u] + 4096);
     __syncthreads();
 
     i32x16 acc[kS][2];
@@ -206,16 +207,16 @@ __global__ __launch_bounds__(512) void k_contract_i8(
         // previous barrier), then fetch stage st + 2 (the last steps re-read
         // the final stage, in bounds)
 #pragma unroll
-        for (int u = 0; u < 5; ++u)
+        for (int u = 0; u < kS; ++u)
             *reinterpret_cast<u32x4*>(&lds[cur ^ 1][lop[u]][lsl[u]][lofs[u]]) = pre[u];
         const int kn = min(st + 2, nsteps - 1);
 #pragma unroll
-        for (int u = 0; u < 5; ++u)
+        for (int u = 0; u < kS; ++u)
             pre[u] = *reinterpret_cast<const u32x4*>(src[u] + (size_t)kn * 4096);
         __syncthreads();
     }
 
-    // epilogue: combine the groups (Horner in 2^-7), undo the row scale, sum
+    // epilogue: combine the groups (Horner in 2^-B), undo the row scale, sum
     // the squares of each column over the tile's rows
     __syncthreads();
     double* red = reinterpret_cast<double*>(&lds[0][0][0][0]);      // [4][128] doubles
@@ -228,8 +229,8 @@ __global__ __launch_bounds__(512) void k_contract_i8(
         for (int x = 0; x < 2; ++x) {
             double z = (double)acc[kS - 1][x][r];
 #pragma unroll
-            for (int g = kS - 2; g >= 0; --g) z = z * 0x1p-7 + (double)acc[g][x][r];
-            z *= 0x1p-12 * inv;
+            for (int g = kS - 2; g >= 0; --g) z = z * group_step<B>() + (double)acc[g][x][r];
+            z *= group0_weight<B>() * inv;
             colsum[x] += z * z;
         }
     }
@@ -259,23 +260,28 @@ __global__ __launch_bounds__(64) void k_probe_i8(const u32x4* __restrict__ a,
 }  // namespace
 
 hipError_t launch_slice_x(const double* src, const int64_t* pkrow, const int* colidx,
-                          int8_t* planes, double* xscale, int n, int np, int mpad, hipStream_t s) {
+                          int8_t* planes, double* xscale, int n, int np, int mpad, int scheme,
+                          hipStream_t s) {
     if (np % kIBM != 0 || mpad % kIBN != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_slice_x, dim3((mpad + 3) / 4), dim3(256), 0, s, src, pkrow, colidx, planes,
+    auto kern = scheme == kDigits4x8 ? k_slice_x<4, 8> : k_slice_x<5, 7>;
+    hipLaunchKernelGGL(kern, dim3((mpad + 3) / 4), dim3(256), 0, s, src, pkrow, colidx, planes,
                        xscale, n, np, mpad);
     return hipGetLastError();
 }
 
 hipError_t launch_slice_m(const double* M, const double* S, int8_t* planes, double* mscale, int np,
-                          int nb, hipStream_t s) {
+                          int nb, int scheme, hipStream_t s) {
     if (np % kIBM != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_slice_m, dim3((np + 3) / 4, nb), dim3(256), 0, s, M, S, planes, mscale, np);
+    auto kern = scheme == kDigits4x8 ? k_slice_m<4, 8> : k_slice_m<5, 7>;
+    hipLaunchKernelGGL(kern, dim3((np + 3) / 4, nb), dim3(256), 0, s, M, S, planes, mscale, np);
     return hipGetLastError();
 }
 
 hipError_t launch_contract_i8(const int8_t* mplanes, const double* mscale, const int8_t* xplanes,
-                              double* qpart, int np, int mpad, int nb, int uf, hipStream_t s) {
-    if (np % kIBM != 0 || mpad % kIBN != 0 || nb <= 0 || np > kMaxK) return hipErrorInvalidValue;
+                              double* qpart, int np, int mpad, int nb, int uf, int scheme,
+                              hipStream_t s) {
+    if (np % kIBM != 0 || mpad % kIBN != 0 || nb <= 0 || np > i8_max_k(scheme))
+        return hipErrorInvalidValue;
     if (uf != 1 && uf != 2 && uf != 4 && uf != 8 && uf != 16 && uf != 32) return hipErrorInvalidValue;
     const int ncol = mpad / kIBN;
     int uc = 32 / uf;
@@ -284,8 +290,9 @@ hipError_t launch_contract_i8(const int8_t* mplanes, const double* mscale, const
     const long units = (long)(np / kIBM) * ((nb + uf - 1) / uf) * ((ncol + uc - 1) / uc);
     const long wgs = (units + 7) / 8 * 8 * uf * uc;
     if (wgs > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_contract_i8, dim3((unsigned)wgs), dim3(512), 0, s, mplanes, mscale, xplanes,
-                       qpart, np, mpad, nb, uf, uc);
+    auto kern = scheme == kDigits4x8 ? k_contract_i8<4, 8> : k_contract_i8<5, 7>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)wgs), dim3(512), 0, s, mplanes, mscale, xplanes, qpart,
+                       np, mpad, nb, uf, uc);
     return hipGetLastError();
 }
 

@@ -2,15 +2,16 @@
 //
 // The blocked Cholesky's trailing updates (A_ik -= sum_{j<k} L_ij L_kj^T) and
 // the inverse factor's sweeps (T_i = sum_{k=j+1}^{i} M_ik L_kj) are products
-// of 64 x 64 blocks.  Each finished block is cut once into kSlices int8
-// slices (mgw_slices.h) with a power-of-two scale per row (per column for the
-// L_kj operand of the sweeps, stored transposed); a product over one 64-wide
-// K block runs as the slice products with i + j < kSlices on
-// v_mfma_i32_32x32x32_i8 (exact in int32), and the K blocks, whose scales
-// differ, are combined in FP64.  The diagonal blocks (factor and inverse in
-// LDS), the panel solves and M_ij = -T_i L_jj^-1 stay FP64.
-// The truncation error per block product is ~2^-35 of the largest element of
-// each row segment; the contraction that follows carries the same order.
+// of 64 x 64 blocks.  Each finished block is cut once into S int8 digits
+// (mgw_slices.h: five 7-bit digits or four balanced 8-bit ones) with a
+// power-of-two scale per row (per column for the L_kj operand of the sweeps,
+// stored transposed); a product over one 64-wide K block runs as the digit
+// products with i + j < S on v_mfma_i32_32x32x32_i8 (exact in int32), and the
+// K blocks, whose scales differ, are combined in FP64.  The diagonal blocks
+// (factor and inverse in LDS), the panel solves and M_ij = -T_i L_jj^-1 stay
+// FP64.  The truncation error per block product is ~2^-35 (5 x 7 bits) or
+// ~2^-32 (4 x 8 bits) of the largest element of each row segment; the
+// contraction that follows carries the same order.
 //
 // The sweeps take two block columns per launch; the one block product of the
 // second column that the launch cannot include (kb = k for column k + 1 of
@@ -27,9 +28,9 @@
 namespace mgw {
 namespace {
 
-constexpr int kS = kSlices;
-static_assert(kSlices == 5, "the int32 group combination of k_gemm_i8blk assumes 5 slices");
-constexpr size_t kBlkBytes = (size_t)kS * 4096;     // slices of one 64 x 64 block
+// digit planes of one 64 x 64 block
+template <int kS>
+constexpr size_t blk_bytes() { return (size_t)kS * 4096; }
 
 __device__ inline size_t blk_index(int w, int nblk, int bi, int bj) {
     return ((size_t)w * nblk + bi) * nblk + bj;
@@ -38,8 +39,9 @@ __device__ inline size_t blk_index(int w, int nblk, int bi, int bj) {
 // Slices of the 64 x 64 block held in LDS (blk) at block index bix: row-scaled
 // into RP/RS (when RP != NULL) and, transposed, column-scaled into CP/CS
 // (when CP != NULL).  RS / CS hold, as int32, the exponents e of the inverse
-// scales 2^e (the factor that restores a row from its slices).  256 threads;
+// scales 2^e (the factor that restores a row from its digits).  256 threads;
 // blk is complete (barrier before the call).
+template <int kS, int B>
 __device__ void slice_block(const double (*blk)[66], double* rsc, double* csc, size_t bix,
                             int8_t* __restrict__ RP, double* __restrict__ RS,
                             int8_t* __restrict__ CP, double* __restrict__ CS) {
@@ -59,28 +61,28 @@ __device__ void slice_block(const double (*blk)[66], double* rsc, double* csc, s
             if (CP) mc = fmax(mc, __shfl_xor(mc, d));
         }
         if ((t & 3) == 0) {
-            rsc[line] = slice_scale(mr);
-            if (CP) csc[line] = slice_scale(mc);
+            rsc[line] = digit_scale<B>(mr);
+            if (CP) csc[line] = digit_scale<B>(mc);
         }
     }
     __syncthreads();
     if (RP) {
-        int8_t* out = RP + bix * kBlkBytes;
+        int8_t* out = RP + bix * blk_bytes<kS>();
         for (int e = t; e < 4096; e += 256) {
             const int r = e >> 6, c = e & 63;
             int8_t sv[kS];
-            slice_value(blk[r][c] * rsc[r], sv);
+            digit_split<kS, B>(blk[r][c] * rsc[r], sv);
 #pragma unroll
             for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = sv[i];
         }
         if (t < 64) reinterpret_cast<int*>(RS)[bix * 64 + t] = ilogb(1.0 / rsc[t]);   // e of 2^e
     }
     if (CP) {
-        int8_t* out = CP + bix * kBlkBytes;
+        int8_t* out = CP + bix * blk_bytes<kS>();
         for (int e = t; e < 4096; e += 256) {
             const int c = e >> 6, r = e & 63;       // transposed: row c of the output
             int8_t sv[kS];
-            slice_value(blk[r][c] * csc[c], sv);
+            digit_split<kS, B>(blk[r][c] * csc[c], sv);
 #pragma unroll
             for (int i = 0; i < kS; ++i) out[(size_t)i * 4096 + e] = sv[i];
         }
@@ -93,6 +95,7 @@ __device__ void slice_block(const double (*blk)[66], double* rsc, double* csc, s
 // two-column sweep left out), then L_ik = A_ik Dinv_k^T written back and cut
 // into row slices (RP/RS) and transposed column slices (CP/CS).  Only block
 // column k is written; column k - 1 is final.
+template <int kS, int B>
 __global__ __launch_bounds__(256) void k_trsm_slice(double* __restrict__ A,
                                                     const double* __restrict__ dinv, int np, int k,
                                                     int pre, int8_t* __restrict__ RP,
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void k_trsm_slice(double* __restrict__ A,
         blk[r][c] = v;
     });
     __syncthreads();
-    slice_block(blk, rsc, csc, blk_index(w, nblk, i, k), RP, RS, CP, CS);
+    slice_block<kS, B>(blk, rsc, csc, blk_index(w, nblk, i, k), RP, RS, CP, CS);
 }
 
 // Column j of the inverse factor M = L^-1, block i = j + blockIdx.x of A_w:
@@ -131,6 +134,7 @@ __global__ __launch_bounds__(256) void k_trsm_slice(double* __restrict__ A,
 // only defined below the diagonal), where the previous column's launch saved
 // it (save != 0: block (j, j-1) of L into block j-1 of slot `next`) before
 // this column's blocks overwrote it.
+template <int kS, int B>
 __global__ __launch_bounds__(256) void k_trtri_m_slice(double* __restrict__ A,
                                                        const double* __restrict__ dinv,
                                                        const double* __restrict__ tbuf,
@@ -175,7 +179,8 @@ __global__ __launch_bounds__(256) void k_trtri_m_slice(double* __restrict__ A,
         });
     }
     __syncthreads();
-    if (slice) slice_block(blk, rsc, nullptr, blk_index(w, nblk, i, j), RP, RS, nullptr, nullptr);
+    if (slice)
+        slice_block<kS, B>(blk, rsc, nullptr, blk_index(w, nblk, i, j), RP, RS, nullptr, nullptr);
 }
 
 // LDS byte offset of 16 B chunk `ch` (0..3) of row `row` in a 64 x 64 byte
@@ -192,28 +197,33 @@ __device__ inline int blk_off(int row, int ch) { return row * 64 + ((ch ^ ((row 
 //   i <= c_l are not written; T_l is the 64 x 64 row-major block i of tbuf slot c_l % 2.
 // Two output columns share every A operand block, halving the left-looking
 // sweeps' operand traffic.  256 * NC threads: waves 4l..4l+3 compute column
-// block l as 2 x 2 waves of 32 x 32.  Per K block the (1 + NC) x kS slice
+// block l as 2 x 2 waves of 32 x 32.  Per K block the (1 + NC) x kS digit
 // blocks and the operands' scale exponents move from global memory straight
 // into LDS (global_load_lds, no register staging) into one of two stage
-// buffers while the other feeds 2 k chunks x 15 slice products per wave; then
+// buffers while the other feeds 2 k chunks x kS (kS + 1) / 2 digit products
+// per wave; then
 // the int32 groups are combined and scaled into FP64 accumulators with the
 // exponents read from LDS.  The two stages are separate LDS objects and the K
 // loop is unrolled by two, so every LDS read names a buffer that no load in
 // flight writes and needs no wait on those loads.  Two waves per SIMD
 // (<= 256 registers per wave).
-template <int NC>
+template <int NC, int kS>
 struct I8Stage {
-    int8_t s[1 + NC][kS][4096];   // slice blocks: operand, slice, swizzled 64 x 64 bytes
+    int8_t s[1 + NC][kS][4096];   // digit blocks: operand, digit, swizzled 64 x 64 bytes
     int sc[1 + NC][64];           // scale exponents per row of each operand
 };
 
-template <int MODE, int NC>
+template <int MODE, int NC, int kS, int B>
 __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gemm_i8blk(
     const int8_t* __restrict__ RP, const double* __restrict__ RS, const int8_t* __restrict__ CP,
     const double* __restrict__ CS, double* __restrict__ Amat, double* __restrict__ tbuf, int np,
     int kj, int rowbeg, int kb0, int kb1, int accumulate) {
-    __shared__ __attribute__((aligned(16))) I8Stage<NC> st0;   // 2 x 40.5 / 60.75 KB
-    __shared__ __attribute__((aligned(16))) I8Stage<NC> st1;
+    constexpr size_t kBlkBytes = blk_bytes<kS>();
+    // exponent of the combined groups' weight: 2^-(12 + 7 x 4), 2^-(14 + 8 x 3)
+    constexpr int kExpOffset = B == 7 ? 40 : 38;
+    static_assert((B == 7 && kS == 5) || (B == 8 && kS == 4), "digit scheme");
+    __shared__ __attribute__((aligned(16))) I8Stage<NC, kS> st0;   // 2 x 40.5 / 60.75 KB (5 digits)
+    __shared__ __attribute__((aligned(16))) I8Stage<NC, kS> st1;
     const int w = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int nblk = np / 64;
     const int i = rowbeg + blockIdx.x;
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     // last wave's lanes < 16 (1 + NC) load the scale exponents (16 B each).
     constexpr int kPieces = (1 + NC) * kS * 4;
     constexpr int kWaves = 4 * NC;
-    auto issue = [&](int kb, I8Stage<NC>& S) {
+    auto issue = [&](int kb, I8Stage<NC, kS>& S) {
         if constexpr (NC == 1) {
             // one output column: almost always a single K block (the remainder
             // of a two-column sweep), where plain loads and LDS stores beat the
@@ -299,7 +309,7 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     // row32(lane, r) for r = 4q..4q+3 is 4 consecutive rows from wr + 8q + 4 kgl)
     i32x16 acc[kS];
     int ea[16];
-    auto products = [&](const I8Stage<NC>& S) {
+    auto products = [&](const I8Stage<NC, kS>& S) {
 #pragma unroll
         for (int g = 0; g < kS; ++g)
 #pragma unroll
@@ -318,7 +328,7 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
                 for (int s = 0; s + j < kS; ++s) acc[s + j] = mfma_i8(a[s], b, acc[s + j]);
             }
         }
-        const int eb = S.sc[1 + cb][wc + l32] - 40;
+        const int eb = S.sc[1 + cb][wc + l32] - kExpOffset;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const u32x4 v = *reinterpret_cast<const u32x4*>(&S.sc[0][wr + 8 * q + 4 * kgl]);
@@ -326,17 +336,25 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
             for (int j = 0; j < 4; ++j) ea[4 * q + j] = (int)v[j] + eb;
         }
     };
-    // combine the groups with the row / column scales:
-    // sum_g G_g 2^-(12 + 7g) = (G0 2^28 + G1 2^21 + G2 2^14 + G3 2^7 + G4) 2^-40
-    // and for K = 64 (|G_g| <= 5 * 2^18) hi = G0 * 128 + G1 and
-    // lo = G3 * 128 + G4 fit in int32, z = hi 2^21 + G2 2^14 + lo < 2^47 is
-    // exact in FP64
+    // combine the groups with the row / column scales, exactly for K = 64:
+    // 5 x 7 bits: sum_g G_g 2^-(12 + 7g) = (G0 2^28 + G1 2^21 + G2 2^14 + G3 2^7 + G4) 2^-40;
+    //   |G_g| <= 5 * 2^18, so hi = G0 * 128 + G1 and lo = G3 * 128 + G4 fit
+    //   in int32 and z = hi 2^21 + G2 2^14 + lo < 2^47 is exact in FP64;
+    // 4 x 8 bits: sum_g G_g 2^-(14 + 8g) = (G0 2^24 + G1 2^16 + G2 2^8 + G3) 2^-38;
+    //   |G_g| <= 4 * 2^20, so hi = G0 * 256 + G1 fits in int32 and
+    //   z = hi 2^16 + G2 2^8 + G3 < 2^47 is exact in FP64
     auto combine = [&] {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int hi = acc[0][r] * 128 + acc[1][r];
-            const int lo = acc[3][r] * 128 + acc[4][r];
-            const double z = fma((double)hi, 0x1p21, fma((double)acc[2][r], 0x1p14, (double)lo));
+            double z;
+            if constexpr (B == 7) {
+                const int hi = acc[0][r] * 128 + acc[1][r];
+                const int lo = acc[3][r] * 128 + acc[4][r];
+                z = fma((double)hi, 0x1p21, fma((double)acc[2][r], 0x1p14, (double)lo));
+            } else {
+                const int hi = acc[0][r] * 256 + acc[1][r];
+                z = fma((double)hi, 0x1p16, fma((double)acc[2][r], 0x1p8, (double)acc[3][r]));
+            }
             acc64[r] += ldexp(z, ea[r]);
         }
     };
@@ -389,20 +407,13 @@ __global__ __launch_bounds__(256 * NC) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
 }
 
-template <int MODE, int NC>
+template <int MODE, int NC, int kS, int B>
 void gemm_i8blk(const int8_t* RP, const double* RS, const int8_t* CP, const double* CS, double* A,
                 double* tbuf, int np, int kj, int rowbeg, int rows, int kb0, int kb1, int acc, int nb,
                 hipStream_t s) {
     if (rows <= 0) return;
-    hipLaunchKernelGGL((k_gemm_i8blk<MODE, NC>), dim3(rows, nb), dim3(256 * NC), 0, s, RP, RS, CP,
-                       CS, A, tbuf, np, kj, rowbeg, kb0, kb1, acc);
-}
-
-}  // namespace
-
-size_t fact_slice_bytes(int np) {
-    const size_t nblk = (size_t)(np / 64);
-    return nblk * nblk * (kBlkBytes + 64 * 8);
+    hipLaunchKernelGGL((k_gemm_i8blk<MODE, NC, kS, B>), dim3(rows, nb), dim3(256 * NC), 0, s, RP, RS,
+                       CP, CS, A, tbuf, np, kj, rowbeg, kb0, kb1, acc);
 }
 
 // Left-looking blocked Cholesky, two block columns per sweep: the update of
@@ -410,26 +421,26 @@ size_t fact_slice_bytes(int np) {
 // the L_i,kb operands), then column k factored and its panel solved and
 // sliced, then column k+1 factored and solved with its remaining product
 // (kb = k) taken first in FP64.
-hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
-                           double* CS, int np, int nb, hipStream_t s) {
-    if (np % 64 != 0) return hipErrorInvalidValue;
+template <int kS, int B>
+hipError_t potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
+                    double* CS, int np, int nb, hipStream_t s) {
     const int nblk = np / 64;
     auto finish = [&](int k, int pre) -> hipError_t {
         if (hipError_t e = launch_potf2_block(A, dinv, info, np, k, nb, s, pre); e != hipSuccess)
             return e;
         const int t = nblk - k - 1;
         if (t > 0)
-            hipLaunchKernelGGL(k_trsm_slice, dim3(t, nb), dim3(256), 0, s, A, dinv, np, k, pre, RP,
-                               RS, CP, CS);
+            hipLaunchKernelGGL((k_trsm_slice<kS, B>), dim3(t, nb), dim3(256), 0, s, A, dinv, np, k,
+                               pre, RP, RS, CP, CS);
         return hipGetLastError();
     };
     for (int k = 0; k < nblk; k += 2) {
         const bool two = k + 1 < nblk;
         if (k > 0) {
             if (two)
-                gemm_i8blk<0, 2>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
+                gemm_i8blk<0, 2, kS, B>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
             else
-                gemm_i8blk<0, 1>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
+                gemm_i8blk<0, 1, kS, B>(RP, RS, CP, CS, A, nullptr, np, k, k, nblk - k, 0, k, 0, nb, s);
         }
         if (hipError_t e = finish(k, 0); e != hipSuccess) return e;
         if (two)
@@ -443,31 +454,53 @@ hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, doubl
 // inverted and sliced (saving L_j,j-1), then column j-1 inverted with its
 // remaining product (kb = j) added first in FP64.  tbuf holds two slots of
 // nb x np x 64 (column parity).
-hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
-                           const int8_t* CP, const double* CS, int np, int nb, hipStream_t s) {
-    if (np % 64 != 0) return hipErrorInvalidValue;
+template <int kS, int B>
+hipError_t trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
+                    const int8_t* CP, const double* CS, int np, int nb, hipStream_t s) {
     const int nblk = np / 64;
     const size_t slot = (size_t)nb * np * 64;
     auto column = [&](int j, int pre, int save) -> hipError_t {
         double* tj = tbuf + (size_t)(j & 1) * slot;
         double* tn = tbuf + (size_t)((j + 1) & 1) * slot;     // slot of column j - 1
         // column 0's blocks are never an operand of a later product: no slices
-        hipLaunchKernelGGL(k_trtri_m_slice, dim3(nblk - j, nb), dim3(256), 0, s, A, dinv, tj, tn, np,
-                           j, pre, save, j > 0 ? 1 : 0, RP, RS);
+        hipLaunchKernelGGL((k_trtri_m_slice<kS, B>), dim3(nblk - j, nb), dim3(256), 0, s, A, dinv, tj,
+                           tn, np, j, pre, save, j > 0 ? 1 : 0, RP, RS);
         return hipGetLastError();
     };
     for (int j = nblk - 1; j >= 0; j -= 2) {
         const bool two = j >= 1;
         if (two)
-            gemm_i8blk<1, 2>(RP, RS, CP, CS, A, tbuf, np, j, j, nblk - j, j + 1, nblk, 0, nb, s);
+            gemm_i8blk<1, 2, kS, B>(RP, RS, CP, CS, A, tbuf, np, j, j, nblk - j, j + 1, nblk, 0, nb, s);
         else
-            gemm_i8blk<1, 1>(RP, RS, CP, CS, A, tbuf, np, j, j + 1, nblk - 1 - j, j + 1, nblk, 0, nb,
-                             s);
+            gemm_i8blk<1, 1, kS, B>(RP, RS, CP, CS, A, tbuf, np, j, j + 1, nblk - 1 - j, j + 1, nblk, 0,
+                                    nb, s);
         if (hipError_t e = column(j, 0, two ? 1 : 0); e != hipSuccess) return e;
         if (two)
             if (hipError_t e = column(j - 1, 1, 0); e != hipSuccess) return e;
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+size_t fact_slice_bytes(int np, int scheme) {
+    const size_t nblk = (size_t)(np / 64);
+    return nblk * nblk * ((size_t)i8_digits(scheme) * 4096 + 64 * 8);
+}
+
+hipError_t launch_potrf_i8(double* A, double* dinv, int* info, int8_t* RP, double* RS, int8_t* CP,
+                           double* CS, int np, int nb, int scheme, hipStream_t s) {
+    if (np % 64 != 0) return hipErrorInvalidValue;
+    return scheme == kDigits4x8 ? potrf_i8<4, 8>(A, dinv, info, RP, RS, CP, CS, np, nb, s)
+                                : potrf_i8<5, 7>(A, dinv, info, RP, RS, CP, CS, np, nb, s);
+}
+
+hipError_t launch_trtri_i8(double* A, const double* dinv, double* tbuf, int8_t* RP, double* RS,
+                           const int8_t* CP, const double* CS, int np, int nb, int scheme,
+                           hipStream_t s) {
+    if (np % 64 != 0) return hipErrorInvalidValue;
+    return scheme == kDigits4x8 ? trtri_i8<4, 8>(A, dinv, tbuf, RP, RS, CP, CS, np, nb, s)
+                                : trtri_i8<5, 7>(A, dinv, tbuf, RP, RS, CP, CS, np, nb, s);
 }
 
 }  // namespace mgw

@@ -255,7 +255,8 @@ int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw
         j->estimate += round_up(n2, 64) * round_up((size_t)job->kdim, 16) * 8;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->judge_n > 0 && (g.precision == MGW_PREC_MIXED || g.precision == MGW_PREC_MIXED_INT8))
+    if (s->judge_n > 0 && (g.precision == MGW_PREC_MIXED || g.precision == MGW_PREC_MIXED_INT8 ||
+                           g.precision == MGW_PREC_MIXED_INT8_4))
         j->estimate += judge_bytes(g);
     j->t_submit = Clock::now();
     j->ticket = s->next_ticket++;

@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libm
 
 MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_MEMORY, \
     MGW_ERR_STATE, MGW_ERR_IO = range(8)
-PREC_FP64, PREC_MIXED, PREC_MIXED_INT8, PREC_FP64_CD = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED, PREC_MIXED_INT8, PREC_FP64_CD, PREC_MIXED_INT8_4 = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

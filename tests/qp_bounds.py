@@ -46,7 +46,9 @@ HA_MEV = 27.211386245988 * 1e3
 QP_TOL_HA = 1e-8
 MEV_HA = 1.0 / HA_MEV
 # relative Sigma_c(i w) precision each form is held to (of max |Sigma_c|)
-SIGMA_REL = {"fp64": 2e-14, "int8": 1e-9, "fp16split": 1e-6}
+# (the int8 forms: five 7-bit digits truncate at ~2^-35 of a row's largest
+# element, four 8-bit digits at ~2^-32, eight times more)
+SIGMA_REL = {"fp64": 2e-14, "int8": 1e-9, "int8x4": 1e-8, "fp16split": 1e-6}
 
 
 def ref(name: str, backend: str = "reference") -> dict:

@@ -280,8 +280,9 @@ def test_multiblock_vs_oracle(hip, ctx, nmo, nocc, nfreq, eta):
 def test_extreme_occupations_vs_oracle(hip, ctx, nmo, nocc):
     """One particle-hole pair (nmo = 2), one occupied or one virtual orbital:
     nph of 1 to 12, all inside one zero-padded 256-wide tile.  FP64 to the
-    oracle at 1e-11, the decomposed-ERI form at 1e-10, the int8 form within
-    1e-9 of FP64; QP energies of the FP64 form equal the oracle's fixed point
+    oracle at 1e-11, the decomposed-ERI form at 1e-10, the int8 forms within
+    1e-9 (5 x 7 bits) and 1e-8 (4 x 8 bits) of FP64; QP energies of the FP64
+    form equal the oracle's fixed point
     on the same Sigma_c."""
     say(f"case: extreme occupation nmo={nmo} nocc={nocc}")
     inp = synthetic_input(nmo, nocc, 5)
@@ -293,8 +294,11 @@ def test_extreme_occupations_vs_oracle(hip, ctx, nmo, nocc):
     dcd = np.abs(cd.sigma_c - want.sigma_c).max() / scale
     mix = hip.run_g0w0(inp, 32, 8, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
     dmix = np.abs(mix.sigma_c - res.sigma_c).max() / scale
-    say(f"   rel |dSigma_c|: FP64 {d:.2e}, decomposed {dcd:.2e}, int8 vs FP64 {dmix:.2e}")
-    assert d < 1e-11 and dcd < 1e-10 and dmix < 1e-9
+    mix4 = hip.run_g0w0(inp, 32, 8, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8_4)
+    dmix4 = np.abs(mix4.sigma_c - res.sigma_c).max() / scale
+    say(f"   rel |dSigma_c|: FP64 {d:.2e}, decomposed {dcd:.2e}, int8 vs FP64 {dmix:.2e}, "
+        f"int8x4 vs FP64 {dmix4:.2e}")
+    assert d < 1e-11 and dcd < 1e-10 and dmix < SIGMA_REL["int8"] and dmix4 < SIGMA_REL["int8x4"]
     qp, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
                          res.sigma_c, res.omegas, list(range(nmo)), 8)
     assert np.allclose(res.qp, qp, rtol=0, atol=1e-12, equal_nan=True)
@@ -362,13 +366,16 @@ def test_mixed_vs_fp64(hip, ctx, case):
 
 
 @pytest.mark.parametrize("case", list(MIXED_CASES))
-def test_mixed_int8_vs_fp64(hip, ctx, case):
-    """int8-slice form against the FP64 form (and the reference's output):
-    Sigma_c(i w) to 1e-9 relative (5 slices truncate at ~2^-35 of a row's
-    largest element, the products are exact); QP energies within 1 meV except
-    the pinned states of tests/golden/qp_mev_exceptions.json (none on H2O)."""
-    say(f"case: int8 mixed vs FP64 {case}")
-    _check_mixed(hip, ctx, case, hip.PREC_MIXED_INT8, "int8", 1e-9)
+@pytest.mark.parametrize("form", ["int8", "int8x4"])
+def test_mixed_int8_vs_fp64(hip, ctx, case, form):
+    """Both int8 forms against the FP64 form (and the reference's output):
+    Sigma_c(i w) to SIGMA_REL of the form (five 7-bit digits truncate at
+    ~2^-35 of a row's largest element: 1e-9; four 8-bit digits at ~2^-32:
+    1e-8; the products are exact); QP energies within 1 meV except the pinned
+    states of tests/golden/qp_mev_exceptions.json (none on H2O)."""
+    say(f"case: {form} mixed vs FP64 {case}")
+    prec = hip.PREC_MIXED_INT8 if form == "int8" else hip.PREC_MIXED_INT8_4
+    _check_mixed(hip, ctx, case, prec, form, SIGMA_REL[form])
 
 
 # ------------------------------------------------------------ decomposed-ERI form

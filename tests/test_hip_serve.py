@@ -101,7 +101,8 @@ def test_graph_replays_match_eager(hip):
     with hip.HipContext(0) as ctx, hip.HipContext(0) as side:
         ctx.upload_eri(inp.eri, inp.nocc)
         side.upload_eri(other.eri, other.nocc)
-        for prec in (hip.PREC_FP64, hip.PREC_MIXED, hip.PREC_MIXED_INT8, hip.PREC_FP64_CD):
+        for prec in (hip.PREC_FP64, hip.PREC_MIXED, hip.PREC_MIXED_INT8, hip.PREC_MIXED_INT8_4,
+                     hip.PREC_FP64_CD):
             ctx.set_graphs(False)
             eager = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
             side_ref = hip.run_g0w0(other, 40, 8, ctx=side, upload=False, precision=prec)
@@ -122,7 +123,7 @@ def test_graph_cache_tells_molecules_of_one_shape_apart(hip):
     which carry n as a kernel argument."""
     say("case: graphs on one context, nmo=30 nocc=6 then nocc=7 (same padded shapes)")
     first, second = synthetic_input(30, 6, 45), synthetic_input(30, 7, 46)
-    for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8, hip.PREC_FP64_CD):
+    for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8, hip.PREC_MIXED_INT8_4, hip.PREC_FP64_CD):
         with hip.HipContext(0) as ctx:
             ctx.upload_eri(second.eri, second.nocc)
             eager = hip.run_g0w0(second, 100, 16, ctx=ctx, upload=False, precision=prec)
@@ -179,14 +180,14 @@ def test_split_factorization_matches_one_stream(hip, monkeypatch):
     monkeypatch.setenv("MGW_FACTOR_STREAMS", "1")
     with hip.HipContext(0) as ctx:
         ctx.upload_eri(inp.eri, inp.nocc)
-        for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8):
+        for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8, hip.PREC_MIXED_INT8_4):
             base[prec] = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
     monkeypatch.setenv("MGW_FACTOR_SPLIT_ORDER", "0")
     for streams in ("2", "4"):
         monkeypatch.setenv("MGW_FACTOR_STREAMS", streams)
         with hip.HipContext(0) as ctx:
             ctx.upload_eri(inp.eri, inp.nocc)
-            for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8):
+            for prec in (hip.PREC_FP64, hip.PREC_MIXED_INT8, hip.PREC_MIXED_INT8_4):
                 ctx.set_graphs(False)
                 eager = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
                 assert np.array_equal(eager.sigma_c, base[prec].sigma_c), (streams, prec)
@@ -198,30 +199,34 @@ def test_split_factorization_matches_one_stream(hip, monkeypatch):
                     assert np.array_equal(res.qp, base[prec].qp, equal_nan=True)
 
 
-def test_error_judge_passes_and_forced_fallback_gives_fp64(hip):
-    """The judge of the int8 form measures its q error on sampled frequencies
-    (~1e-9, below its 1e-7 default) and leaves the result alone; a tolerance
-    no mixed form can meet forces the FP64 fallback, whose result is bit for
-    bit the FP64 form's.  Without fallback the failed judge is only reported."""
-    say("case: error judge and FP64 fallback, nmo=30, int8 form")
+@pytest.mark.parametrize("form", ["int8", "int8x4"])
+def test_error_judge_passes_and_forced_fallback_gives_fp64(hip, form):
+    """The judge of an int8 form measures its q error on sampled frequencies
+    (~1e-9 / ~1e-8, below the 1e-7 / 1e-6 defaults) and leaves the result
+    alone; a tolerance no mixed form can meet forces the FP64 fallback, whose
+    result is bit for bit the FP64 form's.  Without fallback the failed judge
+    is only reported."""
+    say(f"case: error judge and FP64 fallback, nmo=30, {form} form")
+    prec = hip.PREC_MIXED_INT8 if form == "int8" else hip.PREC_MIXED_INT8_4
+    default_tol = 1e-7 if form == "int8" else 1e-6
     inp = synthetic_input(30, 6, 47)
     with hip.HipContext(0) as ctx:
         ctx.upload_eri(inp.eri, inp.nocc)
-        plain = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        plain = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=prec)
         f64 = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_FP64)
         assert ctx.last_judge() == {"max_rel_err": -1.0, "samples": 0, "fell_back": 0}
         ctx.set_judge(3)
-        judged = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        judged = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=prec)
         j = ctx.last_judge()
         say(f"   judge: {j}")
-        assert j["samples"] == 3 and 0.0 < j["max_rel_err"] < 1e-7 and j["fell_back"] == 0
+        assert j["samples"] == 3 and 0.0 < j["max_rel_err"] < default_tol and j["fell_back"] == 0
         assert np.array_equal(judged.sigma_c, plain.sigma_c)
         ctx.set_judge(3, tol=1e-30)
-        fb = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        fb = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=prec)
         assert ctx.last_judge()["fell_back"] == 1
         assert np.array_equal(fb.sigma_c, f64.sigma_c) and np.array_equal(fb.qp, f64.qp, equal_nan=True)
         ctx.set_judge(3, tol=1e-30, fallback=False)
-        kept = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
+        kept = hip.run_g0w0(inp, 100, 16, ctx=ctx, upload=False, precision=prec)
         j = ctx.last_judge()
         assert j["fell_back"] == 0 and j["max_rel_err"] > 1e-30
         assert np.array_equal(kept.sigma_c, plain.sigma_c)
