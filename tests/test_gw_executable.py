@@ -68,7 +68,7 @@ def summary_row(out, state=5):
 
 
 @needs_exe
-@pytest.mark.parametrize("form", ["hip", "hip-int8"])
+@pytest.mark.parametrize("form", ["hip", "hip-int8", "hip-int8x5"])
 def test_gw_executable_h2o_against_regression_outputs(h2o_dir, tmp_path, form):
     """E_c_before_Pade.out within the regression test's 1e-8 component check
     (FP64 also within 1e-11 of it), E_c.out within the spread of the
@@ -79,7 +79,7 @@ def test_gw_executable_h2o_against_regression_outputs(h2o_dir, tmp_path, form):
     out = tmp_path / "out"
     stdout = run_gw(h2o_dir, out, form)
     assert "Using the hip device path on device 0" in stdout
-    if form == "hip-int8":
+    if form.startswith("hip-int8"):
         assert "hip judge: largest relative error of q" in stdout
     got = read_before_pade(out)
     want = fx["e_c_before_pade"]
