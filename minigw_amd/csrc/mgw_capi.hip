@@ -1545,6 +1545,21 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                           "reduce q");
             if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
         };
+        // Factorization breakdowns: the kernels flag frequency w of a batch in
+        // info[w]; after each batch the flags are read, recorded as frequency
+        // f0 + w and cleared for the next batch (outside any captured graph).
+        std::vector<int> failed, hinfo(nb);
+        auto collect_info = [&](int f0, int b) {
+            ctx->d2h(hinfo.data(), info, b);
+            ctx->sync();
+            bool any = false;
+            for (int w = 0; w < b; ++w)
+                if (hinfo[w] != 0) {
+                    failed.push_back(f0 + w);
+                    any = true;
+                }
+            if (any) hip_check(hipMemsetAsync(info, 0, (size_t)b * 4, st), "hipMemsetAsync");
+        };
         for (int f0 = 0; f0 < nfreq; f0 += nb) {
             if (ctx->cancel && __atomic_load_n(ctx->cancel, __ATOMIC_ACQUIRE) != 0) {
                 ctx->sync();
@@ -1578,6 +1593,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 hip_check(hipEventElapsedTime(&a, ev[2], ev[5]), "hipEventElapsedTime");
                 t_graph += a;
                 ++n_contr;
+                collect_info(f0, b);
                 continue;
             }
             {
@@ -1596,6 +1612,7 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
             t_contr += c;
             t_aux += d + e;
             ++n_contr;
+            collect_info(f0, b);
         }
         // Frequencies whose symmetrised dielectric matrix is not positive
         // definite: the plain FP64 form screens them by pivoted LU (their q rows
@@ -1603,17 +1620,11 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         // judge then reruns the job in FP64).
         ctx->lu_freqs = 0;
         {
-            std::vector<int> hinfo(nfreq);
-            ctx->d2h(hinfo.data(), info, nfreq);
-            ctx->sync();
-            std::vector<int> failed;
-            for (int f = 0; f < nfreq; ++f)
-                if (hinfo[f] != 0) failed.push_back(f);
             if (!failed.empty()) {
                 if (cplx || cd || mixed || i8)
                     fail(MGW_ERR_NUMERIC,
-                         "potrf: symmetrised dielectric matrix %d is not positive definite (pivot %d)",
-                         failed[0], hinfo[failed[0]]);
+                         "potrf: symmetrised dielectric matrix %d is not positive definite",
+                         failed[0]);
                 double* W = as<double>(ctx->get(ctx->luz, (size_t)np * np * 8, "E^-1"));
                 for (int f : failed) {
                     hip_check(mgw::launch_set_int(f0p, f, st), "set offset");

@@ -539,6 +539,17 @@ def test_indefinite_dielectric_takes_pivoted_lu(hip, ctx, nmo, nocc):
     d = np.abs(res.sigma_c - want.sigma_c).max() / np.abs(want.sigma_c).max()
     say(f"   G0W0: {lu} of 40 frequencies by pivoted LU, rel |dSigma_c| {d:.2e}")
     assert 0 < lu < 40 and d < 1e-11
+    # the same with frequency batches smaller than the run of failing
+    # frequencies: breakdowns of every batch are found and screened by LU
+    nb = max(1, lu // 3)
+    ctx.set_batch_limit(nb)
+    try:
+        split = hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False)
+    finally:
+        ctx.set_batch_limit(0)
+    say(f"   batches of {nb}: {int(split.timings['lu_freqs'])} frequencies by pivoted LU")
+    assert int(split.timings["freq_batch"]) <= nb and int(split.timings["lu_freqs"]) == lu
+    assert np.array_equal(split.sigma_c, res.sigma_c)
     with pytest.raises(hip.HipError) as e:
         hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8)
     assert e.value.code == hip.MGW_ERR_NUMERIC
