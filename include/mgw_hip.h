@@ -234,6 +234,28 @@ typedef struct mgw_g0w0_out {
 /* One molecule, all frequencies, on the ERI made resident by mgw_hip_upload_eri. */
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out);
 
+/* Several molecules at once, each job on the ERI resident in its own context
+ * (jobs[i] on ctxs[i], all contexts on one device, none twice); no
+ * counterpart in the reference, whose driver runs one molecule per process
+ * (src/gw/sigma_c_driver.cpp:225-260 loops over its frequencies).  Molecules
+ * with the same padded particle-hole order and form (eta == 0, not
+ * MGW_PREC_FP64_CD) share every launch of their frequency batches: their
+ * frequencies are stacked in one batch of dielectric matrices (up to 2048 at
+ * once, within ctxs[0]'s memory budget and batch limit), factored, inverted
+ * and cut into digits by the same launches on ctxs[0]'s stream, with
+ * ctxs[0]'s batch buffers; the contraction and reduction are one launch per
+ * molecule on its own panel.  The rest (setup, LU re-screening, error judge
+ * and FP64 fallback, Sigma_c, Pade/QP) runs per molecule on its context, as
+ * mgw_hip_g0w0 does; every result equals bit for bit the molecule's
+ * mgw_hip_g0w0 result.  Other molecules (eta != 0, decomposed form, an order
+ * of their own) run their batches alone on their own context.  status[i]
+ * (may be NULL) receives each job's code, whose message is
+ * mgw_hip_last_error(ctxs[i]); a job cancelled through its context's flag
+ * while its group runs ends with MGW_ERR_STATE, the others complete.  Returns
+ * MGW_OK when every job succeeded, else the first failing code. */
+int mgw_hip_g0w0_group(mgw_hip_ctx* const* ctxs, const mgw_g0w0_job* jobs, mgw_g0w0_out* outs,
+                       int32_t count, int32_t* status);
+
 /* Error judge and FP64 fallback of the mixed forms (no counterpart in the
  * reference, whose only failure check is the factorization's devInfo,
  * src/workspace/device_screening_workspace.cu:324-334).  With nsample > 0,

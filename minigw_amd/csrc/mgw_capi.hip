@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <limits>
 #include <map>
@@ -122,18 +123,16 @@ void* noalloc_base(int device) {
 
 // ------------------------------------------------------------------ context
 // Everything a captured frequency batch bakes into its kernel arguments: the
-// true particle-hole count n (rows past it get S = 0), the ph padded order
-// nph_pad (a stride of the pair-space dielectric), the factored order fp (=
-// nph_pad, or the padded rank in the decomposed form), the column counts, the
-// batch, the form and the buffers.
+// shapes (true particle-hole count n of each molecule, whose rows past it get
+// S = 0; padded and factored orders; column counts; batch and matrix offsets),
+// the form and every buffer address, as one signature.
 struct GraphKey {
-    int n, nph_pad, fp, mpad, m, b, precision, uf;
-    std::array<const void*, 21> ptr;
-    bool operator==(const GraphKey& o) const {
-        return n == o.n && nph_pad == o.nph_pad && fp == o.fp && mpad == o.mpad && m == o.m &&
-               b == o.b && precision == o.precision && uf == o.uf && ptr == o.ptr;
-    }
+    std::vector<uint64_t> sig;
+    bool operator==(const GraphKey& o) const { return sig == o.sig; }
 };
+
+// most matrices (molecules x frequencies) one stacked batch of a group holds
+constexpr int kMaxStack = 2048;
 
 struct mgw_hip_ctx {
     int device = 0;
@@ -234,6 +233,7 @@ struct mgw_hip_ctx {
     Buf xhi, xlo, xscale, mhi, mlo, mscale;
     Buf hqim, hY, wY, wW;             // complex path (eta != 0)
     Buf hfr, hfc;                     // factor block slices (mixed forms)
+    Buf hinfob;                       // breakdown flags of one frequency batch
     // Cholesky-decomposed ERI form: pair tables, decomposition workspace,
     // factor columns, and B^T / pair vectors / their norms
     Buf cdidx, cdwork, cdl, cdb, cdx, cdn;
@@ -263,7 +263,7 @@ struct mgw_hip_ctx {
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
                &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj, &luz, &ludl, &ludu,
-               &lupiv, &wz, &ppk, &panel};
+               &lupiv, &wz, &ppk, &panel, &hinfob};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -1212,11 +1212,58 @@ int mgw_hip_quadratic_forms_device(mgw_hip_ctx* ctx, const double* x_dev, int32_
 
 // ---------------------------------------------------- hoisted G0W0
 namespace {
-// One molecule in the job's form.  Returns false, having written nothing to
-// `out`, when the error judge of a mixed form failed and the context asks for
-// the FP64 fallback.
-bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
+// One molecule's G0W0 in three phases: mol_prepare (its layout, contraction
+// columns and resident per-molecule data, on its own context), run_batches
+// (the frequency batches, on the lead context's stream; molecules of one
+// padded order share every batch launch, their frequencies stacked), and
+// mol_finish (LU re-screening of broken-down frequencies, error judge,
+// Sigma_c, Pade/QP and downloads, on its own context).  A molecule run alone
+// is a group of one on its own context.
+struct MolRun {
+    mgw_hip_ctx* ctx = nullptr;
+    const mgw_g0w0_job* job = nullptr;
+    int nmo = 0, nocc = 0, nfreq = 0, npar = 0, ns = 0, n = 0, np = 0, m = 0, mu = 0, mpad = 0;
+    int fp = 0, cd_rank = 0, scheme = mgw::kDigits5x7;
+    bool cplx = false, cd = false, mixed = false, i8 = false;
+    size_t digits = 5;
+    std::vector<int> states, colmap;
+    std::vector<std::pair<int, int>> cols;
+    std::vector<double> delta, omega, weight;
+    std::vector<int64_t> pkrow;
+    double cd_ms = 0.0;
+    double *dV = nullptr, *dom = nullptr, *dwt = nullptr, *dmo = nullptr, *ddelta = nullptr;
+    double *dvxc = nullptr, *dq = nullptr, *dqi = nullptr;
+    int *dstates = nullptr, *dcolmap = nullptr;
+    int64_t* dpk = nullptr;
+    double *cdBt = nullptr, *cdLx = nullptr, *cdnrm = nullptr;
+    uint16_t *xh = nullptr, *xl = nullptr;
+    double *xs = nullptr, *xp = nullptr, *xj = nullptr;
+    double *sre = nullptr, *sim = nullptr, *sx = nullptr, *qp = nullptr;
+    int* iters = nullptr;
+    double2* gq = nullptr;
+    int *info = nullptr, *f0p = nullptr;
+    std::vector<int> failed;           // frequencies whose factorization broke down
+    // timings (ms) of the frequency batches the molecule took part in
+    float t_setup = 0, t_diel = 0, t_fact = 0, t_contr = 0, t_aux = 0, t_graph = 0;
+    int n_contr = 0, nb = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};   // setup start / end on the molecule's stream
+
+    MolRun() = default;
+    MolRun(const MolRun&) = delete;
+    MolRun& operator=(const MolRun&) = delete;
+    ~MolRun() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+// Phase 1: validate the job, lay out the particle-hole space and the
+// contraction columns, and build the molecule's resident data on its context
+// (V_ph, the grid, X in the form's representation, output buffers).
+void mol_prepare(MolRun& r, mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         need(ctx && job && out, "mgw_hip_g0w0: NULL argument");
+        r.ctx = ctx;
+        r.job = job;
         const int nmo = job->nmo, nocc = job->nocc, nfreq = job->nfreq, npar = job->npar;
         need(nmo > 1 && nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
         if (ctx->nmo != nmo || ctx->nocc != nocc)
@@ -1235,7 +1282,17 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                  job->precision == MGW_PREC_MIXED_INT8_4,
              "mgw_hip_g0w0: unknown precision %d", job->precision);
         const bool cd = job->precision == MGW_PREC_FP64_CD;
-        std::vector<int> states;
+        r.nmo = nmo;
+        r.nocc = nocc;
+        r.nfreq = nfreq;
+        r.npar = npar;
+        r.cplx = cplx;
+        r.cd = cd;
+        r.mixed = job->precision == MGW_PREC_MIXED;
+        r.i8 = int8_form(job->precision);
+        r.scheme = digit_scheme(job->precision);
+        r.digits = (size_t)mgw::i8_digits(r.scheme);
+        std::vector<int>& states = r.states;
         if (job->nstates <= 0 || !job->states) {
             states.resize(nmo);
             std::iota(states.begin(), states.end(), 0);
@@ -1250,13 +1307,18 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         const PhLayout L = ph_layout(nmo, nocc);
         const int n = L.n, np = L.np;
         const int m = ns * nmo;
+        r.ns = ns;
+        r.n = n;
+        r.np = np;
+        r.m = m;
         // Contraction columns.  With (pk|ia) == (kp|ia) exactly, q(p,k) = q(k,p):
         // each unordered pair {p,k} with p or k among the states is contracted once
         // and colmap sends every (state, k) to its column.  The decomposed form
         // has one vector per unordered pair (its ERI symmetry is checked to a
         // tolerance) and always contracts unordered pairs.
-        std::vector<int> colmap(m);
-        std::vector<std::pair<int, int>> cols;
+        std::vector<int>& colmap = r.colmap;
+        std::vector<std::pair<int, int>>& cols = r.cols;
+        colmap.resize(m);
         {
             std::vector<int> slot(nmo, -1);
             for (int s = 0; s < ns; ++s) slot[states[s]] = s;
@@ -1273,7 +1335,10 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 }
         }
         const int mu = (int)cols.size(), mpad = mgw::round_up(mu, mgw::kPad);
-        std::vector<double> delta(np, 0.0);
+        r.mu = mu;
+        r.mpad = mpad;
+        std::vector<double>& delta = r.delta;
+        delta.assign(np, 0.0);
         for (int i = 0; i < nocc; ++i)
             for (int a = 0; a < L.nvir; ++a) {
                 const double d = job->mo_energy[nocc + a] - job->mo_energy[i];
@@ -1282,47 +1347,41 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                          "particle-hole gap e_%d - e_%d = %g is not positive", nocc + a, i, d);
                 delta[i * L.nvir + a] = d;
             }
-        std::vector<double> omega(nfreq), weight(nfreq);
-        transformed_legendre(nfreq, omega.data(), weight.data());
-        std::vector<int64_t> pkrow(mpad, 0);
+        r.omega.resize(nfreq);
+        r.weight.resize(nfreq);
+        transformed_legendre(nfreq, r.omega.data(), r.weight.data());
+        r.pkrow.assign(mpad, 0);
         const int64_t n2 = (int64_t)nmo * nmo;
-        for (int c = 0; c < mu; ++c) pkrow[c] = ((int64_t)cols[c].first * nmo + cols[c].second) * n2;
+        for (int c = 0; c < mu; ++c) r.pkrow[c] = ((int64_t)cols[c].first * nmo + cols[c].second) * n2;
 
         hipStream_t st = ctx->stream;
-        hipEvent_t ev[8];
-        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-        struct EvGuard {
-            hipEvent_t* e;
-            ~EvGuard() {
-                for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
-            }
-        } evg{ev};
-        hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
-        auto setup_range = std::make_unique<Range>("screening setup");
+        for (auto& e : r.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventRecord(r.ev[0], st), "hipEventRecord");
+        Range setup_range("screening setup");
 
         // resident per-molecule data (the decomposed form needs no V_ph)
-        double* dV = cd ? nullptr : as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
-        double* dom = as<double>(ctx->get(ctx->homega, (size_t)nfreq * 8, "omega"));
-        double* dwt = as<double>(ctx->get(ctx->hweight, (size_t)nfreq * 8, "weight"));
-        double* dmo = as<double>(ctx->get(ctx->hmo, (size_t)(nmo + np) * 8, "mo/delta"));
-        double* ddelta = dmo + nmo;
-        double* dvxc = as<double>(ctx->get(ctx->hvxc, (size_t)nmo * 8, "vxc"));
-        int* dstates = as<int>(ctx->get(ctx->hstates, (size_t)ns * 4, "states"));
-        int64_t* dpk = as<int64_t>(ctx->get(ctx->hpkrow, (size_t)mpad * 8, "pkrow"));
-        double* dq = as<double>(ctx->get(ctx->hq, (size_t)nfreq * m * 8, "q"));
-        double* dqi = cplx ? as<double>(ctx->get(ctx->hqim, (size_t)nfreq * m * 8, "q imag")) : nullptr;
-        ctx->h2d(dom, omega.data(), nfreq);
-        ctx->h2d(dwt, weight.data(), nfreq);
-        ctx->h2d(dmo, job->mo_energy, nmo);
-        ctx->h2d(ddelta, delta.data(), np);
-        ctx->h2d(dvxc, job->vxc_diag, nmo);
-        ctx->h2d(dstates, states.data(), ns);
-        ctx->h2d(dpk, pkrow.data(), mpad);
-        int* dcolmap = as<int>(ctx->get(ctx->hcolmap, (size_t)m * 4, "colmap"));
-        ctx->h2d(dcolmap, colmap.data(), m);
+        r.dV = cd ? nullptr : as<double>(ctx->get(ctx->v, (size_t)np * np * 8, "V_ph"));
+        r.dom = as<double>(ctx->get(ctx->homega, (size_t)nfreq * 8, "omega"));
+        r.dwt = as<double>(ctx->get(ctx->hweight, (size_t)nfreq * 8, "weight"));
+        r.dmo = as<double>(ctx->get(ctx->hmo, (size_t)(nmo + np) * 8, "mo/delta"));
+        r.ddelta = r.dmo + nmo;
+        r.dvxc = as<double>(ctx->get(ctx->hvxc, (size_t)nmo * 8, "vxc"));
+        r.dstates = as<int>(ctx->get(ctx->hstates, (size_t)ns * 4, "states"));
+        r.dpk = as<int64_t>(ctx->get(ctx->hpkrow, (size_t)mpad * 8, "pkrow"));
+        r.dq = as<double>(ctx->get(ctx->hq, (size_t)nfreq * m * 8, "q"));
+        r.dqi = cplx ? as<double>(ctx->get(ctx->hqim, (size_t)nfreq * m * 8, "q imag")) : nullptr;
+        ctx->h2d(r.dom, r.omega.data(), nfreq);
+        ctx->h2d(r.dwt, r.weight.data(), nfreq);
+        ctx->h2d(r.dmo, job->mo_energy, nmo);
+        ctx->h2d(r.ddelta, delta.data(), np);
+        ctx->h2d(r.dvxc, job->vxc_diag, nmo);
+        ctx->h2d(r.dstates, states.data(), ns);
+        ctx->h2d(r.dpk, r.pkrow.data(), mpad);
+        r.dcolmap = as<int>(ctx->get(ctx->hcolmap, (size_t)m * 4, "colmap"));
+        ctx->h2d(r.dcolmap, colmap.data(), m);
         if (!cd) {
             hip_check(mgw::launch_build_v(as<double>(ctx->eri.p), as<int64_t>(ctx->rowbase.p),
-                                          as<int>(ctx->colidx.p), dV, n, np, st),
+                                          as<int>(ctx->colidx.p), r.dV, n, np, st),
                       "build V_ph");
             ctx->n = n;
             ctx->np = np;
@@ -1332,78 +1391,99 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         // Decomposed form: M = L L^T over unique pairs, then B^T (ph rows of L)
         // and the pair vectors of the contraction columns; fp = order of the
         // factored matrices (the rank, padded) instead of np.
-        int fp = np, cd_rank = 0;
-        double cd_ms = 0.0;
-        double *cdBt = nullptr, *cdLx = nullptr, *cdnrm = nullptr;
+        r.fp = np;
         if (cd) {
             const auto c0 = std::chrono::steady_clock::now();
-            const CdResult r = cd_decompose(ctx, L, cols, mpad);
-            cd_rank = r.rank;
-            fp = r.rpad;
-            cdBt = r.bt;
-            cdLx = r.lx;
-            cdnrm = r.nrm;
-            cd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
-                        .count();
+            const CdResult c = cd_decompose(ctx, L, cols, mpad);
+            r.cd_rank = c.rank;
+            r.fp = c.rpad;
+            r.cdBt = c.bt;
+            r.cdLx = c.lx;
+            r.cdnrm = c.nrm;
+            r.cd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
+                          .count();
         }
-        const bool mixed = job->precision == MGW_PREC_MIXED;
-        const bool i8 = int8_form(job->precision);
-        const int scheme = digit_scheme(job->precision);
-        const size_t digits = (size_t)mgw::i8_digits(scheme);
         const mgw::Split16 fmt = mgw::Split16::FP16;
-        uint16_t *xh = nullptr, *xl = nullptr;
-        double* xs = nullptr;
-        double* xp = nullptr;
         if (cd) {
-            xp = cdLx;
-            xs = cdnrm;
-            dV = cdBt;
-        } else if (!mixed && !i8) {
-            xp = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X"));
-            hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, n,
-                                         np, mpad, st),
+            r.xp = r.cdLx;
+            r.xs = r.cdnrm;
+            r.dV = r.cdBt;
+        } else if (!r.mixed && !r.i8) {
+            r.xp = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X"));
+            hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), r.dpk, as<int>(ctx->colidx.p), r.xp,
+                                         n, np, mpad, st),
                       "pack X");
         }
-        if (mixed) {
-            xh = as<uint16_t>(ctx->get(ctx->xhi, (size_t)mpad * np * 2, "X hi"));
-            xl = as<uint16_t>(ctx->get(ctx->xlo, (size_t)mpad * np * 2, "X lo"));
-            xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
-            hip_check(mgw::launch_split_x(fmt, as<double>(ctx->eri.p), dpk,
-                                          as<int>(ctx->colidx.p), xh, xl, xs, n, np, mpad, st),
+        if (r.mixed) {
+            r.xh = as<uint16_t>(ctx->get(ctx->xhi, (size_t)mpad * np * 2, "X hi"));
+            r.xl = as<uint16_t>(ctx->get(ctx->xlo, (size_t)mpad * np * 2, "X lo"));
+            r.xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
+            hip_check(mgw::launch_split_x(fmt, as<double>(ctx->eri.p), r.dpk,
+                                          as<int>(ctx->colidx.p), r.xh, r.xl, r.xs, n, np, mpad, st),
                       "split X");
         }
-        if (i8) {
-            need(np <= mgw::i8_max_k(scheme),
+        if (r.i8) {
+            need(np <= mgw::i8_max_k(r.scheme),
                  "mgw_hip_g0w0: nph %d exceeds the int8 contraction's limit %d", n,
-                 mgw::i8_max_k(scheme));
-            xh = as<uint16_t>(ctx->get(ctx->xhi, digits * mpad * np, "X slices"));
-            xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
-            hip_check(mgw::launch_slice_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p),
-                                          reinterpret_cast<int8_t*>(xh), xs, n, np, mpad, scheme,
+                 mgw::i8_max_k(r.scheme));
+            r.xh = as<uint16_t>(ctx->get(ctx->xhi, r.digits * mpad * np, "X slices"));
+            r.xs = as<double>(ctx->get(ctx->xscale, (size_t)mpad * 8, "X scale"));
+            hip_check(mgw::launch_slice_x(as<double>(ctx->eri.p), r.dpk, as<int>(ctx->colidx.p),
+                                          reinterpret_cast<int8_t*>(r.xh), r.xs, n, np, mpad, r.scheme,
                                           st),
                       "slice X");
         }
         // the error judge's FP64 X (mixed forms), before the batch takes the room
-        double* xj = nullptr;
-        if ((mixed || i8) && ctx->judge_n > 0) {
-            xj = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X (judge)"));
-            hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xj, n,
-                                         np, mpad, st),
+        if ((r.mixed || r.i8) && ctx->judge_n > 0) {
+            r.xj = as<double>(ctx->get(ctx->hxp, (size_t)mpad * np * 8, "packed X (judge)"));
+            hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), r.dpk, as<int>(ctx->colidx.p), r.xj,
+                                         n, np, mpad, st),
                       "pack X (judge)");
         }
-        hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
-        setup_range.reset();
-
         // per-molecule outputs before the batch, which takes the room that is left
-        double* sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
-        double* sim = as<double>(ctx->get(ctx->hsig_im, (size_t)ns * nfreq * 8, "sigma im"));
-        double* sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
-        double* qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
-        int* iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
-        double2* gq = as<double2>(
+        r.sre = as<double>(ctx->get(ctx->hsig_re, (size_t)ns * nfreq * 8, "sigma re"));
+        r.sim = as<double>(ctx->get(ctx->hsig_im, (size_t)ns * nfreq * 8, "sigma im"));
+        r.sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
+        r.qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
+        r.iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
+        r.gq = as<double2>(
             ctx->get(ctx->hsg, mgw::sigma_workspace_bytes(nmo, ns, nfreq), "quadrature factors"));
-        int* info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
-        int* f0p = as<int>(ctx->get(ctx->f0buf, 2 * sizeof(int), "batch offset"));
+        r.info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
+        r.f0p = as<int>(ctx->get(ctx->f0buf, 2 * sizeof(int), "batch offset"));
+        hip_check(hipMemsetAsync(r.info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
+        hip_check(hipEventRecord(r.ev[1], st), "hipEventRecord");
+}
+
+// One frequency batch's share of a molecule: its frequencies [f0, f0 + b) at
+// matrix offset `off` of the batch and partial sums at qoff doubles of qpart.
+struct Seg {
+    MolRun* r;
+    int f0, b, off;
+    size_t qoff;
+};
+
+// Phase 2: the frequency batches of molecules that share the padded order,
+// the form and eta = 0 (a molecule with eta != 0 or the decomposed form runs
+// alone).  Everything is enqueued on lead's stream with lead's batch buffers;
+// the molecules' own data must be complete (their streams synchronized).
+void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols) {
+        MolRun& r0 = *mols[0];
+        const mgw_g0w0_job* job = r0.job;
+        const int np = r0.np, fp = r0.fp, precision = job->precision;
+        const bool cplx = r0.cplx, cd = r0.cd, mixed = r0.mixed, i8 = r0.i8;
+        const int scheme = r0.scheme;
+        const size_t digits = r0.digits;
+        need(!mols.empty(), "run_batches: no molecule");
+        int total = 0, mpad_max = 0;
+        for (MolRun* r : mols) {
+            need(r->np == np && r->job->precision == precision && r->scheme == scheme &&
+                     (mols.size() == 1 || (!r->cplx && !r->cd)),
+                 "mgw_hip_g0w0_group: molecules of one group share the padded order and the form "
+                 "(eta = 0, not decomposed)");
+            total += r->nfreq;
+            mpad_max = std::max(mpad_max, r->mpad);
+        }
+        hipStream_t st = lead->stream;
 
         // frequency batch size from the device budget
         size_t free_b = 0, total_b = 0;
@@ -1411,179 +1491,261 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
         // the batch buffers this context already holds are reused or released
         // before a larger allocation: they count as room
         size_t reusable = 0;
-        for (const mgw_hip_ctx::Buf* b : {&ctx->hA, &ctx->hS, &ctx->hdinv, &ctx->hY, &ctx->hfr,
-                                          &ctx->hfc, &ctx->htbuf, &ctx->hqpart, &ctx->mhi,
-                                          &ctx->mlo, &ctx->mscale})
+        for (const mgw_hip_ctx::Buf* b : {&lead->hA, &lead->hS, &lead->hdinv, &lead->hY, &lead->hfr,
+                                          &lead->hfc, &lead->htbuf, &lead->hqpart, &lead->mhi,
+                                          &lead->mlo, &lead->mscale, &lead->hinfob})
             reusable += b->bytes;
         const size_t avail = free_b + reusable;
         size_t room = avail > (size_t)(2ull << 30) ? avail - (size_t)(2ull << 30) : avail / 2;
-        if (ctx->budget) {
-            const size_t fixed = ctx->held - reusable;
-            room = std::min(room, ctx->budget > fixed ? ctx->budget - fixed : 0);
+        if (lead->budget) {
+            const size_t fixed = lead->held - reusable;
+            room = std::min(room, lead->budget > fixed ? lead->budget - fixed : 0);
         }
-        const size_t per = cd ? batch_bytes(fp, mpad, MGW_PREC_FP64) + (size_t)np * 8
-                              : batch_bytes(np, mpad, job->precision, cplx);
-        int nb = (int)std::min<size_t>((size_t)nfreq, std::max<size_t>(1, room / per));
-        nb = std::min(nb, cd ? std::max(256, ctx->batch_cap) : ctx->batch_cap);
-        const Tuned tn = tuned(nmo, nocc, ns, job->precision);
-        const int pick = ctx->forced_nb ? ctx->forced_nb : tn.nb;
-        if (pick > 0) nb = std::min(nb, pick);
-        if (ctx->max_nb > 0) nb = std::min(nb, ctx->max_nb);
+        const size_t per = cd ? batch_bytes(fp, mpad_max, MGW_PREC_FP64) + (size_t)np * 8
+                              : batch_bytes(np, mpad_max, precision, cplx);
+        int nb = (int)std::min<size_t>((size_t)total, std::max<size_t>(1, room / per));
+        int uf = 0;
+        if (mols.size() == 1) {
+            nb = std::min(nb, cd ? std::max(256, lead->batch_cap) : lead->batch_cap);
+            const Tuned tn = tuned(r0.nmo, r0.nocc, r0.ns, precision);
+            const int pick = lead->forced_nb ? lead->forced_nb : tn.nb;
+            if (pick > 0) nb = std::min(nb, pick);
+            uf = tn.uf;
+        } else {
+            // stacked molecules: at most kMaxStack matrices per launch and the
+            // frequency cap's worth of the largest molecule's bytes
+            nb = std::min(nb, kMaxStack);
+        }
+        if (lead->max_nb > 0) nb = std::min(nb, lead->max_nb);
         // equal batches: the factorization's per-batch latency is paid once per
         // batch whatever its size, so no short last batch
-        nb = (nfreq + (nfreq + nb - 1) / nb - 1) / ((nfreq + nb - 1) / nb);
+        nb = (total + (total + nb - 1) / nb - 1) / ((total + nb - 1) / nb);
         // untuned unit shape: measured at nmo=200, 16 frequencies per unit for
         // the int8 contraction (1-1.5 % faster than 4), 4 for the FP16 split
-        const int uf = ctx->forced_uf ? ctx->forced_uf
-                       : ctx->split_uf ? ctx->split_uf
-                       : tn.uf ? tn.uf
-                       : i8 ? 16
-                                                                : 4;
+        uf = lead->forced_uf ? lead->forced_uf : lead->split_uf ? lead->split_uf : uf ? uf : i8 ? 16 : 4;
         const size_t zf = cplx ? 2 : 1;    // complex entries are two doubles
         const int fnblk = fp / mgw::kNB;
-        double* A = as<double>(ctx->get(ctx->hA, zf * nb * fp * fp * 8, "dielectric batch"));
-        double* S = as<double>(ctx->get(ctx->hS, zf * nb * np * 8, "scale batch"));
-        double* dinv = as<double>(ctx->get(ctx->hdinv, zf * nb * fnblk * 64 * 64 * 8, "dinv"));
-        double* Y = cplx ? as<double>(ctx->get(ctx->hY, (size_t)nb * np * mpad * 16, "complex panel"))
+        const int mpad = mpad_max;
+        double* A = as<double>(lead->get(lead->hA, zf * nb * fp * fp * 8, "dielectric batch"));
+        double* S = as<double>(lead->get(lead->hS, zf * nb * np * 8, "scale batch"));
+        double* dinv = as<double>(lead->get(lead->hdinv, zf * nb * fnblk * 64 * 64 * 8, "dinv"));
+        double* Y = cplx ? as<double>(lead->get(lead->hY, (size_t)nb * np * mpad * 16, "complex panel"))
                          : nullptr;
         char *fr = nullptr, *fc = nullptr;
-        if ((mixed || i8) && ctx->int8_factor) {       // the mixed forms' int8 block products
-            fr = as<char>(ctx->get(ctx->hfr, (size_t)nb * mgw::fact_slice_bytes(np, scheme),
-                                   "factor row slices"));
-            fc = as<char>(ctx->get(ctx->hfc, (size_t)nb * mgw::fact_slice_bytes(np, scheme),
-                                   "factor column slices"));
+        if ((mixed || i8) && lead->int8_factor) {       // the mixed forms' int8 block products
+            fr = as<char>(lead->get(lead->hfr, (size_t)nb * mgw::fact_slice_bytes(np, scheme),
+                                    "factor row slices"));
+            fc = as<char>(lead->get(lead->hfc, (size_t)nb * mgw::fact_slice_bytes(np, scheme),
+                                    "factor column slices"));
         }
-        double* tbuf = as<double>(ctx->get(ctx->htbuf, (size_t)(fr ? 2 : 1) * nb * fp * 64 * 8,
-                                           "trtri scratch"));
+        double* tbuf = as<double>(lead->get(lead->htbuf, (size_t)(fr ? 2 : 1) * nb * fp * 64 * 8,
+                                            "trtri scratch"));
         double* qpart = as<double>(
-            ctx->get(ctx->hqpart, (size_t)nb * (fp / mgw::kCT) * mpad * 8, "qpart"));
+            lead->get(lead->hqpart, (size_t)nb * (fp / mgw::kCT) * mpad * 8, "qpart"));
         uint16_t *mh = nullptr, *ml = nullptr;
         double* ms = nullptr;
         if (mixed) {
-            mh = as<uint16_t>(ctx->get(ctx->mhi, (size_t)nb * np * np * 2, "M hi"));
-            ml = as<uint16_t>(ctx->get(ctx->mlo, (size_t)nb * np * np * 2, "M lo"));
-            ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
+            mh = as<uint16_t>(lead->get(lead->mhi, (size_t)nb * np * np * 2, "M hi"));
+            ml = as<uint16_t>(lead->get(lead->mlo, (size_t)nb * np * np * 2, "M lo"));
+            ms = as<double>(lead->get(lead->mscale, (size_t)nb * np * 8, "M scale"));
         }
         if (i8) {
-            mh = as<uint16_t>(ctx->get(ctx->mhi, (size_t)nb * digits * np * np, "M slices"));
-            ms = as<double>(ctx->get(ctx->mscale, (size_t)nb * np * 8, "M scale"));
+            mh = as<uint16_t>(lead->get(lead->mhi, (size_t)nb * digits * np * np, "M slices"));
+            ms = as<double>(lead->get(lead->mscale, (size_t)nb * np * 8, "M scale"));
         }
-        hip_check(hipMemsetAsync(info, 0, (size_t)nfreq * 4, st), "hipMemsetAsync");
+        int* info = as<int>(lead->get(lead->hinfob, (size_t)nb * 4, "batch info"));
+        hip_check(hipMemsetAsync(info, 0, (size_t)nb * 4, st), "hipMemsetAsync");
+        const mgw::Split16 fmt = mgw::Split16::FP16;
 
+        hipEvent_t ev[8];
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        struct EvGuard {
+            hipEvent_t* e;
+            ~EvGuard() {
+                for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
+            }
+        } evg{ev};
         float t_diel = 0, t_fact = 0, t_contr = 0, t_aux = 0, t_graph = 0;
         int n_contr = 0;
         // One frequency batch: kernels only, so that it can also be captured.
-        auto enqueue_batch = [&](int b, bool timed) {
+        // Per molecule the frequency offset is read on the device (its f0p).
+        auto enqueue_batch = [&](const std::vector<Seg>& segs, int B, bool timed) {
             if (cplx) {
+                const Seg& g = segs[0];
+                MolRun& r = *g.r;
                 if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-                hip_check(mgw::launch_scale_grid_c(ddelta, dom, job->eta, S, n, np, f0p, b, st),
+                hip_check(mgw::launch_scale_grid_c(r.ddelta, r.dom, r.job->eta, S, r.n, np, r.f0p, B,
+                                                   st),
                           "scale (complex)");
-                hip_check(mgw::launch_dielectric_c(dV, S, A, np, b, st), "dielectric (complex)");
+                hip_check(mgw::launch_dielectric_c(r.dV, S, A, np, B, st), "dielectric (complex)");
                 if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-                hip_check(mgw::launch_potrf_c(A, dinv, info, np, b, st), "potrf (complex)");
+                hip_check(mgw::launch_potrf_c(A, dinv, info, np, B, st), "potrf (complex)");
                 if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
-                hip_check(mgw::launch_make_y(S, xp, Y, np, mpad, b, st), "S X (complex)");
+                hip_check(mgw::launch_make_y(S, r.xp, Y, np, r.mpad, B, st), "S X (complex)");
                 if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
-                hip_check(mgw::launch_fsub_c(A, dinv, Y, np, mpad, b, st), "L^-1 S X (complex)");
+                hip_check(mgw::launch_fsub_c(A, dinv, Y, np, r.mpad, B, st), "L^-1 S X (complex)");
                 if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
-                hip_check(mgw::launch_sumsq_c(Y, dcolmap, dq, dqi, np, m, mpad, f0p, b, st),
+                hip_check(mgw::launch_sumsq_c(Y, r.dcolmap, r.dq, r.dqi, np, r.m, r.mpad, r.f0p, B, st),
                           "q (complex)");
                 if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
                 return;
             }
             if (cd) {
+                const Seg& g = segs[0];
+                MolRun& r = *g.r;
                 if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-                hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
-                hip_check(mgw::launch_aux_dielectric(cdBt, S, A, fp, np, b, st),
+                hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, S, r.n, np, r.f0p, B, st),
+                          "scale");
+                hip_check(mgw::launch_aux_dielectric(r.cdBt, S, A, fp, np, B, st),
                           "pair-space dielectric");
                 if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-                factor_batch(ctx, A, dinv, info, tbuf, fp, b);
+                factor_batch(lead, A, dinv, info, tbuf, fp, B);
                 if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
                 if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
-                hip_check(mgw::launch_contract_f64(A, nullptr, cdLx, qpart, fp, mpad, b, st),
+                hip_check(mgw::launch_contract_f64(A, nullptr, r.cdLx, qpart, fp, r.mpad, B, st),
                           "contract (pair space)");
                 if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
-                hip_check(mgw::launch_reduce_q_cd(qpart, cdnrm, dcolmap, dq, fp / mgw::kCT, m, mpad,
-                                                  f0p, b, st),
+                hip_check(mgw::launch_reduce_q_cd(qpart, r.cdnrm, r.dcolmap, r.dq, fp / mgw::kCT, r.m,
+                                                  r.mpad, r.f0p, B, st),
                           "reduce q (pair space)");
                 if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
                 return;
             }
             if (timed) hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-            hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, b, st), "scale");
-            hip_check(mgw::launch_dielectric(dV, S, A, np, b, st), "dielectric");
+            for (const Seg& g : segs) {
+                MolRun& r = *g.r;
+                double* Sg = S + (size_t)g.off * np;
+                hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, Sg, r.n, np, r.f0p, g.b, st),
+                          "scale");
+                hip_check(mgw::launch_dielectric(r.dV, Sg, A + (size_t)g.off * np * np, np, g.b, st),
+                          "dielectric");
+            }
             if (timed) hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
-            factor_batch(ctx, A, dinv, info, tbuf, np, b, fr, fc, scheme);
+            factor_batch(lead, A, dinv, info, tbuf, np, B, fr, fc, scheme);
             if (timed) hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
             if (mixed)
-                hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, b, st), "split M");
+                hip_check(mgw::launch_split_m(fmt, A, S, mh, ml, ms, np, B, st), "split M");
             if (i8)
-                hip_check(mgw::launch_slice_m(A, S, reinterpret_cast<int8_t*>(mh), ms, np, b, scheme,
+                hip_check(mgw::launch_slice_m(A, S, reinterpret_cast<int8_t*>(mh), ms, np, B, scheme,
                                               st),
                           "slice M");
             if (timed) hip_check(hipEventRecord(ev[6], st), "hipEventRecord");
-            if (mixed)
-                hip_check(mgw::launch_contract_split(fmt, mh, ml, ms, xh, xl, qpart, np, mpad, b,
-                                                     uf, st),
-                          "contract split");
-            else if (i8)
-                hip_check(mgw::launch_contract_i8(reinterpret_cast<const int8_t*>(mh), ms,
-                                                  reinterpret_cast<const int8_t*>(xh), qpart, np,
-                                                  mpad, b, uf, scheme, st),
-                          "contract int8");
-            else
-                hip_check(mgw::launch_contract_f64(A, S, xp, qpart, np, mpad, b, st), "contract");
+            for (const Seg& g : segs) {
+                MolRun& r = *g.r;
+                double* qg = qpart + g.qoff;
+                const size_t o = (size_t)g.off;
+                if (mixed)
+                    hip_check(mgw::launch_contract_split(fmt, mh + o * np * np, ml + o * np * np,
+                                                         ms + o * np, r.xh, r.xl, qg, np, r.mpad,
+                                                         g.b, uf, st),
+                              "contract split");
+                else if (i8)
+                    hip_check(mgw::launch_contract_i8(reinterpret_cast<const int8_t*>(mh) +
+                                                          o * digits * np * np,
+                                                      ms + o * np, reinterpret_cast<const int8_t*>(r.xh),
+                                                      qg, np, r.mpad, g.b, uf, scheme, st),
+                              "contract int8");
+                else
+                    hip_check(mgw::launch_contract_f64(A + o * np * np, S + o * np, r.xp, qg, np,
+                                                       r.mpad, g.b, st),
+                              "contract");
+            }
             if (timed) hip_check(hipEventRecord(ev[7], st), "hipEventRecord");
-            if (mixed || i8)
-                hip_check(mgw::launch_reduce_q_scaled(qpart, xs, dcolmap, dq,
-                                                      mixed ? mgw::contract_split_row_tiles(np)
-                                                            : mgw::contract_i8_row_tiles(np),
-                                                      m, mpad, f0p, b, st),
-                          "reduce q");
-            else
-                hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0p, b, st),
-                          "reduce q");
+            for (const Seg& g : segs) {
+                MolRun& r = *g.r;
+                double* qg = qpart + g.qoff;
+                if (mixed || i8)
+                    hip_check(mgw::launch_reduce_q_scaled(qg, r.xs, r.dcolmap, r.dq,
+                                                          mixed ? mgw::contract_split_row_tiles(np)
+                                                                : mgw::contract_i8_row_tiles(np),
+                                                          r.m, r.mpad, r.f0p, g.b, st),
+                              "reduce q");
+                else
+                    hip_check(mgw::launch_reduce_q(qg, r.dcolmap, r.dq, np / mgw::kCT, r.m, r.mpad,
+                                                   r.f0p, g.b, st),
+                              "reduce q");
+            }
             if (timed) hip_check(hipEventRecord(ev[5], st), "hipEventRecord");
         };
-        // Factorization breakdowns: the kernels flag frequency w of a batch in
-        // info[w]; after each batch the flags are read, recorded as frequency
-        // f0 + w and cleared for the next batch (outside any captured graph).
-        std::vector<int> failed, hinfo(nb);
-        auto collect_info = [&](int f0, int b) {
-            ctx->d2h(hinfo.data(), info, b);
-            ctx->sync();
+        // Factorization breakdowns: the kernels flag matrix w of a batch in
+        // info[w]; after each batch the flags are read, recorded as the
+        // molecule's frequency f0 + (w - off) and cleared for the next batch
+        // (outside any captured graph).
+        std::vector<int> hinfo(nb);
+        auto collect_info = [&](const std::vector<Seg>& segs, int B) {
+            lead->d2h(hinfo.data(), info, B);
+            lead->sync();
             bool any = false;
-            for (int w = 0; w < b; ++w)
-                if (hinfo[w] != 0) {
-                    failed.push_back(f0 + w);
-                    any = true;
-                }
-            if (any) hip_check(hipMemsetAsync(info, 0, (size_t)b * 4, st), "hipMemsetAsync");
+            for (const Seg& g : segs)
+                for (int w = 0; w < g.b; ++w)
+                    if (hinfo[g.off + w] != 0) {
+                        g.r->failed.push_back(g.f0 + w);
+                        any = true;
+                    }
+            if (any) hip_check(hipMemsetAsync(info, 0, (size_t)B * 4, st), "hipMemsetAsync");
         };
-        for (int f0 = 0; f0 < nfreq; f0 += nb) {
-            if (ctx->cancel && __atomic_load_n(ctx->cancel, __ATOMIC_ACQUIRE) != 0) {
-                ctx->sync();
-                fail(MGW_ERR_STATE, "mgw_hip_g0w0: cancelled after %d of %d frequencies", f0, nfreq);
+        // batches over the molecules' frequencies in order
+        size_t mi = 0;
+        int fi = 0, done = 0;
+        while (done < total) {
+            // a molecule alone stops between batches when cancelled (a grouped
+            // one is dropped after its group's batches)
+            if (mols.size() == 1 && r0.ctx->cancel &&
+                __atomic_load_n(r0.ctx->cancel, __ATOMIC_ACQUIRE) != 0) {
+                lead->sync();
+                fail(MGW_ERR_STATE, "mgw_hip_g0w0: cancelled after %d of %d frequencies", done,
+                     r0.nfreq);
             }
-            const int b = std::min(nb, nfreq - f0);
-            hip_check(mgw::launch_set_int(f0p, f0, st), "set offset");
-            if (ctx->use_graphs && !cplx) {
+            const int B = std::min(nb, total - done);
+            std::vector<Seg> segs;
+            size_t qoff = 0;
+            for (int off = 0; off < B;) {
+                MolRun* r = mols[mi];
+                const int b = std::min(B - off, r->nfreq - fi);
+                segs.push_back({r, fi, b, off, qoff});
+                qoff += (size_t)b * (fp / mgw::kCT) * r->mpad;
+                off += b;
+                fi += b;
+                if (fi == r->nfreq) {
+                    ++mi;
+                    fi = 0;
+                }
+            }
+            for (const Seg& g : segs)
+                hip_check(mgw::launch_set_int(g.r->f0p, g.f0, st), "set offset");
+            if (lead->use_graphs && !cplx) {
                 // Replay a graph of this batch when one with the same shape and
-                // buffers was captured before (the offset is read on the device).
-                const GraphKey key{n, np, fp, mpad, m, b, job->precision, uf,
-                                   {A, S, dinv, tbuf, info, qpart, dq, xp, xh, xl, xs, mh, ml, ms, dV,
-                                    ddelta, dom, dcolmap, f0p, fr, fc}};
-                hipGraphExec_t exec = ctx->find_graph(key);
+                // buffers was captured before (the offsets are read on the device).
+                GraphKey key;
+                key.sig = {(uint64_t)np, (uint64_t)fp, (uint64_t)B, (uint64_t)precision, (uint64_t)uf,
+                           (uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)S, (uint64_t)(uintptr_t)dinv,
+                           (uint64_t)(uintptr_t)tbuf, (uint64_t)(uintptr_t)info,
+                           (uint64_t)(uintptr_t)qpart, (uint64_t)(uintptr_t)mh,
+                           (uint64_t)(uintptr_t)ml, (uint64_t)(uintptr_t)ms, (uint64_t)(uintptr_t)fr,
+                           (uint64_t)(uintptr_t)fc};
+                for (const Seg& g : segs) {
+                    const MolRun& r = *g.r;
+                    for (uint64_t v : {(uint64_t)r.n, (uint64_t)r.mpad, (uint64_t)r.m, (uint64_t)g.b,
+                                       (uint64_t)g.off, (uint64_t)g.qoff,
+                                       (uint64_t)(uintptr_t)r.dq, (uint64_t)(uintptr_t)r.xp,
+                                       (uint64_t)(uintptr_t)r.xh, (uint64_t)(uintptr_t)r.xl,
+                                       (uint64_t)(uintptr_t)r.xs, (uint64_t)(uintptr_t)r.dV,
+                                       (uint64_t)(uintptr_t)r.ddelta, (uint64_t)(uintptr_t)r.dom,
+                                       (uint64_t)(uintptr_t)r.dcolmap, (uint64_t)(uintptr_t)r.f0p,
+                                       (uint64_t)(uintptr_t)r.cdLx, (uint64_t)(uintptr_t)r.cdnrm})
+                        key.sig.push_back(v);
+                }
+                hipGraphExec_t exec = lead->find_graph(key);
                 if (!exec) {
                     hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
                               "hipStreamBeginCapture");
-                    enqueue_batch(b, false);
+                    enqueue_batch(segs, B, false);
                     hipGraph_t graph = nullptr;
                     hip_check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
                     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(graph);
                     hip_check(e, "hipGraphInstantiate");
-                    ctx->graphs.push_back({key, exec});
+                    lead->graphs.push_back({key, exec});
                 }
                 hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
                 hip_check(hipGraphLaunch(exec, st), "hipGraphLaunch");
@@ -1592,51 +1754,76 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 float a = 0;
                 hip_check(hipEventElapsedTime(&a, ev[2], ev[5]), "hipEventElapsedTime");
                 t_graph += a;
-                ++n_contr;
-                collect_info(f0, b);
-                continue;
-            }
-            {
-                Range batch_range("build W_c + sigma_c (frequency batch)");
-                enqueue_batch(b, true);
-                hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
-            }
-            float a = 0, bb = 0, c = 0, d = 0, e = 0;
-            hip_check(hipEventElapsedTime(&a, ev[2], ev[3]), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&bb, ev[3], ev[4]), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&c, ev[6], ev[7]), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&d, ev[4], ev[6]), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&e, ev[7], ev[5]), "hipEventElapsedTime");
-            t_diel += a;
-            t_fact += bb;
-            t_contr += c;
-            t_aux += d + e;
-            ++n_contr;
-            collect_info(f0, b);
-        }
-        // Frequencies whose symmetrised dielectric matrix is not positive
-        // definite: the plain FP64 form screens them by pivoted LU (their q rows
-        // are replaced); the other forms report the breakdown (the mixed forms'
-        // judge then reruns the job in FP64).
-        ctx->lu_freqs = 0;
-        {
-            if (!failed.empty()) {
-                if (cplx || cd || mixed || i8)
-                    fail(MGW_ERR_NUMERIC,
-                         "potrf: symmetrised dielectric matrix %d is not positive definite",
-                         failed[0]);
-                double* W = as<double>(ctx->get(ctx->luz, (size_t)np * np * 8, "E^-1"));
-                for (int f : failed) {
-                    hip_check(mgw::launch_set_int(f0p, f, st), "set offset");
-                    hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, 1, st), "scale");
-                    lu_screen(ctx, dV, S, A, W, np, 1);
-                    hip_check(mgw::launch_contract_dense(W, S, xp, qpart, np, mpad, 1, st),
-                              "contract (dense)");
-                    hip_check(mgw::launch_reduce_q(qpart, dcolmap, dq, np / mgw::kCT, m, mpad, f0p, 1, st),
-                              "reduce q");
+            } else {
+                {
+                    Range batch_range("build W_c + sigma_c (frequency batch)");
+                    enqueue_batch(segs, B, true);
+                    hip_check(hipEventSynchronize(ev[5]), "hipEventSynchronize");
                 }
-                ctx->lu_freqs = (int)failed.size();
+                float a = 0, bb = 0, c = 0, d = 0, e = 0;
+                hip_check(hipEventElapsedTime(&a, ev[2], ev[3]), "hipEventElapsedTime");
+                hip_check(hipEventElapsedTime(&bb, ev[3], ev[4]), "hipEventElapsedTime");
+                hip_check(hipEventElapsedTime(&c, ev[6], ev[7]), "hipEventElapsedTime");
+                hip_check(hipEventElapsedTime(&d, ev[4], ev[6]), "hipEventElapsedTime");
+                hip_check(hipEventElapsedTime(&e, ev[7], ev[5]), "hipEventElapsedTime");
+                t_diel += a;
+                t_fact += bb;
+                t_contr += c;
+                t_aux += d + e;
             }
+            ++n_contr;
+            collect_info(segs, B);
+            done += B;
+        }
+        for (MolRun* r : mols) {
+            r->t_diel = t_diel;
+            r->t_fact = t_fact;
+            r->t_contr = t_contr;
+            r->t_aux = t_aux;
+            r->t_graph = t_graph;
+            r->n_contr = n_contr;
+            r->nb = nb;
+        }
+}
+
+// Phase 3 (on the molecule's own context and stream): frequencies whose
+// dielectric matrix was not positive definite re-screened by pivoted LU
+// (plain FP64 form; the other forms report the breakdown), the error judge of
+// the mixed forms, Sigma_c, Sigma_x, Pade/QP, downloads and timings.  Returns
+// false, having written nothing to `out`, when the judge failed and the
+// context asks for the FP64 fallback.
+bool mol_finish(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out) {
+        const mgw_g0w0_job* job = r.job;
+        const int nmo = r.nmo, nfreq = r.nfreq, npar = r.npar, ns = r.ns, n = r.n, np = r.np;
+        const int m = r.m, mpad = r.mpad;
+        const bool i8 = r.i8, mixed = r.mixed;
+        hipStream_t st = ctx->stream;
+        // one frequency's buffers of the batch phase (the context's batch
+        // buffers when it led the batches, else allocated here)
+        auto one = [&](mgw_hip_ctx::Buf& b, size_t bytes, const char* name) {
+            return as<double>(ctx->get(b, std::max(bytes, b.bytes), name));
+        };
+        ctx->lu_freqs = 0;
+        std::sort(r.failed.begin(), r.failed.end());
+        if (!r.failed.empty()) {
+            if (r.cplx || r.cd || mixed || i8)
+                fail(MGW_ERR_NUMERIC, "potrf: symmetrised dielectric matrix %d is not positive definite",
+                     r.failed[0]);
+            double* A = one(ctx->hA, (size_t)np * np * 8, "dielectric batch");
+            double* S = one(ctx->hS, (size_t)np * 8, "scale batch");
+            double* qpart = one(ctx->hqpart, (size_t)(np / mgw::kCT) * mpad * 8, "qpart");
+            double* W = as<double>(ctx->get(ctx->luz, (size_t)np * np * 8, "E^-1"));
+            for (int f : r.failed) {
+                hip_check(mgw::launch_set_int(r.f0p, f, st), "set offset");
+                hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, S, n, np, r.f0p, 1, st), "scale");
+                lu_screen(ctx, r.dV, S, A, W, np, 1);
+                hip_check(mgw::launch_contract_dense(W, S, r.xp, qpart, np, mpad, 1, st),
+                          "contract (dense)");
+                hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, r.dq, np / mgw::kCT, m, mpad, r.f0p, 1,
+                                               st),
+                          "reduce q");
+            }
+            ctx->lu_freqs = (int)r.failed.size();
         }
         ctx->wc_ready = false;
 
@@ -1650,22 +1837,27 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                                : job->precision == MGW_PREC_MIXED_INT8 ? 1e-7
                                : i8                                   ? 1e-6
                                                                       : 1e-5;
+            double* A = one(ctx->hA, (size_t)np * np * 8, "dielectric batch");
+            double* S = one(ctx->hS, (size_t)np * 8, "scale batch");
+            double* dinv = one(ctx->hdinv, (size_t)(np / mgw::kNB) * 64 * 64 * 8, "dinv");
+            double* tbuf = one(ctx->htbuf, (size_t)np * 64 * 8, "trtri scratch");
+            double* qpart = one(ctx->hqpart, (size_t)(np / mgw::kCT) * mpad * 8, "qpart");
             double* qj = as<double>(ctx->get(ctx->hqj, (size_t)m * 8, "judge q"));
             std::vector<double> qa(m), qb(m);
             double worst = 0.0;
             for (int i = 0; i < nj; ++i) {
                 const int f = (int)(((int64_t)2 * i + 1) * nfreq / (2 * nj));
-                hip_check(mgw::launch_set_int(f0p, f, st), "set offset");
-                hip_check(mgw::launch_set_int(f0p + 1, 0, st), "set offset");
-                hip_check(mgw::launch_scale_from_grid(ddelta, dom, S, n, np, f0p, 1, st), "scale");
-                hip_check(mgw::launch_dielectric(dV, S, A, np, 1, st), "dielectric");
-                factor_batch(ctx, A, dinv, info + f, tbuf, np, 1);
-                hip_check(mgw::launch_contract_f64(A, S, xj, qpart, np, mpad, 1, st), "contract");
-                hip_check(mgw::launch_reduce_q(qpart, dcolmap, qj, np / mgw::kCT, m, mpad, f0p + 1, 1,
-                                               st),
+                hip_check(mgw::launch_set_int(r.f0p, f, st), "set offset");
+                hip_check(mgw::launch_set_int(r.f0p + 1, 0, st), "set offset");
+                hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, S, n, np, r.f0p, 1, st), "scale");
+                hip_check(mgw::launch_dielectric(r.dV, S, A, np, 1, st), "dielectric");
+                factor_batch(ctx, A, dinv, r.info + f, tbuf, np, 1);
+                hip_check(mgw::launch_contract_f64(A, S, r.xj, qpart, np, mpad, 1, st), "contract");
+                hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, qj, np / mgw::kCT, m, mpad, r.f0p + 1,
+                                               1, st),
                           "reduce q");
                 ctx->d2h(qa.data(), qj, m);
-                ctx->d2h(qb.data(), dq + (size_t)f * m, m);
+                ctx->d2h(qb.data(), r.dq + (size_t)f * m, m);
                 ctx->sync();
                 double e = 0.0, sc = 0.0;
                 for (int c = 0; c < m; ++c) {
@@ -1675,89 +1867,191 @@ bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
                 const double rel = sc > 0.0 ? e / sc : (e > 0.0 ? INFINITY : 0.0);
                 worst = std::max(worst, std::isnan(rel) ? INFINITY : rel);
             }
-            check_info(ctx, info, nfreq, "potrf (judge)");
+            check_info(ctx, r.info, nfreq, "potrf (judge)");
             ctx->judge_err = worst;
             ctx->judge_used = nj;
             if (!(worst <= tol) && ctx->judge_fallback) return false;
         }
-        if (out->q) ctx->d2h(out->q, dq, (size_t)nfreq * m);
+        if (out->q) ctx->d2h(out->q, r.dq, (size_t)nfreq * m);
 
-        hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+        hipEvent_t ev[3];
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        struct EvGuard {
+            hipEvent_t* e;
+            ~EvGuard() {
+                for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+            }
+        } evg{ev};
+        hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
         {
-            Range r("sigma_c wall");
-            hip_check(mgw::launch_sigma_c(dq, dqi, dmo, dom, dwt, job->fermi_energy, nmo, ns, nfreq,
-                                          gq, sre, sim, st),
+            Range rg("sigma_c wall");
+            hip_check(mgw::launch_sigma_c(r.dq, r.dqi, r.dmo, r.dom, r.dwt, job->fermi_energy, nmo, ns,
+                                          nfreq, r.gq, r.sre, r.sim, st),
                       "sigma_c");
         }
-        hip_check(hipEventRecord(ev[3], st), "hipEventRecord");
+        hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
         {
-            Range r("exchange");
-            hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), sx, nmo, nocc, st), "sigma_x");
+            Range rg("exchange");
+            hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), r.sx, nmo, r.nocc, st), "sigma_x");
         }
         {
-            Range r("pade/output pade");
+            Range rg("pade/output pade");
             std::vector<double> nan(nmo, std::numeric_limits<double>::quiet_NaN());
             std::vector<int> zero(nmo, 0);
-            ctx->h2d(qp, nan.data(), nmo);
-            ctx->h2d(iters, zero.data(), nmo);
-            hip_check(mgw::launch_pade_qp(sre, sim, dom, dmo, sx, dvxc, dstates, job->fermi_energy,
-                                          ns, nfreq, npar, qp, iters, st),
+            ctx->h2d(r.qp, nan.data(), nmo);
+            ctx->h2d(r.iters, zero.data(), nmo);
+            hip_check(mgw::launch_pade_qp(r.sre, r.sim, r.dom, r.dmo, r.sx, r.dvxc, r.dstates,
+                                          job->fermi_energy, ns, nfreq, npar, r.qp, r.iters, st),
                       "pade/qp");
-            hip_check(hipEventRecord(ev[4], st), "hipEventRecord");
-            if (out->sigma_c_re) ctx->d2h(out->sigma_c_re, sre, (size_t)ns * nfreq);
-            if (out->sigma_c_im) ctx->d2h(out->sigma_c_im, sim, (size_t)ns * nfreq);
-            if (out->sigma_x_diag) ctx->d2h(out->sigma_x_diag, sx, nmo);
-            ctx->d2h(out->qp, qp, nmo);
-            if (out->qp_iterations) ctx->d2h(out->qp_iterations, iters, nmo);
+            hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
+            if (out->sigma_c_re) ctx->d2h(out->sigma_c_re, r.sre, (size_t)ns * nfreq);
+            if (out->sigma_c_im) ctx->d2h(out->sigma_c_im, r.sim, (size_t)ns * nfreq);
+            if (out->sigma_x_diag) ctx->d2h(out->sigma_x_diag, r.sx, nmo);
+            ctx->d2h(out->qp, r.qp, nmo);
+            if (out->qp_iterations) ctx->d2h(out->qp_iterations, r.iters, nmo);
             ctx->sync();
         }
-        if (out->omega) std::copy(omega.begin(), omega.end(), out->omega);
-        if (out->weight) std::copy(weight.begin(), weight.end(), out->weight);
+        if (out->omega) std::copy(r.omega.begin(), r.omega.end(), out->omega);
+        if (out->weight) std::copy(r.weight.begin(), r.weight.end(), out->weight);
         float tv = 0, tsig = 0, tqp = 0, ttot = 0;
-        hip_check(hipEventElapsedTime(&tv, ev[0], ev[1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&tsig, ev[2], ev[3]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&tqp, ev[3], ev[4]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&ttot, ev[0], ev[4]), "hipEventElapsedTime");
-        const double t[7] = {tv, t_diel, t_fact, t_contr, tsig, tqp, ttot};
+        hip_check(hipEventElapsedTime(&tv, r.ev[0], r.ev[1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&tsig, ev[0], ev[1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&tqp, ev[1], ev[2]), "hipEventElapsedTime");
+        // total: from the start of the molecule's setup to its QP energies
+        hip_check(hipEventElapsedTime(&ttot, r.ev[0], ev[2]), "hipEventElapsedTime");
+        const double t[7] = {tv, r.t_diel, r.t_fact, r.t_contr, tsig, tqp, ttot};
         std::copy(t, t + 7, ctx->timings);
-        ctx->timings[7] = nb;
-        ctx->timings[8] = n_contr;
-        ctx->timings[9] = t_aux;
-        ctx->timings[10] = mu;
-        ctx->timings[11] = t_graph;
-        ctx->timings[12] = cd_rank;
-        ctx->timings[13] = cd_ms;
+        ctx->timings[7] = r.nb;
+        ctx->timings[8] = r.n_contr;
+        ctx->timings[9] = r.t_aux;
+        ctx->timings[10] = r.mu;
+        ctx->timings[11] = r.t_graph;
+        ctx->timings[12] = r.cd_rank;
+        ctx->timings[13] = r.cd_ms;
         ctx->timings[14] = ctx->lu_freqs;
         return true;
+}
+
+// One molecule on its own context.
+bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
+    MolRun r;
+    mol_prepare(r, ctx, job, out);
+    run_batches(ctx, {&r});
+    return mol_finish(r, ctx, out);
+}
+
+// The FP64 fallback of a mixed form: a failed judge (returns false) or a
+// factorization breakdown under an active judge reruns the job in FP64.
+void with_fallback(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out,
+                   const std::function<bool()>& run) {
+    ctx->fell_back = 0;
+    ctx->judge_err = -1.0;
+    ctx->judge_used = 0;
+    const bool mixed = mixed_form(job->precision);
+    bool done = false;
+    try {
+        done = run();
+    } catch (const MgwError& e) {
+        // a mixed form's factorization breakdown falls back like a failed judge
+        if (!(mixed && ctx->judge_n > 0 && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC)) throw;
+        ctx->fell_back = 2;
+    }
+    if (done) return;
+    if (!ctx->fell_back) ctx->fell_back = 1;
+    mgw_g0w0_job plain = *job;
+    plain.precision = MGW_PREC_FP64;
+    const double err = ctx->judge_err;
+    const int used = ctx->judge_used;
+    g0w0_body(ctx, &plain, out);
+    ctx->judge_err = err;
+    ctx->judge_used = used;
 }
 }  // namespace
 
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
     return guarded(ctx, [&] {
         need(ctx && job && out, "mgw_hip_g0w0: NULL argument");
-        ctx->fell_back = 0;
-        ctx->judge_err = -1.0;
-        ctx->judge_used = 0;
-        const bool mixed = mixed_form(job->precision);
-        bool done = false;
-        try {
-            done = g0w0_body(ctx, job, out);
-        } catch (const MgwError& e) {
-            // a mixed form's factorization breakdown falls back like a failed judge
-            if (!(mixed && ctx->judge_n > 0 && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC))
-                throw;
-            ctx->fell_back = 2;
-        }
-        if (done) return;
-        if (!ctx->fell_back) ctx->fell_back = 1;
-        mgw_g0w0_job plain = *job;
-        plain.precision = MGW_PREC_FP64;
-        const double err = ctx->judge_err;
-        const int used = ctx->judge_used;
-        g0w0_body(ctx, &plain, out);
-        ctx->judge_err = err;
-        ctx->judge_used = used;
+        with_fallback(ctx, job, out, [&] { return g0w0_body(ctx, job, out); });
     });
+}
+
+int mgw_hip_g0w0_group(mgw_hip_ctx* const* ctxs, const mgw_g0w0_job* jobs, mgw_g0w0_out* outs,
+                       int32_t count, int32_t* status) {
+    mgw_hip_ctx* lead = ctxs && count > 0 ? ctxs[0] : nullptr;
+    std::vector<int> st(count > 0 ? count : 0, MGW_OK);
+    std::vector<std::string> msg(st.size());
+    const int rc = guarded(lead, [&] {
+        need(ctxs && jobs && outs && count > 0, "mgw_hip_g0w0_group: NULL argument or count < 1");
+        for (int i = 0; i < count; ++i) {
+            need(ctxs[i] != nullptr, "mgw_hip_g0w0_group: context %d is NULL", i);
+            need(ctxs[i]->device == lead->device,
+                 "mgw_hip_g0w0_group: the contexts must share one device");
+            for (int k = 0; k < i; ++k)
+                need(ctxs[k] != ctxs[i],
+                     "mgw_hip_g0w0_group: context %d appears twice (a context holds one resident ERI)",
+                     i);
+        }
+        auto job_error = [&](int i, const MgwError& e) {
+            st[i] = e.code;
+            msg[i] = e.what();
+        };
+        std::vector<std::unique_ptr<MolRun>> runs(count);
+        for (int i = 0; i < count; ++i) {
+            runs[i] = std::make_unique<MolRun>();
+            try {
+                mol_prepare(*runs[i], ctxs[i], &jobs[i], &outs[i]);
+            } catch (const MgwError& e) {
+                job_error(i, e);
+            }
+        }
+        for (int i = 0; i < count; ++i)
+            if (st[i] == MGW_OK) ctxs[i]->sync();
+        // molecules of one padded order and form share the frequency batches;
+        // eta != 0 and the decomposed form run alone, each on its own context
+        std::vector<std::vector<MolRun*>> groups;
+        std::map<std::pair<int, int>, size_t> slot;
+        for (int i = 0; i < count; ++i) {
+            if (st[i] != MGW_OK) continue;
+            MolRun* r = runs[i].get();
+            if (r->cplx || r->cd) {
+                groups.push_back({r});
+                continue;
+            }
+            const auto key = std::make_pair(r->np, r->job->precision);
+            auto it = slot.find(key);
+            if (it == slot.end()) {
+                slot[key] = groups.size();
+                groups.push_back({r});
+            } else {
+                groups[it->second].push_back(r);
+            }
+        }
+        for (auto& g : groups) run_batches(g.size() == 1 ? g[0]->ctx : lead, g);
+        for (int i = 0; i < count; ++i) {
+            if (st[i] != MGW_OK) continue;
+            mgw_hip_ctx* c = ctxs[i];
+            try {
+                // a job cancelled while its group ran keeps no result
+                if (c->cancel && __atomic_load_n(c->cancel, __ATOMIC_ACQUIRE) != 0)
+                    fail(MGW_ERR_STATE, "mgw_hip_g0w0: cancelled during its group's frequency batches");
+                with_fallback(c, &jobs[i], &outs[i], [&] { return mol_finish(*runs[i], c, &outs[i]); });
+            } catch (const MgwError& e) {
+                job_error(i, e);
+            }
+        }
+    });
+    if (rc != MGW_OK) std::fill(st.begin(), st.end(), rc);
+    int first = rc;
+    for (size_t i = 0; i < st.size(); ++i) {
+        if (rc == MGW_OK && st[i] != MGW_OK) {
+            ctxs[i]->error = msg[i];
+            if (first == MGW_OK) first = st[i];
+        } else if (rc == MGW_OK && ctxs[i]) {
+            ctxs[i]->error.clear();
+        }
+        if (status) status[i] = st[i];
+    }
+    return first;
 }
 
 int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_t* states,

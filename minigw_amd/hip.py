@@ -86,6 +86,8 @@ def _load():
         "mgw_hip_quadratic_forms_device": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, _dp,
                                                      _dp]),
         "mgw_hip_g0w0": (C.c_int, [vp, C.POINTER(G0W0Job), C.POINTER(G0W0Out)]),
+        "mgw_hip_g0w0_group": (C.c_int, [C.POINTER(vp), C.POINTER(G0W0Job), C.POINTER(G0W0Out),
+                                         C.c_int32, _ip]),
         "mgw_hip_pade_qp": (C.c_int, [vp, C.c_int32, C.c_int32, _ip, C.c_int32, C.c_int32, _dp,
                                       _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _ip]),
         "mgw_hip_last_timings": (C.c_int, [vp, _dp, C.c_int32]),
@@ -418,6 +420,63 @@ def run_g0w0(inp, nfreq: int = 200, npar: int = 16, states=None, eta: float = 0.
     ctx.call("mgw_hip_g0w0", C.byref(job), C.byref(out))
     return HipG0W0Result(om, w, list(range(nmo)) if st is None else st.tolist(), sx,
                          sre + 1j * sim, qp, it, ctx.last_timings(), q)
+
+
+def _job_and_out(inp, nfreq, npar, states, eta, precision, want_q):
+    """The C job and output records of one molecule, and the arrays they point to."""
+    nmo = inp.nmo
+    st = None if states is None else np.ascontiguousarray(states, dtype=np.int32)
+    ns = nmo if st is None else len(st)
+    mo = _f64(inp.mo_energy)
+    vxc = _f64(np.diag(inp.vxc))
+    job = G0W0Job(nmo, inp.nocc, nfreq, npar, 0 if st is None else ns, _ptr(st, _ip),
+                  _ptr(mo), _ptr(vxc), float(inp.fermi_energy), float(eta), precision)
+    arr = dict(st=st, mo=mo, vxc=vxc, om=np.zeros(nfreq), w=np.zeros(nfreq), sx=np.zeros(nmo),
+               sre=np.zeros((ns, nfreq)), sim=np.zeros((ns, nfreq)), qp=np.zeros(nmo),
+               it=np.zeros(nmo, dtype=np.int32), q=np.zeros((nfreq, ns * nmo)) if want_q else None)
+    out = G0W0Out(_ptr(arr["om"]), _ptr(arr["w"]), _ptr(arr["sx"]), _ptr(arr["sre"]),
+                  _ptr(arr["sim"]), _ptr(arr["qp"]), _ptr(arr["it"], _ip), _ptr(arr["q"]))
+    return job, out, arr
+
+
+def _result(arr, nmo, timings) -> "HipG0W0Result":
+    st = arr["st"]
+    return HipG0W0Result(arr["om"], arr["w"], list(range(nmo)) if st is None else st.tolist(),
+                         arr["sx"], arr["sre"] + 1j * arr["sim"], arr["qp"], arr["it"], timings,
+                         arr["q"])
+
+
+def run_g0w0_group(inps, ctxs, nfreq=200, npar=16, precision: int = PREC_FP64, eta=0.0,
+                   states=None, want_q: bool = False):
+    """Several molecules through mgw_hip_g0w0_group: inps[i] on the ERI already
+    resident in ctxs[i] (ctxs[0] leads: its stream and batch buffers run the
+    shared frequency batches).  nfreq, npar, precision, eta and states may be
+    one value for all or a list per molecule.  Returns (results, status):
+    results[i] is None where status[i] != MGW_OK."""
+    k = len(inps)
+    if len(ctxs) != k or k == 0:
+        raise ValueError("one context per molecule")
+
+    def per(v):
+        return list(v) if isinstance(v, (list, tuple)) else [v] * k
+    nf, npr, prec, et = per(nfreq), per(npar), per(precision), per(eta)
+    per_mol = (isinstance(states, list) and len(states) == k
+               and all(v is None or isinstance(v, (list, tuple, np.ndarray)) for v in states))
+    sts = list(states) if per_mol else [states] * k
+    jobs = (G0W0Job * k)()
+    outs = (G0W0Out * k)()
+    keep = []
+    for i, inp in enumerate(inps):
+        jobs[i], outs[i], arr = _job_and_out(inp, nf[i], npr[i], sts[i], et[i], prec[i], want_q)
+        keep.append(arr)
+    handles = (C.c_void_p * k)(*[c.h for c in ctxs])
+    status = np.zeros(k, dtype=np.int32)
+    rc = lib.mgw_hip_g0w0_group(handles, jobs, outs, k, _ptr(status, _ip))
+    if rc != MGW_OK and not np.any(status == MGW_OK):
+        _check("mgw_hip_g0w0_group", rc, ctxs[0].h)
+    res = [(_result(keep[i], inps[i].nmo, ctxs[i].last_timings()) if status[i] == MGW_OK else None)
+           for i in range(k)]
+    return res, status
 
 
 def pade_qp(sigma_c, omegas, sigma_x_diag, mo_energy, vxc_diag, fermi: float, npar: int = 16,
