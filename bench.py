@@ -189,12 +189,16 @@ def _factor_job(nmo: int, nocc: int, seed: int):
     return dict(nmo=nmo, nocc=nocc, mo=mo, fermi=fermi, b=b, vxc=vxc)
 
 
-def run_batch(hip, count: int, workers: int, nfreq: int, npar: int, prec: int) -> dict:
+def run_batch(hip, count: int, workers: int, nfreq: int, npar: int, prec: int,
+              grouping: int = 16) -> dict:
     """Uneven-nmo batch: `count` molecules (nmo log-uniform in [24, 120],
-    nocc = round(nmo / 5)), all submitted at once to the request loop."""
+    nocc = round(nmo / 5)), all submitted at once to the request loop, whose
+    workers group molecules of one padded order (up to `grouping` per group;
+    1: every molecule alone on its worker)."""
     from minigw_amd.synthetic import batch_molecules
     mols = [_factor_job(*m) for m in batch_molecules(count)]
     with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=64) as srv:
+        srv.set_grouping(grouping)
         # warm-up: compile-free, but first-touch allocations and graph capture
         t = srv.submit(factor=mols[0]["b"], mo_energy=mols[0]["mo"], vxc=mols[0]["vxc"],
                        nocc=mols[0]["nocc"], fermi=mols[0]["fermi"], nfreq=nfreq, npar=npar,
@@ -212,7 +216,7 @@ def run_batch(hip, count: int, workers: int, nfreq: int, npar: int, prec: int) -
             "qp_solves_per_s": round(solves / wall, 2), "molecules_per_s": round(count / wall, 3),
             "wall_s": round(wall, 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
             "p99_ms": round(float(np.percentile(lat, 99)), 1), "workers": workers,
-            "peak_concurrent_jobs": st["peak_running"]}
+            "grouping": grouping, "peak_concurrent_jobs": st["peak_running"]}
 
 
 def run_serve(hip, jobs: int, rates, workers: int, npar: int, prec: int) -> list:
@@ -451,6 +455,8 @@ def main() -> int:
     if args.batch_molecules > 0:
         log("bench: uneven-nmo batch ...")
         out["batch"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq, args.npar, prec)
+        out["batch_ungrouped"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq,
+                                           args.npar, prec, grouping=1)
         if cd_too:
             out["batch_decomposed"] = run_batch(hip, args.batch_molecules, args.workers, args.nfreq,
                                                 args.npar, 3)

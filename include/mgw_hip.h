@@ -329,6 +329,14 @@ int mgw_hip_server_destroy(mgw_hip_server* server);
 int mgw_hip_server_set_judge(mgw_hip_server* server, int32_t nsample, double tol,
                              int32_t fallback);
 
+/* Most jobs a worker runs together (1..64, default 16; 1 turns grouping off):
+ * admitted jobs that share a padded particle-hole order and form (eta == 0,
+ * not MGW_PREC_FP64_CD) run through mgw_hip_g0w0_group, their frequencies
+ * stacked in shared batches; each result is bit for bit the job's result
+ * alone.  A grouped job's cancellation takes effect when its group's batches
+ * end.  No counterpart in the reference. */
+int mgw_hip_server_set_grouping(mgw_hip_server* server, int32_t max_jobs);
+
 /* Fault injection for tests of failure handling: each job started from now
  * on gets scale (0 < scale <= 1) times its estimate as its context's device
  * budget, so that a scale below 1 makes the job's allocations fail with

@@ -1,7 +1,11 @@
 // Continuous-batching request loop for G0W0 jobs on one device.
 //
 // Worker threads each own a context (its own HIP stream), so jobs admitted
-// while others run overlap on the GPU.  Admission is by an HBM estimate:
+// while others run overlap on the GPU.  A worker that takes a job also takes
+// every other admitted job that can share its frequency batches (same
+// padded particle-hole order and form, eta == 0, not the decomposed form; up
+// to mgw_hip_server_set_grouping jobs) and runs them through
+// mgw_hip_g0w0_group: one launch sequence factors all their frequencies.  Admission is by an HBM estimate:
 // a job starts when the bytes reserved by running jobs plus its estimate
 // (resident ERI + V + frequency batch + contraction scratch,
 // mgw_hip_estimate_bytes, plus the error judge's FP64 panel for the mixed
@@ -76,6 +80,7 @@ struct mgw_hip_server {
     double judge_tol = 0.0;
     int32_t judge_fallback = 1;
     double budget_scale = 1.0;        // fault-injection hook: per-job device budget scale
+    int32_t group_max = 16;           // jobs one worker runs together (1: no grouping)
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServeJob*> pending;
@@ -95,12 +100,28 @@ struct mgw_hip_server {
         ++completed;
     }
 
+    // Jobs that can share frequency batches (mgw_hip_g0w0_group): eta == 0,
+    // not the decomposed form, same precision and padded particle-hole order.
+    static bool groupable(const ServeJob* j) {
+        return j->job.eta == 0.0 && j->job.precision != MGW_PREC_FP64_CD;
+    }
+    static int padded_order(const ServeJob* j) {
+        const int n = j->job.nocc * (j->job.nmo - j->job.nocc);
+        return (n + 255) / 256 * 256;
+    }
+    static bool same_group(const ServeJob* a, const ServeJob* b) {
+        return groupable(b) && a->job.precision == b->job.precision &&
+               padded_order(a) == padded_order(b);
+    }
+
     void work() {
-        mgw_hip_ctx* ctx = nullptr;
-        const int rc = mgw_hip_create(device, &ctx);
-        if (rc == MGW_OK) mgw_hip_set_graphs(ctx, 1);   // batches of a job replay one graph
+        // pool[0] leads a group (its stream and batch buffers run the shared
+        // batches); pool[k] holds the k-th grouped job's resident ERI
+        std::vector<mgw_hip_ctx*> pool(1, nullptr);
+        const int rc = mgw_hip_create(device, &pool[0]);
+        if (rc == MGW_OK) mgw_hip_set_graphs(pool[0], 1);   // batches of a job replay one graph
         for (;;) {
-            ServeJob* j = nullptr;
+            std::vector<ServeJob*> group;
             int32_t jn = 0, jf = 0;
             double jt = 0.0, scale = 1.0;
             {
@@ -120,7 +141,7 @@ struct mgw_hip_server {
                     cv_done.notify_all();
                     break;
                 }
-                j = pending.front();
+                ServeJob* j = pending.front();
                 pending.pop_front();
                 if (j->cancel || j->estimate > budget || rc != MGW_OK) {
                     if (j->cancel)
@@ -133,46 +154,123 @@ struct mgw_hip_server {
                     cv_work.notify_all();
                     continue;
                 }
+                group.push_back(j);
                 reserved += j->estimate;
-                ++running;
+                // every other admitted job that shares the head job's batches
+                // joins it (continuous batching: later arrivals that fit the
+                // budget start with it)
+                if (group_max > 1 && groupable(j)) {
+                    for (auto it = pending.begin();
+                         it != pending.end() && (int)group.size() < group_max;) {
+                        ServeJob* c = *it;
+                        if (!c->cancel && same_group(j, c) && reserved + c->estimate <= budget) {
+                            reserved += c->estimate;
+                            group.push_back(c);
+                            it = pending.erase(it);
+                        } else {
+                            ++it;
+                        }
+                    }
+                }
+                running += (int)group.size();
                 peak_running = std::max(peak_running, running);
-                j->status = -2;
-                j->t_start = Clock::now();
+                for (ServeJob* g : group) {
+                    g->status = -2;
+                    g->t_start = Clock::now();
+                }
                 jn = judge_n;
                 jt = judge_tol;
                 jf = judge_fallback;
                 scale = budget_scale;
             }
-            int st = mgw_hip_set_memory_budget(ctx, std::max<size_t>(1, (size_t)((double)j->estimate * scale)));
-            if (st == MGW_OK) st = mgw_hip_set_batch_limit(ctx, batch_freqs);
-            if (st == MGW_OK) st = mgw_hip_set_judge(ctx, jn, jt, jf);
-            if (st == MGW_OK) st = mgw_hip_set_cancel_flag(ctx, &j->cancel);
-            if (st == MGW_OK) {
-                st = j->factor ? mgw_hip_upload_eri_factor(ctx, j->factor, j->job.nmo, j->job.nocc, j->kdim)
-                               : mgw_hip_upload_eri(ctx, j->eri, j->job.nmo, j->job.nocc);
+            const int k = (int)group.size();
+            while ((int)pool.size() < k) {
+                mgw_hip_ctx* c = nullptr;
+                if (mgw_hip_create(device, &c) != MGW_OK) break;
+                mgw_hip_set_graphs(c, 1);
+                pool.push_back(c);
             }
-            if (st == MGW_OK) st = mgw_hip_g0w0(ctx, &j->job, &j->out);
-            std::string err = st == MGW_OK ? "" : mgw_hip_last_error(ctx);
-            double jerr = -1.0;
-            int32_t fell = 0;
-            if (st == MGW_OK) mgw_hip_last_judge(ctx, &jerr, nullptr, &fell);
-            mgw_hip_set_cancel_flag(ctx, nullptr);
-            mgw_hip_release_buffers(ctx);
+            std::vector<int> st(k, MGW_OK);
+            std::vector<std::string> err(k);
+            if ((int)pool.size() < k) {
+                // no context for the whole group: the jobs beyond the pool fail
+                for (int i = (int)pool.size(); i < k; ++i) {
+                    st[i] = MGW_ERR_HIP;
+                    err[i] = "worker could not create a device context for a grouped job";
+                }
+            }
+            // the lead's budget and batch limit cover the group's stacked batches
+            size_t sum_est = 0;
+            int32_t sum_nb = 0;
+            for (ServeJob* g : group) {
+                sum_est += g->estimate;
+                sum_nb += std::min(g->job.nfreq, batch_freqs);
+            }
+            std::vector<mgw_hip_ctx*> ctxs;
+            std::vector<mgw_g0w0_job> jobsv;
+            std::vector<mgw_g0w0_out> outsv;
+            std::vector<int> slot;
+            for (int i = 0; i < k; ++i) {
+                if (st[i] != MGW_OK) continue;
+                ServeJob* j = group[i];
+                mgw_hip_ctx* ctx = pool[i];
+                const size_t est = i == 0 ? sum_est : j->estimate;
+                int r = mgw_hip_set_memory_budget(ctx, std::max<size_t>(1, (size_t)((double)est * scale)));
+                if (r == MGW_OK) r = mgw_hip_set_batch_limit(ctx, i == 0 ? sum_nb : batch_freqs);
+                if (r == MGW_OK) r = mgw_hip_set_judge(ctx, jn, jt, jf);
+                if (r == MGW_OK) r = mgw_hip_set_cancel_flag(ctx, &j->cancel);
+                if (r == MGW_OK) {
+                    r = j->factor ? mgw_hip_upload_eri_factor(ctx, j->factor, j->job.nmo, j->job.nocc, j->kdim)
+                                  : mgw_hip_upload_eri(ctx, j->eri, j->job.nmo, j->job.nocc);
+                }
+                if (r != MGW_OK) {
+                    st[i] = r;
+                    err[i] = mgw_hip_last_error(ctx);
+                    continue;
+                }
+                ctxs.push_back(ctx);
+                jobsv.push_back(j->job);
+                outsv.push_back(j->out);
+                slot.push_back(i);
+            }
+            if (!ctxs.empty()) {
+                std::vector<int32_t> gst(ctxs.size(), MGW_OK);
+                if (ctxs.size() == 1)
+                    gst[0] = mgw_hip_g0w0(ctxs[0], &jobsv[0], &outsv[0]);
+                else
+                    mgw_hip_g0w0_group(ctxs.data(), jobsv.data(), outsv.data(), (int32_t)ctxs.size(),
+                                       gst.data());
+                for (size_t q = 0; q < ctxs.size(); ++q) {
+                    st[slot[q]] = gst[q];
+                    if (gst[q] != MGW_OK) err[slot[q]] = mgw_hip_last_error(ctxs[q]);
+                }
+            }
+            std::vector<double> jerr(k, -1.0);
+            std::vector<int32_t> fell(k, 0);
+            for (int i = 0; i < k && i < (int)pool.size(); ++i) {
+                if (st[i] == MGW_OK) mgw_hip_last_judge(pool[i], &jerr[i], nullptr, &fell[i]);
+                mgw_hip_set_cancel_flag(pool[i], nullptr);
+                mgw_hip_release_buffers(pool[i]);
+            }
             {
                 std::lock_guard<std::mutex> lk(mu);
-                j->status = st;
-                j->error = err;
-                j->judge_err = jerr;
-                j->fell_back = fell;
-                j->t_end = Clock::now();
-                reserved -= j->estimate;
-                --running;
-                ++completed;
+                for (int i = 0; i < k; ++i) {
+                    ServeJob* j = group[i];
+                    j->status = st[i];
+                    j->error = err[i];
+                    j->judge_err = jerr[i];
+                    j->fell_back = fell[i];
+                    j->t_end = Clock::now();
+                    reserved -= j->estimate;
+                    --running;
+                    ++completed;
+                }
             }
             cv_done.notify_all();
             cv_work.notify_all();
         }
-        if (ctx) mgw_hip_destroy(ctx);
+        for (mgw_hip_ctx* c : pool)
+            if (c) mgw_hip_destroy(c);
     }
 
     ServeJob* find(int64_t ticket) {
@@ -213,6 +311,13 @@ int mgw_hip_server_set_judge(mgw_hip_server* s, int32_t nsample, double tol, int
     s->judge_n = nsample;
     s->judge_tol = tol;
     s->judge_fallback = fallback != 0;
+    return MGW_OK;
+}
+
+int mgw_hip_server_set_grouping(mgw_hip_server* s, int32_t max_jobs) {
+    if (!s || max_jobs < 1 || max_jobs > 64) return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->group_max = max_jobs;
     return MGW_OK;
 }
 

@@ -113,6 +113,7 @@ def _load():
         "mgw_hip_server_job_error": (C.c_char_p, [vp, C.c_int64]),
         "mgw_hip_server_set_judge": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32]),
         "mgw_hip_server_set_budget_scale": (C.c_int, [vp, C.c_double]),
+        "mgw_hip_server_set_grouping": (C.c_int, [vp, C.c_int32]),
         "mgw_hip_server_cancel": (C.c_int, [vp, C.c_int64]),
         "mgw_hip_server_job_judge": (C.c_int, [vp, C.c_int64, _dp, _ip]),
         "mgw_hip_server_release": (C.c_int, [vp, C.c_int64]),
@@ -557,6 +558,10 @@ class HipServer:
         """Error judge of mixed jobs submitted from now on (0 samples: off)."""
         _check("mgw_hip_server_set_judge",
                lib.mgw_hip_server_set_judge(self.h, int(nsample), float(tol), 1 if fallback else 0))
+
+    def set_grouping(self, max_jobs: int):
+        """Most jobs a worker runs together through mgw_hip_g0w0_group (1: off)."""
+        _check("mgw_hip_server_set_grouping", lib.mgw_hip_server_set_grouping(self.h, int(max_jobs)))
 
     def set_budget_scale(self, scale: float):
         """Fault injection: per-job device budget = scale x estimate."""

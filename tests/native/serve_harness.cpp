@@ -10,6 +10,7 @@
 // jobs queued.  Exit status 0 = every scenario passed.
 #include "mgw_hip.h"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -125,12 +126,32 @@ int mgw_hip_release_buffers(mgw_hip_ctx* ctx) {
 }
 
 // Outputs: qp[p] = 1000 * nmo + p (+ 0.5 when rerun in FP64), sigma = nfreq.
+// The judge and outputs of one finished job.
+static void finish_job(FakeCtx* c, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
+    const bool mixed = job->precision == MGW_PREC_MIXED || job->precision == MGW_PREC_MIXED_INT8 ||
+                       job->precision == MGW_PREC_MIXED_INT8_4;
+    c->judge_err = mixed && c->judge_n > 0 ? 1e-9 : -1.0;
+    // a tolerance below the stand-in's error is a failed judge
+    c->fell_back = mixed && c->judge_n > 0 && c->judge_fallback && c->judge_tol > 0.0 &&
+                           c->judge_tol < 1e-9
+                       ? 1
+                       : 0;
+    const int ns = job->nstates > 0 && job->states ? job->nstates : job->nmo;
+    for (int p = 0; p < job->nmo; ++p) out->qp[p] = 1000.0 * job->nmo + p + (c->fell_back ? 0.5 : 0.0);
+    if (out->sigma_c_re)
+        for (int i = 0; i < ns * job->nfreq; ++i) out->sigma_c_re[i] = job->nfreq;
+}
+
+static void note_running(int count) {
+    const int now = (g_running += count);
+    for (int p = g_peak.load(); now > p && !g_peak.compare_exchange_weak(p, now);) {
+    }
+}
+
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
     auto* c = reinterpret_cast<FakeCtx*>(ctx);
     if (c->nmo != job->nmo) return ctx_fail(c, MGW_ERR_STATE, "no resident ERI for the job");
-    const int now = ++g_running;
-    for (int p = g_peak.load(); now > p && !g_peak.compare_exchange_weak(p, now);) {
-    }
+    note_running(1);
     const int nb = c->batch_limit > 0 ? c->batch_limit : 64;
     int status = MGW_OK;
     for (int f0 = 0; f0 < job->nfreq; f0 += nb) {
@@ -142,18 +163,52 @@ int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
     }
     --g_running;
     if (status != MGW_OK) return status;
-    const bool mixed = job->precision == MGW_PREC_MIXED || job->precision == MGW_PREC_MIXED_INT8;
-    c->judge_err = mixed && c->judge_n > 0 ? 1e-9 : -1.0;
-    // a tolerance below the stand-in's error is a failed judge
-    c->fell_back = mixed && c->judge_n > 0 && c->judge_fallback && c->judge_tol > 0.0 &&
-                           c->judge_tol < 1e-9
-                       ? 1
-                       : 0;
-    const int ns = job->nstates > 0 && job->states ? job->nstates : job->nmo;
-    for (int p = 0; p < job->nmo; ++p) out->qp[p] = 1000.0 * job->nmo + p + (c->fell_back ? 0.5 : 0.0);
-    if (out->sigma_c_re)
-        for (int i = 0; i < ns * job->nfreq; ++i) out->sigma_c_re[i] = job->nfreq;
+    finish_job(c, job, out);
     return MGW_OK;
+}
+
+// Groups: the stacked batches of all jobs (one sleep per batch of the lead's
+// limit), then each job finished on its own context; a job cancelled while its
+// group ran ends with MGW_ERR_STATE.  g_groups counts calls with two or more
+// jobs, g_group_peak the largest.
+std::atomic<int> g_groups{0}, g_group_peak{0};
+
+int mgw_hip_g0w0_group(mgw_hip_ctx* const* ctxs, const mgw_g0w0_job* jobs, mgw_g0w0_out* outs,
+                       int32_t count, int32_t* status) {
+    if (!ctxs || !jobs || !outs || count < 1) return MGW_ERR_ARG;
+    int total = 0, maxnmo = 0;
+    for (int i = 0; i < count; ++i) {
+        for (int k = 0; k < i; ++k)
+            if (ctxs[k] == ctxs[i]) return MGW_ERR_ARG;
+        auto* c = reinterpret_cast<FakeCtx*>(ctxs[i]);
+        if (c->nmo != jobs[i].nmo) {
+            if (status) status[i] = ctx_fail(c, MGW_ERR_STATE, "no resident ERI for the job");
+            return MGW_ERR_STATE;
+        }
+        total += jobs[i].nfreq;
+        maxnmo = std::max(maxnmo, jobs[i].nmo);
+    }
+    ++g_groups;
+    for (int p = g_group_peak.load(); count > p && !g_group_peak.compare_exchange_weak(p, count);) {
+    }
+    note_running(count);
+    const auto* lead = reinterpret_cast<FakeCtx*>(ctxs[0]);
+    const int nb = lead->batch_limit > 0 ? lead->batch_limit : 64;
+    for (int f0 = 0; f0 < total; f0 += nb)
+        std::this_thread::sleep_for(std::chrono::microseconds(200 * maxnmo / 8 + 100));
+    g_running -= count;
+    int first = MGW_OK;
+    for (int i = 0; i < count; ++i) {
+        auto* c = reinterpret_cast<FakeCtx*>(ctxs[i]);
+        int st = MGW_OK;
+        if (c->cancel && __atomic_load_n(c->cancel, __ATOMIC_ACQUIRE) != 0)
+            st = ctx_fail(c, MGW_ERR_STATE, "mgw_hip_g0w0: cancelled during its group's batches");
+        else
+            finish_job(c, &jobs[i], &outs[i]);
+        if (status) status[i] = st;
+        if (first == MGW_OK) first = st;
+    }
+    return first;
 }
 
 int mgw_hip_last_judge(const mgw_hip_ctx* ctx, double* err, int32_t* samples, int32_t* fell) {
@@ -310,6 +365,49 @@ void scenario_cancel_and_fallback() {
     CHECK(mgw_hip_server_destroy(s) == MGW_OK);
 }
 
+void scenario_grouping() {
+    // one worker: a long job runs while jobs of one padded order queue up;
+    // they start together as one group, a cancelled member ends alone
+    mgw_hip_server* s = nullptr;
+    CHECK(mgw_hip_server_create(0, 1, (size_t)1 << 40, 8, &s) == MGW_OK);
+    CHECK(mgw_hip_server_set_grouping(s, 0) == MGW_ERR_ARG);
+    CHECK(mgw_hip_server_set_grouping(s, 4) == MGW_OK);
+    g_groups = 0;
+    g_group_peak = 0;
+    Job first = make_job(64, 200);
+    CHECK(submit(s, first) == MGW_OK);
+    int32_t running = 0;
+    while (running == 0) mgw_hip_server_stats(s, nullptr, &running, nullptr, nullptr);
+    // nmo 16..20 with nocc = nmo / 5: particle-hole orders 39..64, all padded to 256
+    std::vector<Job> jobs;
+    for (int i = 0; i < 6; ++i) jobs.push_back(make_job(16 + (i % 5), 24 + i, MGW_PREC_MIXED_INT8_4));
+    for (auto& j : jobs) CHECK(submit(s, j) == MGW_OK);
+    CHECK(mgw_hip_server_cancel(s, jobs[1].ticket) == MGW_OK);   // pending: ends before it starts
+    CHECK(wait_status(s, first) == MGW_OK);
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const int st = wait_status(s, jobs[i]);
+        if (i == 1) {
+            CHECK(st == MGW_ERR_STATE);
+            continue;
+        }
+        CHECK(st == MGW_OK);
+        check_outputs(jobs[i]);
+    }
+    CHECK(g_groups.load() >= 1 && g_group_peak.load() == 4);     // capped at 4 per group
+    size_t reserved = 1;
+    CHECK(mgw_hip_server_stats(s, nullptr, &running, nullptr, &reserved) == MGW_OK);
+    CHECK(running == 0 && reserved == 0);
+    // grouping off: every job alone
+    CHECK(mgw_hip_server_set_grouping(s, 1) == MGW_OK);
+    g_groups = 0;
+    std::vector<Job> more;
+    for (int i = 0; i < 4; ++i) more.push_back(make_job(16, 24));
+    for (auto& j : more) CHECK(submit(s, j) == MGW_OK);
+    for (auto& j : more) CHECK(wait_status(s, j) == MGW_OK);
+    CHECK(g_groups.load() == 0);
+    CHECK(mgw_hip_server_destroy(s) == MGW_OK);
+}
+
 void scenario_destroy_with_queue() {
     mgw_hip_server* s = nullptr;
     CHECK(mgw_hip_server_create(0, 1, (size_t)1 << 40, 8, &s) == MGW_OK);
@@ -341,6 +439,7 @@ int main() {
     scenario_mix();
     scenario_admission_and_faults();
     scenario_cancel_and_fallback();
+    scenario_grouping();
     scenario_destroy_with_queue();
     scenario_arguments();
     CHECK(g_contexts.load() == 0);                      // every worker context destroyed
