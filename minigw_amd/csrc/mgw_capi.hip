@@ -250,7 +250,7 @@ struct mgw_hip_ctx {
     double judge_err = -1.0;          // largest relative q error of the last judged run (-1: none)
     int judge_used = 0;               // frequencies the last run judged
     int fell_back = 0;                // the last mgw_hip_g0w0 reran in FP64 (1: judge, 2: numeric)
-    Buf hqj;                          // the judge's q row
+    Buf hqj, hjom, hjinfo;            // the judge's q rows, sampled grid and breakdown flags
     // pivoted-LU screening of a dielectric matrix that is not positive definite
     Buf luz, ludl, ludu, lupiv, wz;   // W = E^-1 (batch path / workspace), block inverses, pivots
     bool wc_dense = false;            // the workspace's W_c is held as dense E^-1 (wz)
@@ -263,7 +263,7 @@ struct mgw_hip_ctx {
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
                &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj, &luz, &ludl, &ludu,
-               &lupiv, &wz, &ppk, &panel, &hinfob};
+               &lupiv, &wz, &ppk, &panel, &hinfob, &hjom, &hjinfo};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -1837,37 +1837,47 @@ bool mol_finish(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out) {
                                : job->precision == MGW_PREC_MIXED_INT8 ? 1e-7
                                : i8                                   ? 1e-6
                                                                       : 1e-5;
-            double* A = one(ctx->hA, (size_t)np * np * 8, "dielectric batch");
-            double* S = one(ctx->hS, (size_t)np * 8, "scale batch");
-            double* dinv = one(ctx->hdinv, (size_t)(np / mgw::kNB) * 64 * 64 * 8, "dinv");
-            double* tbuf = one(ctx->htbuf, (size_t)np * 64 * 8, "trtri scratch");
-            double* qpart = one(ctx->hqpart, (size_t)(np / mgw::kCT) * mpad * 8, "qpart");
-            double* qj = as<double>(ctx->get(ctx->hqj, (size_t)m * 8, "judge q"));
-            std::vector<double> qa(m), qb(m);
+            // the sampled frequencies factored together: their omegas in a
+            // grid of their own, one FP64 batch of nj matrices
+            std::vector<int> jf(nj);
+            std::vector<double> jom(nj);
+            for (int i = 0; i < nj; ++i) {
+                jf[i] = (int)(((int64_t)2 * i + 1) * nfreq / (2 * nj));
+                jom[i] = r.omega[jf[i]];
+            }
+            double* djom = as<double>(ctx->get(ctx->hjom, (size_t)nj * 8, "judge grid"));
+            int* jinfo = as<int>(ctx->get(ctx->hjinfo, (size_t)nj * 4, "judge info"));
+            double* A = one(ctx->hA, (size_t)nj * np * np * 8, "dielectric batch");
+            double* S = one(ctx->hS, (size_t)nj * np * 8, "scale batch");
+            double* dinv = one(ctx->hdinv, (size_t)nj * (np / mgw::kNB) * 64 * 64 * 8, "dinv");
+            double* tbuf = one(ctx->htbuf, (size_t)nj * np * 64 * 8, "trtri scratch");
+            double* qpart = one(ctx->hqpart, (size_t)nj * (np / mgw::kCT) * mpad * 8, "qpart");
+            double* qj = as<double>(ctx->get(ctx->hqj, (size_t)nj * m * 8, "judge q"));
+            ctx->h2d(djom, jom.data(), nj);
+            hip_check(hipMemsetAsync(jinfo, 0, (size_t)nj * 4, st), "hipMemsetAsync");
+            hip_check(mgw::launch_set_int(r.f0p + 1, 0, st), "set offset");
+            hip_check(mgw::launch_scale_from_grid(r.ddelta, djom, S, n, np, r.f0p + 1, nj, st), "scale");
+            hip_check(mgw::launch_dielectric(r.dV, S, A, np, nj, st), "dielectric");
+            factor_batch(ctx, A, dinv, jinfo, tbuf, np, nj);
+            hip_check(mgw::launch_contract_f64(A, S, r.xj, qpart, np, mpad, nj, st), "contract");
+            hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, qj, np / mgw::kCT, m, mpad, r.f0p + 1, nj,
+                                           st),
+                      "reduce q");
+            std::vector<double> qa((size_t)nj * m), qb((size_t)nj * m);
+            ctx->d2h(qa.data(), qj, (size_t)nj * m);
+            for (int i = 0; i < nj; ++i) ctx->d2h(qb.data() + (size_t)i * m, r.dq + (size_t)jf[i] * m, m);
+            ctx->sync();
+            check_info(ctx, jinfo, nj, "potrf (judge)");
             double worst = 0.0;
             for (int i = 0; i < nj; ++i) {
-                const int f = (int)(((int64_t)2 * i + 1) * nfreq / (2 * nj));
-                hip_check(mgw::launch_set_int(r.f0p, f, st), "set offset");
-                hip_check(mgw::launch_set_int(r.f0p + 1, 0, st), "set offset");
-                hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, S, n, np, r.f0p, 1, st), "scale");
-                hip_check(mgw::launch_dielectric(r.dV, S, A, np, 1, st), "dielectric");
-                factor_batch(ctx, A, dinv, r.info + f, tbuf, np, 1);
-                hip_check(mgw::launch_contract_f64(A, S, r.xj, qpart, np, mpad, 1, st), "contract");
-                hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, qj, np / mgw::kCT, m, mpad, r.f0p + 1,
-                                               1, st),
-                          "reduce q");
-                ctx->d2h(qa.data(), qj, m);
-                ctx->d2h(qb.data(), r.dq + (size_t)f * m, m);
-                ctx->sync();
                 double e = 0.0, sc = 0.0;
                 for (int c = 0; c < m; ++c) {
-                    e = std::max(e, std::fabs(qa[c] - qb[c]));
-                    sc = std::max(sc, std::fabs(qa[c]));
+                    e = std::max(e, std::fabs(qa[(size_t)i * m + c] - qb[(size_t)i * m + c]));
+                    sc = std::max(sc, std::fabs(qa[(size_t)i * m + c]));
                 }
                 const double rel = sc > 0.0 ? e / sc : (e > 0.0 ? INFINITY : 0.0);
                 worst = std::max(worst, std::isnan(rel) ? INFINITY : rel);
             }
-            check_info(ctx, r.info, nfreq, "potrf (judge)");
             ctx->judge_err = worst;
             ctx->judge_used = nj;
             if (!(worst <= tol) && ctx->judge_fallback) return false;
