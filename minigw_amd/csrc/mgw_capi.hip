@@ -1713,7 +1713,14 @@ void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols) {
             }
             for (const Seg& g : segs)
                 hip_check(mgw::launch_set_int(g.r->f0p, g.f0, st), "set offset");
-            if (lead->use_graphs && !cplx) {
+            // A graph captured across the factorization's side streams (fork
+            // and join events) keeps the device memory of the buffers it
+            // referenced after they are freed and the graph is destroyed
+            // (measured: 16 GB per nph = 5402 molecule); batches that split
+            // run eagerly (at those orders a batch takes hundreds of ms, so
+            // replay saves nothing measurable).
+            const bool splits = fp >= lead->split_order && std::min(B, lead->factor_streams) >= 2;
+            if (lead->use_graphs && !cplx && !splits) {
                 // Replay a graph of this batch when one with the same shape and
                 // buffers was captured before (the offsets are read on the device).
                 GraphKey key;

@@ -172,9 +172,12 @@ def test_autotune_picks_a_batch_and_keeps_results(hip):
 
 def test_split_factorization_matches_one_stream(hip, monkeypatch):
     """The factorization split over two (and four) streams, forced on a small
-    molecule, gives bit for bit the one-stream results, eagerly and replayed
-    from captured graphs (the fork and join are captured too)."""
-    say("case: factorization on 1 / 2 / 4 streams, nmo=30, eager and graph replays")
+    molecule, gives bit for bit the one-stream results; with graphs on, the
+    batches that split run eagerly (a graph captured across the side streams
+    keeps the memory of the buffers it referenced after they are freed), and
+    device memory in use returns to its level after the context's buffers are
+    released."""
+    say("case: factorization on 1 / 2 / 4 streams, nmo=30, eager; graphs off for split batches")
     inp = synthetic_input(30, 6, 44)
     base = {}
     monkeypatch.setenv("MGW_FACTOR_STREAMS", "1")
@@ -194,7 +197,7 @@ def test_split_factorization_matches_one_stream(hip, monkeypatch):
                 ctx.set_graphs(True)
                 for rep in range(3):
                     res = hip.run_g0w0(inp, 150, 16, ctx=ctx, upload=False, precision=prec)
-                    assert res.timings["graph_batches"] > 0
+                    assert res.timings["graph_batches"] == 0
                     assert np.array_equal(res.sigma_c, base[prec].sigma_c), (streams, prec, rep)
                     assert np.array_equal(res.qp, base[prec].qp, equal_nan=True)
 
@@ -310,3 +313,40 @@ def test_serve_faults_cancel_and_fallback(hip):
         inp = GwInput(mo_energy=j["mo"], eri=None, vxc=j["vxc"], nocc=j["nocc"], fermi_energy=j["fermi"])
         f64 = hip.run_g0w0(inp, 40, 12, ctx=ctx, upload=False)
     assert np.array_equal(fb.sigma_c, f64.sigma_c)
+
+
+def test_split_batches_leave_no_device_memory_held(hip):
+    """A molecule whose order splits the factorization over two streams
+    (nph = 4096), run with graphs on: once the context's buffers are released,
+    the device memory in use returns to its level before the job (a graph
+    captured across the side streams used to keep the memory of the buffers
+    it referenced).  A small job first reserves the runtime's per-queue
+    scratch (k_pade_qp uses 3 KB per lane), which stays."""
+    import ctypes
+    rt = ctypes.CDLL("libamdhip64.so")
+
+    def used_gib():
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        assert rt.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+        return (total.value - free.value) / 2**30
+
+    say("case: nmo=160 nocc=32 (nph 4096, split factorization) with graphs, memory after release")
+    small = synthetic_input(18, 4, 51)
+    mo, fermi, b, vxc = synthetic_factor(160, 32, 52)
+    big = GwInput(mo_energy=mo, eri=None, vxc=vxc, nocc=32, fermi_energy=fermi)
+    with hip.HipContext(0) as ctx:
+        ctx.set_graphs(True)
+        ctx.set_batch_limit(8)
+        hip.run_g0w0(small, 40, 12, ctx=ctx, precision=hip.PREC_MIXED_INT8_4)
+        ctx.release_buffers()
+        before = used_gib()
+        ctx.upload_eri_factor(b, 160, 32)
+        for rep in range(2):
+            res = hip.run_g0w0(big, 40, 12, ctx=ctx, upload=False, precision=hip.PREC_MIXED_INT8_4)
+            assert res.timings["graph_batches"] == 0 and np.all(np.isfinite(res.sigma_c))
+        peak = used_gib()
+        ctx.release_buffers()
+        after = used_gib()
+    say(f"   device memory in use: {before:.2f} GiB before, {peak:.2f} with the job, {after:.2f} after release")
+    assert peak > before + 5.0
+    assert after < before + 0.25
