@@ -1466,7 +1466,7 @@ struct Seg {
 // the form and eta = 0 (a molecule with eta != 0 or the decomposed form runs
 // alone).  Everything is enqueued on lead's stream with lead's batch buffers;
 // the molecules' own data must be complete (their streams synchronized).
-void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols) {
+void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols, bool allow_graphs) {
         MolRun& r0 = *mols[0];
         const mgw_g0w0_job* job = r0.job;
         const int np = r0.np, fp = r0.fp, precision = job->precision;
@@ -1720,7 +1720,7 @@ void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols) {
             // run eagerly (at those orders a batch takes hundreds of ms, so
             // replay saves nothing measurable).
             const bool splits = fp >= lead->split_order && std::min(B, lead->factor_streams) >= 2;
-            if (lead->use_graphs && !cplx && !splits) {
+            if (allow_graphs && lead->use_graphs && !cplx && !splits) {
                 // Replay a graph of this batch when one with the same shape and
                 // buffers was captured before (the offsets are read on the device).
                 GraphKey key;
@@ -1793,151 +1793,187 @@ void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols) {
         }
 }
 
-// Phase 3 (on the molecule's own context and stream): frequencies whose
+// Phase 3a (on the molecule's own context and stream): frequencies whose
 // dielectric matrix was not positive definite re-screened by pivoted LU
-// (plain FP64 form; the other forms report the breakdown), the error judge of
-// the mixed forms, Sigma_c, Sigma_x, Pade/QP, downloads and timings.  Returns
-// false, having written nothing to `out`, when the judge failed and the
-// context asks for the FP64 fallback.
-bool mol_finish(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out) {
-        const mgw_g0w0_job* job = r.job;
-        const int nmo = r.nmo, nfreq = r.nfreq, npar = r.npar, ns = r.ns, n = r.n, np = r.np;
-        const int m = r.m, mpad = r.mpad;
-        const bool i8 = r.i8, mixed = r.mixed;
+// (plain FP64 form; the other forms report the breakdown).
+void mol_lu(MolRun& r, mgw_hip_ctx* ctx) {
+        const int n = r.n, np = r.np, m = r.m, mpad = r.mpad;
         hipStream_t st = ctx->stream;
-        // one frequency's buffers of the batch phase (the context's batch
-        // buffers when it led the batches, else allocated here)
+        ctx->lu_freqs = 0;
+        ctx->wc_ready = false;
+        std::sort(r.failed.begin(), r.failed.end());
+        if (r.failed.empty()) return;
+        if (r.cplx || r.cd || r.mixed || r.i8)
+            fail(MGW_ERR_NUMERIC, "potrf: symmetrised dielectric matrix %d is not positive definite",
+                 r.failed[0]);
+        // one frequency's buffers (the context's batch buffers when it led
+        // the batches, else allocated here)
         auto one = [&](mgw_hip_ctx::Buf& b, size_t bytes, const char* name) {
             return as<double>(ctx->get(b, std::max(bytes, b.bytes), name));
         };
-        ctx->lu_freqs = 0;
-        std::sort(r.failed.begin(), r.failed.end());
-        if (!r.failed.empty()) {
-            if (r.cplx || r.cd || mixed || i8)
-                fail(MGW_ERR_NUMERIC, "potrf: symmetrised dielectric matrix %d is not positive definite",
-                     r.failed[0]);
-            double* A = one(ctx->hA, (size_t)np * np * 8, "dielectric batch");
-            double* S = one(ctx->hS, (size_t)np * 8, "scale batch");
-            double* qpart = one(ctx->hqpart, (size_t)(np / mgw::kCT) * mpad * 8, "qpart");
-            double* W = as<double>(ctx->get(ctx->luz, (size_t)np * np * 8, "E^-1"));
-            for (int f : r.failed) {
-                hip_check(mgw::launch_set_int(r.f0p, f, st), "set offset");
-                hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, S, n, np, r.f0p, 1, st), "scale");
-                lu_screen(ctx, r.dV, S, A, W, np, 1);
-                hip_check(mgw::launch_contract_dense(W, S, r.xp, qpart, np, mpad, 1, st),
-                          "contract (dense)");
-                hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, r.dq, np / mgw::kCT, m, mpad, r.f0p, 1,
-                                               st),
-                          "reduce q");
-            }
-            ctx->lu_freqs = (int)r.failed.size();
-        }
-        ctx->wc_ready = false;
-
-        // Error judge of the mixed forms: a sample of the frequencies recomputed
-        // with every step in FP64 (FP64 dielectric, factorization, contraction),
-        // its q compared with the form's.  The relative error is the largest
-        // |q_form - q_fp64| over the largest |q_fp64| of each sampled frequency.
-        if ((mixed || i8) && ctx->judge_n > 0) {
-            const int nj = std::min(ctx->judge_n, nfreq);
-            const double tol = ctx->judge_tol > 0.0               ? ctx->judge_tol
-                               : job->precision == MGW_PREC_MIXED_INT8 ? 1e-7
-                               : i8                                   ? 1e-6
-                                                                      : 1e-5;
-            // the sampled frequencies factored together: their omegas in a
-            // grid of their own, one FP64 batch of nj matrices
-            std::vector<int> jf(nj);
-            std::vector<double> jom(nj);
-            for (int i = 0; i < nj; ++i) {
-                jf[i] = (int)(((int64_t)2 * i + 1) * nfreq / (2 * nj));
-                jom[i] = r.omega[jf[i]];
-            }
-            double* djom = as<double>(ctx->get(ctx->hjom, (size_t)nj * 8, "judge grid"));
-            int* jinfo = as<int>(ctx->get(ctx->hjinfo, (size_t)nj * 4, "judge info"));
-            double* A = one(ctx->hA, (size_t)nj * np * np * 8, "dielectric batch");
-            double* S = one(ctx->hS, (size_t)nj * np * 8, "scale batch");
-            double* dinv = one(ctx->hdinv, (size_t)nj * (np / mgw::kNB) * 64 * 64 * 8, "dinv");
-            double* tbuf = one(ctx->htbuf, (size_t)nj * np * 64 * 8, "trtri scratch");
-            double* qpart = one(ctx->hqpart, (size_t)nj * (np / mgw::kCT) * mpad * 8, "qpart");
-            double* qj = as<double>(ctx->get(ctx->hqj, (size_t)nj * m * 8, "judge q"));
-            ctx->h2d(djom, jom.data(), nj);
-            hip_check(hipMemsetAsync(jinfo, 0, (size_t)nj * 4, st), "hipMemsetAsync");
-            hip_check(mgw::launch_set_int(r.f0p + 1, 0, st), "set offset");
-            hip_check(mgw::launch_scale_from_grid(r.ddelta, djom, S, n, np, r.f0p + 1, nj, st), "scale");
-            hip_check(mgw::launch_dielectric(r.dV, S, A, np, nj, st), "dielectric");
-            factor_batch(ctx, A, dinv, jinfo, tbuf, np, nj);
-            hip_check(mgw::launch_contract_f64(A, S, r.xj, qpart, np, mpad, nj, st), "contract");
-            hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, qj, np / mgw::kCT, m, mpad, r.f0p + 1, nj,
-                                           st),
+        double* A = one(ctx->hA, (size_t)np * np * 8, "dielectric batch");
+        double* S = one(ctx->hS, (size_t)np * 8, "scale batch");
+        double* qpart = one(ctx->hqpart, (size_t)(np / mgw::kCT) * mpad * 8, "qpart");
+        double* W = as<double>(ctx->get(ctx->luz, (size_t)np * np * 8, "E^-1"));
+        for (int f : r.failed) {
+            hip_check(mgw::launch_set_int(r.f0p, f, st), "set offset");
+            hip_check(mgw::launch_scale_from_grid(r.ddelta, r.dom, S, n, np, r.f0p, 1, st), "scale");
+            lu_screen(ctx, r.dV, S, A, W, np, 1);
+            hip_check(mgw::launch_contract_dense(W, S, r.xp, qpart, np, mpad, 1, st),
+                      "contract (dense)");
+            hip_check(mgw::launch_reduce_q(qpart, r.dcolmap, r.dq, np / mgw::kCT, m, mpad, r.f0p, 1, st),
                       "reduce q");
-            std::vector<double> qa((size_t)nj * m), qb((size_t)nj * m);
-            ctx->d2h(qa.data(), qj, (size_t)nj * m);
-            for (int i = 0; i < nj; ++i) ctx->d2h(qb.data() + (size_t)i * m, r.dq + (size_t)jf[i] * m, m);
-            ctx->sync();
-            check_info(ctx, jinfo, nj, "potrf (judge)");
-            double worst = 0.0;
-            for (int i = 0; i < nj; ++i) {
-                double e = 0.0, sc = 0.0;
-                for (int c = 0; c < m; ++c) {
-                    e = std::max(e, std::fabs(qa[(size_t)i * m + c] - qb[(size_t)i * m + c]));
-                    sc = std::max(sc, std::fabs(qa[(size_t)i * m + c]));
-                }
-                const double rel = sc > 0.0 ? e / sc : (e > 0.0 ? INFINITY : 0.0);
-                worst = std::max(worst, std::isnan(rel) ? INFINITY : rel);
-            }
-            ctx->judge_err = worst;
-            ctx->judge_used = nj;
-            if (!(worst <= tol) && ctx->judge_fallback) return false;
         }
-        if (out->q) ctx->d2h(out->q, r.dq, (size_t)nfreq * m);
+        ctx->lu_freqs = (int)r.failed.size();
+}
 
-        hipEvent_t ev[3];
-        for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-        struct EvGuard {
-            hipEvent_t* e;
-            ~EvGuard() {
-                for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+// Phase 3b: the error judge of the mixed forms.  A sample of the frequencies
+// is recomputed with every step in FP64 (FP64 dielectric, factorization,
+// contraction) and its q compared with the form's: the relative error is the
+// largest |q_form - q_fp64| over the largest |q_fp64| of each sampled
+// frequency.  The sampled frequencies of a molecule are a frequency batch of
+// their own, run by run_batches through a view of the molecule: its grid the
+// sampled omegas, its output the judge's q rows, its panel the FP64 X.
+struct JudgeRun {
+    MolRun* r = nullptr;
+    MolRun view;
+    mgw_g0w0_job job{};
+    std::vector<int> jf;
+};
+
+bool judged(const MolRun& r) {
+    return (r.mixed || r.i8) && r.ctx->judge_n > 0;
+}
+
+void judge_view(MolRun& r, JudgeRun& j) {
+        mgw_hip_ctx* ctx = r.ctx;
+        const int nj = std::min(ctx->judge_n, r.nfreq);
+        j.r = &r;
+        j.jf.resize(nj);
+        std::vector<double> jom(nj);
+        for (int i = 0; i < nj; ++i) {
+            j.jf[i] = (int)(((int64_t)2 * i + 1) * r.nfreq / (2 * nj));
+            jom[i] = r.omega[j.jf[i]];
+        }
+        j.job = *r.job;
+        j.job.precision = MGW_PREC_FP64;
+        j.job.nfreq = nj;
+        MolRun& v = j.view;
+        v.ctx = ctx;
+        v.job = &j.job;
+        v.nmo = r.nmo;
+        v.nocc = r.nocc;
+        v.nfreq = nj;
+        v.npar = r.npar;
+        v.ns = r.ns;
+        v.n = r.n;
+        v.np = r.np;
+        v.fp = r.np;
+        v.m = r.m;
+        v.mu = r.mu;
+        v.mpad = r.mpad;
+        v.omega = jom;
+        v.dom = as<double>(ctx->get(ctx->hjom, (size_t)nj * 8, "judge grid"));
+        v.dq = as<double>(ctx->get(ctx->hqj, (size_t)nj * r.m * 8, "judge q"));
+        v.ddelta = r.ddelta;
+        v.dV = r.dV;
+        v.dcolmap = r.dcolmap;
+        v.xp = r.xj;
+        v.f0p = r.f0p + 1;
+        ctx->h2d(v.dom, v.omega.data(), nj);
+        ctx->sync();
+}
+
+// The judge's verdict on its batch: true when the form's result stands.
+bool judge_verdict(JudgeRun& j) {
+        MolRun& r = *j.r;
+        mgw_hip_ctx* ctx = r.ctx;
+        const int nj = (int)j.jf.size(), m = r.m;
+        if (!j.view.failed.empty())
+            fail(MGW_ERR_NUMERIC,
+                 "potrf (judge): symmetrised dielectric matrix %d is not positive definite",
+                 j.jf[j.view.failed[0]]);
+        const double tol = ctx->judge_tol > 0.0                 ? ctx->judge_tol
+                           : r.job->precision == MGW_PREC_MIXED_INT8 ? 1e-7
+                           : r.i8                                   ? 1e-6
+                                                                    : 1e-5;
+        std::vector<double> qa((size_t)nj * m), qb((size_t)nj * m);
+        ctx->d2h(qa.data(), j.view.dq, (size_t)nj * m);
+        for (int i = 0; i < nj; ++i) ctx->d2h(qb.data() + (size_t)i * m, r.dq + (size_t)j.jf[i] * m, m);
+        ctx->sync();
+        double worst = 0.0;
+        for (int i = 0; i < nj; ++i) {
+            double e = 0.0, sc = 0.0;
+            for (int c = 0; c < m; ++c) {
+                e = std::max(e, std::fabs(qa[(size_t)i * m + c] - qb[(size_t)i * m + c]));
+                sc = std::max(sc, std::fabs(qa[(size_t)i * m + c]));
             }
-        } evg{ev};
-        hip_check(hipEventRecord(ev[0], st), "hipEventRecord");
+            const double rel = sc > 0.0 ? e / sc : (e > 0.0 ? INFINITY : 0.0);
+            worst = std::max(worst, std::isnan(rel) ? INFINITY : rel);
+        }
+        ctx->judge_err = worst;
+        ctx->judge_used = nj;
+        return worst <= tol || !ctx->judge_fallback;
+}
+
+// Phase 3c: Sigma_c, Sigma_x, Pade/QP and the downloads, enqueued on the
+// molecule's stream (mol_tail_end waits for them and records the timings).
+struct Tail {
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<double> nan;
+    std::vector<int> zero;
+    ~Tail() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+void mol_tail_begin(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out, Tail& t) {
+        const mgw_g0w0_job* job = r.job;
+        const int nmo = r.nmo, nfreq = r.nfreq, ns = r.ns;
+        hipStream_t st = ctx->stream;
+        if (out->q) ctx->d2h(out->q, r.dq, (size_t)nfreq * r.m);
+        for (auto& e : t.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventRecord(t.ev[0], st), "hipEventRecord");
         {
             Range rg("sigma_c wall");
             hip_check(mgw::launch_sigma_c(r.dq, r.dqi, r.dmo, r.dom, r.dwt, job->fermi_energy, nmo, ns,
                                           nfreq, r.gq, r.sre, r.sim, st),
                       "sigma_c");
         }
-        hip_check(hipEventRecord(ev[1], st), "hipEventRecord");
+        hip_check(hipEventRecord(t.ev[1], st), "hipEventRecord");
         {
             Range rg("exchange");
             hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), r.sx, nmo, r.nocc, st), "sigma_x");
         }
-        {
-            Range rg("pade/output pade");
-            std::vector<double> nan(nmo, std::numeric_limits<double>::quiet_NaN());
-            std::vector<int> zero(nmo, 0);
-            ctx->h2d(r.qp, nan.data(), nmo);
-            ctx->h2d(r.iters, zero.data(), nmo);
-            hip_check(mgw::launch_pade_qp(r.sre, r.sim, r.dom, r.dmo, r.sx, r.dvxc, r.dstates,
-                                          job->fermi_energy, ns, nfreq, npar, r.qp, r.iters, st),
-                      "pade/qp");
-            hip_check(hipEventRecord(ev[2], st), "hipEventRecord");
-            if (out->sigma_c_re) ctx->d2h(out->sigma_c_re, r.sre, (size_t)ns * nfreq);
-            if (out->sigma_c_im) ctx->d2h(out->sigma_c_im, r.sim, (size_t)ns * nfreq);
-            if (out->sigma_x_diag) ctx->d2h(out->sigma_x_diag, r.sx, nmo);
-            ctx->d2h(out->qp, r.qp, nmo);
-            if (out->qp_iterations) ctx->d2h(out->qp_iterations, r.iters, nmo);
-            ctx->sync();
-        }
+        Range rg("pade/output pade");
+        t.nan.assign(nmo, std::numeric_limits<double>::quiet_NaN());
+        t.zero.assign(nmo, 0);
+        ctx->h2d(r.qp, t.nan.data(), nmo);
+        ctx->h2d(r.iters, t.zero.data(), nmo);
+        hip_check(mgw::launch_pade_qp(r.sre, r.sim, r.dom, r.dmo, r.sx, r.dvxc, r.dstates,
+                                      job->fermi_energy, ns, nfreq, r.npar, r.qp, r.iters, st),
+                  "pade/qp");
+        hip_check(hipEventRecord(t.ev[2], st), "hipEventRecord");
+        if (out->sigma_c_re) ctx->d2h(out->sigma_c_re, r.sre, (size_t)ns * nfreq);
+        if (out->sigma_c_im) ctx->d2h(out->sigma_c_im, r.sim, (size_t)ns * nfreq);
+        if (out->sigma_x_diag) ctx->d2h(out->sigma_x_diag, r.sx, nmo);
+        ctx->d2h(out->qp, r.qp, nmo);
+        if (out->qp_iterations) ctx->d2h(out->qp_iterations, r.iters, nmo);
+}
+
+void mol_tail_end(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out, Tail& t) {
+        ctx->sync();
         if (out->omega) std::copy(r.omega.begin(), r.omega.end(), out->omega);
         if (out->weight) std::copy(r.weight.begin(), r.weight.end(), out->weight);
         float tv = 0, tsig = 0, tqp = 0, ttot = 0;
         hip_check(hipEventElapsedTime(&tv, r.ev[0], r.ev[1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&tsig, ev[0], ev[1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&tqp, ev[1], ev[2]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&tsig, t.ev[0], t.ev[1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&tqp, t.ev[1], t.ev[2]), "hipEventElapsedTime");
         // total: from the start of the molecule's setup to its QP energies
-        hip_check(hipEventElapsedTime(&ttot, r.ev[0], ev[2]), "hipEventElapsedTime");
-        const double t[7] = {tv, r.t_diel, r.t_fact, r.t_contr, tsig, tqp, ttot};
-        std::copy(t, t + 7, ctx->timings);
+        hip_check(hipEventElapsedTime(&ttot, r.ev[0], t.ev[2]), "hipEventElapsedTime");
+        const double tm[7] = {tv, r.t_diel, r.t_fact, r.t_contr, tsig, tqp, ttot};
+        std::copy(tm, tm + 7, ctx->timings);
         ctx->timings[7] = r.nb;
         ctx->timings[8] = r.n_contr;
         ctx->timings[9] = r.t_aux;
@@ -1946,35 +1982,37 @@ bool mol_finish(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out) {
         ctx->timings[12] = r.cd_rank;
         ctx->timings[13] = r.cd_ms;
         ctx->timings[14] = ctx->lu_freqs;
-        return true;
 }
 
-// One molecule on its own context.
+// One molecule on its own context.  Returns false, having written nothing to
+// `out`, when the error judge failed and the context asks for the FP64
+// fallback.
 bool g0w0_body(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
     MolRun r;
     mol_prepare(r, ctx, job, out);
-    run_batches(ctx, {&r});
-    return mol_finish(r, ctx, out);
+    run_batches(ctx, {&r}, true);
+    mol_lu(r, ctx);
+    if (judged(r)) {
+        JudgeRun j;
+        judge_view(r, j);
+        run_batches(ctx, {&j.view}, false);
+        if (!judge_verdict(j)) return false;
+    }
+    Tail t;
+    mol_tail_begin(r, ctx, out, t);
+    mol_tail_end(r, ctx, out, t);
+    return true;
 }
 
-// The FP64 fallback of a mixed form: a failed judge (returns false) or a
-// factorization breakdown under an active judge reruns the job in FP64.
-void with_fallback(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out,
-                   const std::function<bool()>& run) {
-    ctx->fell_back = 0;
-    ctx->judge_err = -1.0;
-    ctx->judge_used = 0;
-    const bool mixed = mixed_form(job->precision);
-    bool done = false;
-    try {
-        done = run();
-    } catch (const MgwError& e) {
-        // a mixed form's factorization breakdown falls back like a failed judge
-        if (!(mixed && ctx->judge_n > 0 && ctx->judge_fallback && e.code == MGW_ERR_NUMERIC)) throw;
-        ctx->fell_back = 2;
-    }
-    if (done) return;
-    if (!ctx->fell_back) ctx->fell_back = 1;
+// A mixed form's fallback to FP64: 1 after a failed judge, 2 after a
+// factorization breakdown under an active judge (whose error is kept).
+bool falls_back(const mgw_g0w0_job* job, const mgw_hip_ctx* ctx, const MgwError& e) {
+    return mixed_form(job->precision) && ctx->judge_n > 0 && ctx->judge_fallback &&
+           e.code == MGW_ERR_NUMERIC;
+}
+
+void rerun_fp64(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out, int why) {
+    ctx->fell_back = why;
     mgw_g0w0_job plain = *job;
     plain.precision = MGW_PREC_FP64;
     const double err = ctx->judge_err;
@@ -1983,12 +2021,26 @@ void with_fallback(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out,
     ctx->judge_err = err;
     ctx->judge_used = used;
 }
+
+void reset_judge(mgw_hip_ctx* ctx) {
+    ctx->fell_back = 0;
+    ctx->judge_err = -1.0;
+    ctx->judge_used = 0;
+}
 }  // namespace
 
 int mgw_hip_g0w0(mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
     return guarded(ctx, [&] {
         need(ctx && job && out, "mgw_hip_g0w0: NULL argument");
-        with_fallback(ctx, job, out, [&] { return g0w0_body(ctx, job, out); });
+        reset_judge(ctx);
+        int why = 0;
+        try {
+            if (!g0w0_body(ctx, job, out)) why = 1;
+        } catch (const MgwError& e) {
+            if (!falls_back(job, ctx, e)) throw;
+            why = 2;
+        }
+        if (why) rerun_fp64(ctx, job, out, why);
     });
 }
 
@@ -2014,6 +2066,7 @@ int mgw_hip_g0w0_group(mgw_hip_ctx* const* ctxs, const mgw_g0w0_job* jobs, mgw_g
         };
         std::vector<std::unique_ptr<MolRun>> runs(count);
         for (int i = 0; i < count; ++i) {
+            reset_judge(ctxs[i]);
             runs[i] = std::make_unique<MolRun>();
             try {
                 mol_prepare(*runs[i], ctxs[i], &jobs[i], &outs[i]);
@@ -2023,35 +2076,82 @@ int mgw_hip_g0w0_group(mgw_hip_ctx* const* ctxs, const mgw_g0w0_job* jobs, mgw_g
         }
         for (int i = 0; i < count; ++i)
             if (st[i] == MGW_OK) ctxs[i]->sync();
-        // molecules of one padded order and form share the frequency batches;
-        // eta != 0 and the decomposed form run alone, each on its own context
-        std::vector<std::vector<MolRun*>> groups;
-        std::map<std::pair<int, int>, size_t> slot;
-        for (int i = 0; i < count; ++i) {
-            if (st[i] != MGW_OK) continue;
-            MolRun* r = runs[i].get();
-            if (r->cplx || r->cd) {
-                groups.push_back({r});
-                continue;
+        // molecules of one padded order and form share batches; eta != 0 and
+        // the decomposed form run alone, each on its own context
+        auto batch_groups = [&](const std::vector<MolRun*>& mols, bool graphs) {
+            std::vector<std::vector<MolRun*>> groups;
+            std::map<std::pair<int, int>, size_t> slot;
+            for (MolRun* r : mols) {
+                if (r->cplx || r->cd) {
+                    groups.push_back({r});
+                    continue;
+                }
+                const auto key = std::make_pair(r->np, r->job->precision);
+                auto it = slot.find(key);
+                if (it == slot.end()) {
+                    slot[key] = groups.size();
+                    groups.push_back({r});
+                } else {
+                    groups[it->second].push_back(r);
+                }
             }
-            const auto key = std::make_pair(r->np, r->job->precision);
-            auto it = slot.find(key);
-            if (it == slot.end()) {
-                slot[key] = groups.size();
-                groups.push_back({r});
-            } else {
-                groups[it->second].push_back(r);
-            }
+            for (auto& g : groups) run_batches(g.size() == 1 ? g[0]->ctx : lead, g, graphs);
+        };
+        {
+            std::vector<MolRun*> mols;
+            for (int i = 0; i < count; ++i)
+                if (st[i] == MGW_OK) mols.push_back(runs[i].get());
+            batch_groups(mols, true);
         }
-        for (auto& g : groups) run_batches(g.size() == 1 ? g[0]->ctx : lead, g);
+        // per job: 0 the form's result stands, 1 / 2 rerun in FP64 (judge, breakdown)
+        std::vector<int> why(count, 0);
+        auto job_failure = [&](int i, const MgwError& e) {
+            if (falls_back(&jobs[i], ctxs[i], e))
+                why[i] = 2;
+            else
+                job_error(i, e);
+        };
         for (int i = 0; i < count; ++i) {
             if (st[i] != MGW_OK) continue;
-            mgw_hip_ctx* c = ctxs[i];
             try {
                 // a job cancelled while its group ran keeps no result
-                if (c->cancel && __atomic_load_n(c->cancel, __ATOMIC_ACQUIRE) != 0)
+                if (ctxs[i]->cancel && __atomic_load_n(ctxs[i]->cancel, __ATOMIC_ACQUIRE) != 0)
                     fail(MGW_ERR_STATE, "mgw_hip_g0w0: cancelled during its group's frequency batches");
-                with_fallback(c, &jobs[i], &outs[i], [&] { return mol_finish(*runs[i], c, &outs[i]); });
+                mol_lu(*runs[i], ctxs[i]);
+            } catch (const MgwError& e) {
+                job_failure(i, e);
+            }
+        }
+        // the judges' sampled frequencies, stacked like the molecules' own
+        std::vector<std::unique_ptr<JudgeRun>> judges(count);
+        {
+            std::vector<MolRun*> views;
+            for (int i = 0; i < count; ++i)
+                if (st[i] == MGW_OK && !why[i] && judged(*runs[i])) {
+                    judges[i] = std::make_unique<JudgeRun>();
+                    judge_view(*runs[i], *judges[i]);
+                    views.push_back(&judges[i]->view);
+                }
+            batch_groups(views, false);
+        }
+        for (int i = 0; i < count; ++i) {
+            if (!judges[i]) continue;
+            try {
+                if (!judge_verdict(*judges[i])) why[i] = 1;
+            } catch (const MgwError& e) {
+                job_failure(i, e);
+            }
+        }
+        // the tails of every molecule enqueued before any is waited for
+        std::vector<Tail> tails(count);
+        for (int i = 0; i < count; ++i)
+            if (st[i] == MGW_OK && !why[i]) mol_tail_begin(*runs[i], ctxs[i], &outs[i], tails[i]);
+        for (int i = 0; i < count; ++i)
+            if (st[i] == MGW_OK && !why[i]) mol_tail_end(*runs[i], ctxs[i], &outs[i], tails[i]);
+        for (int i = 0; i < count; ++i) {
+            if (st[i] != MGW_OK || !why[i]) continue;
+            try {
+                rerun_fp64(ctxs[i], &jobs[i], &outs[i], why[i]);
             } catch (const MgwError& e) {
                 job_error(i, e);
             }
