@@ -133,3 +133,34 @@ def test_group_mixes_alone_forms_and_reports_per_job(hip):
     finally:
         for c in ctxs:
             c.close()
+
+
+def test_group_judges_each_member(hip):
+    """Grouped mixed jobs under the error judge: the judges' sampled
+    frequencies of all members run as one stacked FP64 batch; a member whose
+    judge tolerance no form can meet reruns in FP64 (its result bit for bit
+    the FP64 form's) while the others keep their int8 results, each equal to
+    its run alone under the same judge."""
+    prec = hip.PREC_MIXED_INT8_4
+    inps = _molecules()[:4]
+    say("case: group of 4 (nph 256), int8x4, judge on every member, member 2 forced to fall back")
+    ctxs = _contexts(hip, inps)
+    try:
+        for i, c in enumerate(ctxs):
+            c.set_judge(2, tol=1e-30 if i == 2 else 0.0)
+        res, status = hip.run_g0w0_group(inps, ctxs, 80, 12, prec)
+        assert list(status) == [hip.MGW_OK] * 4
+        judges = [c.last_judge() for c in ctxs]
+        say(f"   judges: {judges}")
+        for i, j in enumerate(judges):
+            assert j["samples"] == 2 and 0.0 < j["max_rel_err"] < 1e-6
+            assert j["fell_back"] == (1 if i == 2 else 0)
+        for i, (inp, c) in enumerate(zip(inps, ctxs)):
+            alone = hip.run_g0w0(inp, 80, 12, ctx=c, upload=False, precision=prec)
+            assert c.last_judge()["fell_back"] == (1 if i == 2 else 0)
+            assert _same(res[i], alone), i
+        f64 = hip.run_g0w0(inps[2], 80, 12, ctx=ctxs[2], upload=False, precision=hip.PREC_FP64)
+        assert _same(res[2], f64)
+    finally:
+        for c in ctxs:
+            c.close()
