@@ -86,24 +86,31 @@ __global__ void k_dielectric(const double* __restrict__ v, const double* __restr
 // held in registers: thread t owns row i = t / 4, columns c0 = 16 (t % 4) ..
 // c0 + 15 of the working block T (lower part of A_kk) and of B (starts as I,
 // ends as L^-1 up to row scaling).  Step j: the owners publish column j of T
-// and row j of B to LDS (double-buffered, one barrier per step); every row
-// i > j then applies the rank-1 update with the unscaled column and pivot
-// p_j = T[j][j]:  T[i][l] -= (T[i][j] / p_j) T[l][j] (j < l <= i),
-// B[i][l] -= (T[i][j] / p_j) B[j][l] (l <= j), as masked FMAs on vector-loaded
-// operands (no branches).  Finally L[i][c] = T[i][c] / sqrt(p_c) (c <= i) and
-// L^-1[i][c] = B[i][c] / sqrt(p_i).
+// (zero for rows i <= j; the pivot p_j = T[j][j] goes to its own slot) and row
+// j of B to LDS (double-buffered, one barrier per step); every row then
+// applies the rank-1 update with the unscaled column
+//   T[i][l] -= (T[i][j] / p_j) T[l][j],  B[i][l] -= (T[i][j] / p_j) B[j][l]
+// to all its 32 registers as plain FMAs.  The published zeros make it exact
+// where it must not act: rows i <= j have multiplier 0, columns l <= j of T
+// take 0 (so finished columns and the pivot column keep their values), and
+// B[j][l] is exactly 0 for l > j (B stays lower triangular).  The upper part
+// of T (l > i) collects values that nothing reads: rows publish 0 once the
+// step passes them and the final stores keep only l <= i.  Steps are
+// unrolled by 16 so the published register is fixed at compile time.
+// Finally L[i][c] = T[i][c] / sqrt(p_c) (c <= i) and L^-1[i][c] = B[i][c] /
+// sqrt(p_i).
 // pre != 0: the block first takes the product of block column k - 1 that the
 // calling sweep left out, A_kk -= L_k,k-1 L_k,k-1^T (FP64 MFMA tile, staged
 // through LDS).
 __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* __restrict__ dinv,
                                                int* __restrict__ info, int np, int k, int pre) {
     __shared__ double col[2][64], row[2][64];
-    __shared__ double rs[64];                    // 1 / sqrt(p_j)
+    __shared__ double pivs[64];                  // p_j, then 1 / sqrt(p_j)
     __shared__ double As[kBK][64 + kLdsPad];
     __shared__ double Bs[kBK][64 + kLdsPad];
     __shared__ double upd[64][65];
     const int w = blockIdx.x, t = threadIdx.x;
-    const int i = t >> 2, c0 = (t & 3) * 16;
+    const int i = t >> 2, q = t & 3, c0 = q * 16;
     const int nblk = np / kNB;
     double* Aw = A + (size_t)w * np * np;
     const size_t off = (size_t)k * kNB * np + (size_t)k * kNB;
@@ -118,61 +125,69 @@ __global__ __launch_bounds__(256) void k_potf2(double* __restrict__ A, double* _
     }
     double T[16], B[16];
     {
-        const double* src = Aw + off + (size_t)i * np + c0;
+        // the whole 64 x 64 block is in bounds: load it unconditionally, mask after
+        const double2* src = reinterpret_cast<const double2*>(Aw + off + (size_t)i * np + c0);
+        double v[16];
+#pragma unroll
+        for (int c = 0; c < 16; c += 2) {
+            const double2 x = src[c / 2];
+            v[c] = x.x;
+            v[c + 1] = x.y;
+        }
+        if (pre) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) v[c] -= upd[i][c0 + c];
+        }
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-            T[c] = (c0 + c <= i) ? (pre ? src[c] - upd[i][c0 + c] : src[c]) : 0.0;
-            B[c] = (c0 + c == i) ? 1.0 : 0.0;
+            T[c] = c0 + c <= i ? v[c] : 0.0;
+            B[c] = c0 + c == i ? 1.0 : 0.0;
         }
     }
     int bad = 0;
-    for (int j = 0; j < 64; ++j) {
-        const int buf = j & 1, jj = j & 15;
-        // publish (selects, no branches around the registers)
-        double cv = T[0];
+    for (int jb = 0; jb < 4; ++jb) {
 #pragma unroll
-        for (int c = 1; c < 16; ++c) cv = c == jj ? T[c] : cv;
-        if ((t & 3) == (j >> 4)) col[buf][i] = cv;
-        if (i == j) {
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = 16 * jb + jj, buf = jj & 1;
+            if (q == jb) {
+                col[buf][i] = i > j ? T[jj] : 0.0;
+                if (i == j) pivs[j] = T[jj];
+            }
+            if (i == j) {
 #pragma unroll
-            for (int c = 0; c < 16; c += 2)
-                *reinterpret_cast<double2*>(&row[buf][c0 + c]) = make_double2(B[c], B[c + 1]);
+                for (int c = 0; c < 16; c += 2)
+                    *reinterpret_cast<double2*>(&row[buf][c0 + c]) = make_double2(B[c], B[c + 1]);
+            }
+            __syncthreads();
+            const double piv = pivs[j];
+            if (!(piv > 0.0) && !bad) bad = j + 1;
+            const double ip = 1.0 / (piv > 0.0 ? piv : 1.0);
+            const double a = col[buf][i] * ip;
+#pragma unroll
+            for (int c = 0; c < 16; c += 2) {
+                const double2 x = *reinterpret_cast<const double2*>(&col[buf][c0 + c]);
+                const double2 y = *reinterpret_cast<const double2*>(&row[buf][c0 + c]);
+                T[c] = fma(-a, x.x, T[c]);
+                T[c + 1] = fma(-a, x.y, T[c + 1]);
+                B[c] = fma(-a, y.x, B[c]);
+                B[c + 1] = fma(-a, y.y, B[c + 1]);
+            }
         }
-        __syncthreads();
-        const double piv = col[buf][j];
-        if (!(piv > 0.0) && !bad) bad = j + 1;
-        const double pv = piv > 0.0 ? piv : 1.0;
-        const double ip = 1.0 / pv;
-        if (t == 0) rs[j] = 1.0 / sqrt(pv);
-        double cl[16], rl[16];
-#pragma unroll
-        for (int c = 0; c < 16; c += 2) {
-            const double2 x = *reinterpret_cast<const double2*>(&col[buf][c0 + c]);
-            const double2 y = *reinterpret_cast<const double2*>(&row[buf][c0 + c]);
-            cl[c] = x.x;
-            cl[c + 1] = x.y;
-            rl[c] = y.x;
-            rl[c + 1] = y.y;
-        }
-        const double a = i > j ? col[buf][i] * ip : 0.0;    // rows <= j are final
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const int l = c0 + c;
-            const double ut = (l > j && l <= i) ? cl[c] : 0.0;
-            const double ub = l <= j ? rl[c] : 0.0;
-            T[c] = fma(-a, ut, T[c]);
-            B[c] = fma(-a, ub, B[c]);
-        }
+    }
+    __syncthreads();                             // every wave has read pivs[63]
+    if (t < 64) {
+        const double p = pivs[t];
+        pivs[t] = 1.0 / sqrt(p > 0.0 ? p : 1.0);
     }
     __syncthreads();
     if (bad && t == 0 && info[w] == 0) info[w] = k * kNB + bad;
     double* D = dinv + ((size_t)w * nblk + k) * 64 * 64;
     double* dst = Aw + off + (size_t)i * np + c0;
-    const double ri = rs[i];
+    const double ri = pivs[i];
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const int l = c0 + c;
-        dst[c] = l <= i ? T[c] * rs[l] : 0.0;          // zeros above the diagonal
+        dst[c] = l <= i ? T[c] * pivs[l] : 0.0;        // zeros above the diagonal
         D[i * 64 + l] = l <= i ? B[c] * ri : 0.0;
     }
 }
