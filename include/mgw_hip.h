@@ -91,9 +91,10 @@ int mgw_hip_device_for_local_rank(size_t local_rank, size_t tasks_per_gpu, int* 
  * (0: FP64 block products in the mixed forms' factorization),
  * MGW_SPLIT_UNIT_FREQS (unit shape of the mixed contractions),
  * MGW_FACTOR_STREAMS (1, 2 or 4 streams for the parts of a frequency batch
- * in the factorization; default 2) and MGW_FACTOR_SPLIT_ORDER (smallest
- * matrix order that is split; default 4096).  The last three change
- * scheduling only, never a result. */
+ * in the factorization; default 2), MGW_FACTOR_SPLIT_ORDER (smallest
+ * matrix order that is split; default 4096) and MGW_ERI_SLAB_BYTES (largest
+ * slab of a streamed ERI upload; default 512 MiB, at least one row p).  The
+ * last four change scheduling only, never a result. */
 int mgw_hip_create(int device, mgw_hip_ctx** out);
 int mgw_hip_destroy(mgw_hip_ctx* ctx);
 
@@ -136,19 +137,54 @@ int mgw_hip_quadratic_forms_host(mgw_hip_ctx* ctx, const double* x_rowmajor, int
 
 /* ---- DevicePqPhPanelView (include/workspace/device_pq_ph_panel.hpp:67-99) */
 
-/* Upload the MO ERI (nmo^4, C order) and keep it resident in HBM; replaces
- * DevicePqPhPanelView(..., FullEriResident) (src/workspace/device_pq_ph_panel.cu:204-252). */
+/* How a context holds the ERI; mirrors DevicePqPhPanelStorageMode
+ * (include/workspace/device_pq_ph_panel.hpp:37-41, chosen in
+ * src/workspace/device_pq_ph_panel.cu:168-190).
+ *   MGW_ERI_RESIDENT: the whole nmo^4 tensor in HBM (FullEriResident).
+ *   MGW_ERI_STREAMED: the ERI passes through the device once, in slabs of
+ *     whole rows p (MGW_ERI_SLAB_BYTES each), and only what the path reads stays
+ *     resident: the particle-hole block (pk|ia) (nmo^2 x nocc nvir values) and
+ *     the exchange diagonal (pi|pi).  Replaces StreamingPanel, which keeps the
+ *     ERI on the host and copies every panel; every result is bit for bit the
+ *     resident one.  The decomposed form (MGW_PREC_FP64_CD) and
+ *     mgw_hip_download_eri need the whole tensor (MGW_ERR_UNSUPPORTED).
+ *   MGW_ERI_AUTO: RESIDENT while the tensor takes less than a quarter of the
+ *     free device memory (of the context's budget left, if smaller), else
+ *     STREAMED, by the reference's rule. */
+enum mgw_eri_storage {
+    MGW_ERI_AUTO = 0,
+    MGW_ERI_RESIDENT = 1,
+    MGW_ERI_STREAMED = 2
+};
+
+/* Storage of the ERI for the uploads that follow on ctx (default MGW_ERI_AUTO). */
+int mgw_hip_set_eri_storage(mgw_hip_ctx* ctx, int32_t storage);
+
+/* Storage the uploaded ERI took (MGW_ERI_RESIDENT or MGW_ERI_STREAMED) and the
+ * device bytes it holds; replaces storage_mode() / estimated_device_bytes()
+ * (include/workspace/device_pq_ph_panel.hpp:85-88).  MGW_ERR_STATE before an
+ * upload. */
+int mgw_hip_eri_storage(const mgw_hip_ctx* ctx, int32_t* storage, size_t* device_bytes);
+
+/* Device bytes an ERI of this size holds in a storage (RESIDENT or STREAMED,
+ * the latter with its upload slab of MGW_ERI_SLAB_BYTES as now set); 0 for
+ * invalid sizes. */
+size_t mgw_hip_eri_bytes(int32_t nmo, int32_t nocc, int32_t storage);
+
+/* Upload the MO ERI (nmo^4, C order) into the context's storage; replaces
+ * DevicePqPhPanelView's constructor (src/workspace/device_pq_ph_panel.cu:204-252). */
 int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t nocc);
 
-/* Build the resident ERI on the device from its factor B (nmo^2 x kdim,
- * row-major, symmetric in (p,q)): eri[pq][rs] = sum_K B[pq][K] B[rs][K],
- * formed with FP64 MFMA and exactly symmetric in (pq, rs).  Saves the host
- * product and the nmo^4 upload for factor-form inputs (density fitting /
- * Cholesky vectors, the seeded synthetic molecules). */
+/* Build the ERI on the device from its factor B (nmo^2 x kdim, row-major,
+ * symmetric in (p,q)): eri[pq][rs] = sum_K B[pq][K] B[rs][K], formed with
+ * FP64 MFMA and exactly symmetric in (pq, rs), into the context's storage (a
+ * streamed one forms one slab of rows at a time, same values bit for bit).
+ * Saves the host product and the nmo^4 upload for factor-form inputs
+ * (density fitting / Cholesky vectors, the seeded synthetic molecules). */
 int mgw_hip_upload_eri_factor(mgw_hip_ctx* ctx, const double* b, int32_t nmo, int32_t nocc,
                               int32_t kdim);
 
-/* Copy the resident ERI (nmo^4, C order) back to the host. */
+/* Copy the resident ERI (nmo^4, C order) back to the host (MGW_ERI_RESIDENT only). */
 int mgw_hip_download_eri(mgw_hip_ctx* ctx, double* eri);
 
 /* Free every device buffer the context holds (the context stays usable). */
@@ -342,6 +378,18 @@ int mgw_hip_server_set_grouping(mgw_hip_server* server, int32_t max_jobs);
  * budget, so that a scale below 1 makes the job's allocations fail with
  * MGW_ERR_MEMORY as an HBM shortfall would.  Default 1. */
 int mgw_hip_server_set_budget_scale(mgw_hip_server* server, double scale);
+
+/* ERI storage of the jobs submitted from now on (enum mgw_eri_storage):
+ * MGW_ERI_RESIDENT (default) or MGW_ERI_STREAMED for every job; with
+ * MGW_ERI_AUTO a job starts resident when its resident estimate fits what is
+ * left of the budget, else streamed when its streamed estimate fits, and
+ * fails with MGW_ERR_MEMORY only if even that exceeds the whole budget.  No
+ * counterpart in the reference (its panel view chooses per process). */
+int mgw_hip_server_set_eri_storage(mgw_hip_server* server, int32_t storage);
+
+/* Storage a started job's ERI took (MGW_ERI_RESIDENT or MGW_ERI_STREAMED;
+ * MGW_ERI_AUTO while the job has not started). */
+int mgw_hip_server_job_eri_storage(mgw_hip_server* server, int64_t ticket, int32_t* storage);
 
 /* Queue a job.  mo_energy / vxc_diag / states are copied; the ERI or factor
  * and the output buffers must stay valid until mgw_hip_server_wait reports the

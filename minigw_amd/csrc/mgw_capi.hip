@@ -32,6 +32,14 @@ namespace {
 
 thread_local std::string g_thread_error;
 
+// Largest slab of whole ERI rows p that a streamed upload moves at once:
+// MGW_ERI_SLAB_BYTES, default 512 MiB.
+size_t env_slab_bytes() {
+    if (const char* env = std::getenv("MGW_ERI_SLAB_BYTES"))
+        return (size_t)std::max(1ll, std::atoll(env));
+    return (size_t)512 << 20;
+}
+
 struct MgwError : std::runtime_error {
     int code;
     MgwError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -193,6 +201,15 @@ struct mgw_hip_ctx {
     Buf eri, v, colidx, rowbase, delta;
     int nmo = 0, nocc = 0;        // of the resident ERI (0 = none)
     bool eri_pk_sym = false;      // (pk|ia) == (kp|ia) exactly for the resident ERI
+    // ERI storage: requested for the next upload, taken by the held one
+    // (MGW_ERI_RESIDENT: eri = nmo^4 tensor; MGW_ERI_STREAMED: eri = the
+    // (pk|ia) block, rows of nph values, and exsd = (pi|pi)), values per row
+    // (p, k) of what eri holds
+    int eri_req = MGW_ERI_AUTO;
+    int eri_storage = 0;
+    int64_t eri_rs = 0;
+    Buf exsd, eslab;
+    size_t slab_bytes = 0;            // largest streamed slab (env_slab_bytes at creation)
     Buf flag, hcolmap, wxp, hxp, f0buf;
 
     // frequency batch forced by the autotuner while it times candidates (0 = auto)
@@ -263,7 +280,7 @@ struct mgw_hip_ctx {
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
                &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj, &luz, &ludl, &ludu,
-               &lupiv, &wz, &ppk, &panel, &hinfob, &hjom, &hjinfo};
+               &lupiv, &wz, &ppk, &panel, &hinfob, &hjom, &hjinfo, &exsd, &eslab};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -378,7 +395,12 @@ struct PhLayout {
     std::vector<int64_t> rowbase;   // np: (j*nmo + nocc + b) * nmo^2 (0 in the padding)
 };
 
-PhLayout ph_layout(int nmo, int nocc) {
+// values per (p, k) row of the ERI as a context holds it
+int64_t eri_row_values(int nmo, int nocc, int storage) {
+    return storage == MGW_ERI_STREAMED ? (int64_t)nocc * (nmo - nocc) : (int64_t)nmo * nmo;
+}
+
+PhLayout ph_layout(int nmo, int nocc, int storage = MGW_ERI_RESIDENT) {
     PhLayout L;
     L.nmo = nmo;
     L.nocc = nocc;
@@ -387,12 +409,14 @@ PhLayout ph_layout(int nmo, int nocc) {
     L.np = mgw::round_up(std::max(L.n, 1), mgw::kPad);
     L.colidx.assign(L.np, 0);
     L.rowbase.assign(L.np, 0);
-    const int64_t n2 = (int64_t)nmo * nmo;
+    // the streamed storage holds only the ph columns, in ph order
+    const bool held_ph = storage == MGW_ERI_STREAMED;
+    const int64_t rs = eri_row_values(nmo, nocc, storage);
     for (int i = 0; i < nocc; ++i)
         for (int a = 0; a < L.nvir; ++a) {
             const int ph = i * L.nvir + a;
-            L.colidx[ph] = i * nmo + nocc + a;
-            L.rowbase[ph] = ((int64_t)i * nmo + nocc + a) * n2;
+            L.colidx[ph] = held_ph ? ph : i * nmo + nocc + a;
+            L.rowbase[ph] = ((int64_t)i * nmo + nocc + a) * rs;
         }
     return L;
 }
@@ -713,6 +737,7 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         if (const char* env = std::getenv("MGW_FACTOR_STREAMS"))
             ctx->factor_streams = env[0] == '1' ? 1 : env[0] == '4' ? 4 : 2;
         if (const char* env = std::getenv("MGW_FACTOR_SPLIT_ORDER")) ctx->split_order = std::max(0, std::atoi(env));
+        ctx->slab_bytes = env_slab_bytes();
         if (const char* env = std::getenv("MGW_FREQ_BATCH_CAP"))
             ctx->batch_cap = std::min(1024, std::max(1, std::atoi(env)));
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1021,38 +1046,138 @@ int mgw_hip_quadratic_forms_host(mgw_hip_ctx* ctx, const double* x, int32_t n, i
 
 // ---------------------------------------------------- DevicePqPhPanelView
 namespace {
-// ERI-resident bookkeeping shared by both upload paths: ph tables and the
-// (pk|ia) == (kp|ia) check.
-void finish_eri_upload(mgw_hip_ctx* ctx, int nmo, int nocc) {
-    const PhLayout L = ph_layout(nmo, nocc);
+// ERI bookkeeping shared by both upload paths: ph tables of the storage and
+// the (pk|ia) == (kp|ia) check.
+void finish_eri_upload(mgw_hip_ctx* ctx, int nmo, int nocc, int storage) {
+    const PhLayout L = ph_layout(nmo, nocc, storage);
     int* col = as<int>(ctx->get(ctx->colidx, (size_t)L.np * 4, "colidx"));
     int64_t* rb = as<int64_t>(ctx->get(ctx->rowbase, (size_t)L.np * 8, "rowbase"));
     ctx->h2d(col, L.colidx.data(), L.np);
     ctx->h2d(rb, L.rowbase.data(), L.np);
     int* dflag = as<int>(ctx->get(ctx->flag, sizeof(int), "flag"));
     hip_check(hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
-    hip_check(mgw::launch_check_pk_symmetry(as<double>(ctx->eri.p), nmo, nocc, dflag, ctx->stream),
+    const bool held_ph = storage == MGW_ERI_STREAMED;
+    hip_check(mgw::launch_check_pk_symmetry(as<double>(ctx->eri.p), nmo, nocc,
+                                            eri_row_values(nmo, nocc, storage),
+                                            held_ph ? L.nvir : nmo, held_ph ? 0 : nocc, dflag,
+                                            ctx->stream),
               "symmetry check");
     int asym = 1;
     ctx->d2h(&asym, dflag, 1);
     ctx->sync();
     ctx->eri_pk_sym = (asym == 0);
     ctx->eri_pair_sym = -1;
+    ctx->eri_storage = storage;
+    ctx->eri_rs = eri_row_values(nmo, nocc, storage);
     ctx->nmo = nmo;
     ctx->nocc = nocc;
 }
+
+
+size_t resident_eri_bytes(int nmo) { return (size_t)nmo * nmo * nmo * nmo * 8; }
+
+// rows p per slab: as many as fit slab_bytes, at least one
+int slab_rows(int nmo, size_t slab_bytes) {
+    const size_t row = (size_t)nmo * nmo * nmo * 8;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nmo, slab_bytes / row));
+}
+
+size_t streamed_eri_bytes(int nmo, int nocc, size_t slab_bytes) {
+    const size_t held = ((size_t)nmo * nmo * nocc * (nmo - nocc) + (size_t)nmo * nocc) * 8;
+    return held + (size_t)slab_rows(nmo, slab_bytes) * nmo * nmo * nmo * 8;
+}
+
+// The storage an upload takes: the requested one, or for MGW_ERI_AUTO the
+// reference's rule (src/workspace/device_pq_ph_panel.cu:168-190): resident
+// only while the tensor takes less than a quarter of the free device memory
+// (of the context's budget left, when that is smaller).  The context's
+// previous ERI is released first so that it does not count against the new one.
+int choose_storage(mgw_hip_ctx* ctx, int nmo) {
+    ctx->nmo = 0;
+    ctx->eri_storage = 0;
+    ctx->release(ctx->eri);
+    ctx->release(ctx->exsd);
+    if (ctx->eri_req != MGW_ERI_AUTO) return ctx->eri_req;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return MGW_ERI_STREAMED;
+    }
+    if (ctx->budget) free_b = std::min(free_b, ctx->budget > ctx->held ? ctx->budget - ctx->held : 0);
+    return resident_eri_bytes(nmo) < free_b / 4 ? MGW_ERI_RESIDENT : MGW_ERI_STREAMED;
+}
+
+// Streamed storage: slabs of whole rows p, each copied from the host ERI or
+// formed from the factor (db: Np + 64 rows of Kp, zero padded), gathered into
+// the held (pk|ia) block and (pi|pi), one slab buffer reused in stream order.
+void upload_streamed(mgw_hip_ctx* ctx, int nmo, int nocc, const double* host, const double* db,
+                     int Kp) {
+    const int n = nocc * (nmo - nocc), N = nmo * nmo;
+    const size_t row = (size_t)nmo * nmo * nmo;
+    const int rows = slab_rows(nmo, ctx->slab_bytes);
+    double* eph = as<double>(ctx->get(ctx->eri, (size_t)N * n * 8, "ERI ph block"));
+    double* sxd = as<double>(ctx->get(ctx->exsd, (size_t)nmo * nocc * 8, "exchange diagonal"));
+    double* slab = as<double>(ctx->get(ctx->eslab, (size_t)rows * row * 8, "ERI slab"));
+    for (int p0 = 0; p0 < nmo; p0 += rows) {
+        const int r = std::min(rows, nmo - p0);
+        if (host)
+            ctx->h2d(slab, host + (size_t)p0 * row, (size_t)r * row);
+        else
+            hip_check(mgw::launch_eri_rows_from_factor(db, N, Kp, p0 * nmo, r * nmo, slab,
+                                                       ctx->stream),
+                      "ERI rows from factor");
+        hip_check(mgw::launch_gather_ph(slab, p0, r, nmo, nocc, eph, sxd, ctx->stream),
+                  "gather ph block");
+    }
+    ctx->sync();
+    ctx->release(ctx->eslab);
+}
 }  // namespace
+
+int mgw_hip_set_eri_storage(mgw_hip_ctx* ctx, int32_t storage) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, "mgw_hip_set_eri_storage: ctx is NULL");
+        need(storage == MGW_ERI_AUTO || storage == MGW_ERI_RESIDENT || storage == MGW_ERI_STREAMED,
+             "mgw_hip_set_eri_storage: unknown storage %d", storage);
+        ctx->eri_req = storage;
+    });
+}
+
+int mgw_hip_eri_storage(const mgw_hip_ctx* ctx, int32_t* storage, size_t* device_bytes) {
+    if (!ctx || !storage) {
+        g_thread_error = "mgw_hip_eri_storage: NULL argument";
+        return MGW_ERR_ARG;
+    }
+    if (!ctx->nmo) {
+        g_thread_error = "mgw_hip_eri_storage: no ERI uploaded";
+        return MGW_ERR_STATE;
+    }
+    *storage = ctx->eri_storage;
+    if (device_bytes) *device_bytes = ctx->eri.bytes + ctx->exsd.bytes;
+    return MGW_OK;
+}
+
+size_t mgw_hip_eri_bytes(int32_t nmo, int32_t nocc, int32_t storage) {
+    if (nmo <= 1 || nocc <= 0 || nocc >= nmo) return 0;
+    if (storage == MGW_ERI_RESIDENT) return resident_eri_bytes(nmo);
+    if (storage == MGW_ERI_STREAMED) return streamed_eri_bytes(nmo, nocc, env_slab_bytes());
+    return 0;
+}
 
 int mgw_hip_upload_eri(mgw_hip_ctx* ctx, const double* eri, int32_t nmo, int32_t nocc) {
     return guarded(ctx, [&] {
         need(ctx && eri, "mgw_hip_upload_eri: NULL argument");
         need(nmo > 0, "MolecularIntegrals: ERI tensor must have shape [nmo,nmo,nmo,nmo]");
         need(nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
-        const size_t count = (size_t)nmo * nmo * nmo * nmo;
-        ctx->nmo = 0;
-        double* d = as<double>(ctx->get(ctx->eri, count * 8, "ERI"));
-        ctx->h2d(d, eri, count);
-        finish_eri_upload(ctx, nmo, nocc);
+        const int storage = choose_storage(ctx, nmo);
+        if (storage == MGW_ERI_STREAMED) {
+            upload_streamed(ctx, nmo, nocc, eri, nullptr, 0);
+        } else {
+            const size_t count = (size_t)nmo * nmo * nmo * nmo;
+            double* d = as<double>(ctx->get(ctx->eri, count * 8, "ERI"));
+            ctx->h2d(d, eri, count);
+        }
+        finish_eri_upload(ctx, nmo, nocc, storage);
     });
 }
 
@@ -1064,17 +1189,23 @@ int mgw_hip_upload_eri_factor(mgw_hip_ctx* ctx, const double* b, int32_t nmo, in
         need(nocc > 0 && nocc < nmo, "OrbitalSpace: invalid number of occupied orbitals");
         const int N = nmo * nmo;
         const int Np = mgw::round_up(N, 64), Kp = mgw::round_up(kdim, 16);
-        std::vector<double> bp((size_t)Np * Kp, 0.0);
+        const int storage = choose_storage(ctx, nmo);
+        // 64 zero rows past Np: the row-slab product reads whole 64-row tiles
+        const int rows_b = storage == MGW_ERI_STREAMED ? Np + 64 : Np;
+        std::vector<double> bp((size_t)rows_b * Kp, 0.0);
         for (int r = 0; r < N; ++r)
             std::copy(b + (size_t)r * kdim, b + (size_t)(r + 1) * kdim, bp.begin() + (size_t)r * Kp);
-        ctx->nmo = 0;
-        double* d = as<double>(ctx->get(ctx->eri, (size_t)N * N * 8, "ERI"));
         double* db = as<double>(ctx->get(ctx->wsrc, bp.size() * 8, "ERI factor"));
         ctx->h2d(db, bp.data(), bp.size());
-        hip_check(mgw::launch_eri_from_factor(db, N, Kp, d, ctx->stream), "ERI from factor");
-        ctx->sync();
+        if (storage == MGW_ERI_STREAMED) {
+            upload_streamed(ctx, nmo, nocc, nullptr, db, Kp);
+        } else {
+            double* d = as<double>(ctx->get(ctx->eri, (size_t)N * N * 8, "ERI"));
+            hip_check(mgw::launch_eri_from_factor(db, N, Kp, d, ctx->stream), "ERI from factor");
+            ctx->sync();
+        }
         ctx->release(ctx->wsrc);
-        finish_eri_upload(ctx, nmo, nocc);
+        finish_eri_upload(ctx, nmo, nocc, storage);
     });
 }
 
@@ -1082,6 +1213,9 @@ int mgw_hip_download_eri(mgw_hip_ctx* ctx, double* eri) {
     return guarded(ctx, [&] {
         need(ctx && eri, "mgw_hip_download_eri: NULL argument");
         if (!ctx->nmo) fail(MGW_ERR_STATE, "mgw_hip_download_eri: no resident ERI");
+        if (ctx->eri_storage != MGW_ERI_RESIDENT)
+            fail(MGW_ERR_UNSUPPORTED,
+                 "mgw_hip_download_eri: the streamed storage holds only the (pk|ia) block");
         const size_t count = (size_t)ctx->nmo * ctx->nmo * ctx->nmo * ctx->nmo;
         ctx->d2h(eri, ctx->eri.p, count);
         ctx->sync();
@@ -1094,6 +1228,7 @@ int mgw_hip_release_buffers(mgw_hip_ctx* ctx) {
         ctx->sync();
         for (auto* b : ctx->all) ctx->release(*b);
         ctx->nmo = ctx->nocc = 0;
+        ctx->eri_storage = 0;
         ctx->n = ctx->np = 0;
         ctx->v_ready = ctx->wc_ready = false;
     });
@@ -1109,13 +1244,12 @@ int mgw_hip_quadratic_forms(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t wid
         need(p >= 0 && p < nmo, "DevicePqPhPanelView::fill_panel: p index out of range");
         need(k0 >= 0 && width >= 0 && k0 <= nmo && width <= nmo - k0,
              "DevicePqPhPanelView::fill_panel: k panel exceeds nmo");
-        const PhLayout L = ph_layout(nmo, ctx->nocc);
+        const PhLayout L = ph_layout(nmo, ctx->nocc, ctx->eri_storage);
         need(L.n == ctx->n, "quadratic forms: V_ph dimension %d does not match the ERI (%d)",
              ctx->n, L.n);
         if (width == 0) return;
         std::vector<int64_t> pk(width);
-        const int64_t n2 = (int64_t)nmo * nmo;
-        for (int c = 0; c < width; ++c) pk[c] = ((int64_t)p * nmo + k0 + c) * n2;
+        for (int c = 0; c < width; ++c) pk[c] = ((int64_t)p * nmo + k0 + c) * ctx->eri_rs;
         contract_single(ctx, as<double>(ctx->eri.p), pk.data(), width, L.colidx.data(), q_re,
                         q_im);
     });
@@ -1151,7 +1285,7 @@ int mgw_hip_quadratic_forms_noalloc(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int
         double* qpart = reinterpret_cast<double*>(base + kNoAllocPk + kNoAllocX);
         int* f0p = reinterpret_cast<int*>(base + kNoAllocPk + kNoAllocX + kNoAllocQpart);
         hipStream_t st = ctx->stream;
-        hip_check(mgw::launch_pk_rows(dpk, p, k0, width, nmo, mpad, st), "panel rows");
+        hip_check(mgw::launch_pk_rows(dpk, p, k0, width, nmo, mpad, ctx->eri_rs, st), "panel rows");
         hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, ctx->n,
                                      np, mpad, st),
                   "pack X");
@@ -1178,7 +1312,8 @@ int mgw_hip_fill_panel(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int32_t width, c
         const int mpad = mgw::round_up(std::max(width, 1), mgw::kPad);
         int64_t* dpk = as<int64_t>(ctx->get(ctx->ppk, (size_t)mpad * 8, "panel rows"));
         double* xp = as<double>(ctx->get(ctx->panel, (size_t)mpad * L.np * 8, "device panel"));
-        hip_check(mgw::launch_pk_rows(dpk, p, k0, width, nmo, mpad, ctx->stream), "panel rows");
+        hip_check(mgw::launch_pk_rows(dpk, p, k0, width, nmo, mpad, ctx->eri_rs, ctx->stream),
+                  "panel rows");
         hip_check(mgw::launch_pack_x(as<double>(ctx->eri.p), dpk, as<int>(ctx->colidx.p), xp, L.n,
                                      L.np, mpad, ctx->stream),
                   "pack X");
@@ -1282,6 +1417,10 @@ void mol_prepare(MolRun& r, mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_
                  job->precision == MGW_PREC_MIXED_INT8_4,
              "mgw_hip_g0w0: unknown precision %d", job->precision);
         const bool cd = job->precision == MGW_PREC_FP64_CD;
+        if (cd && ctx->eri_storage != MGW_ERI_RESIDENT)
+            fail(MGW_ERR_UNSUPPORTED,
+                 "MGW_PREC_FP64_CD decomposes the whole ERI pair matrix: it needs MGW_ERI_RESIDENT "
+                 "storage");
         r.nmo = nmo;
         r.nocc = nocc;
         r.nfreq = nfreq;
@@ -1351,8 +1490,8 @@ void mol_prepare(MolRun& r, mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_
         r.weight.resize(nfreq);
         transformed_legendre(nfreq, r.omega.data(), r.weight.data());
         r.pkrow.assign(mpad, 0);
-        const int64_t n2 = (int64_t)nmo * nmo;
-        for (int c = 0; c < mu; ++c) r.pkrow[c] = ((int64_t)cols[c].first * nmo + cols[c].second) * n2;
+        for (int c = 0; c < mu; ++c)
+            r.pkrow[c] = ((int64_t)cols[c].first * nmo + cols[c].second) * ctx->eri_rs;
 
         hipStream_t st = ctx->stream;
         for (auto& e : r.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -1944,7 +2083,13 @@ void mol_tail_begin(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out, Tail& t) {
         hip_check(hipEventRecord(t.ev[1], st), "hipEventRecord");
         {
             Range rg("exchange");
-            hip_check(mgw::launch_sigma_x(as<double>(ctx->eri.p), r.sx, nmo, r.nocc, st), "sigma_x");
+            // (pi|pi): eri[p, i, p, i] of the tensor, or the streamed storage's diagonal
+            const int64_t n2 = (int64_t)nmo * nmo;
+            const bool res = ctx->eri_storage == MGW_ERI_RESIDENT;
+            hip_check(mgw::launch_sigma_x(as<double>(res ? ctx->eri.p : ctx->exsd.p), r.sx, nmo,
+                                          r.nocc, res ? n2 * nmo + nmo : r.nocc, res ? n2 + 1 : 1,
+                                          st),
+                      "sigma_x");
         }
         Range rg("pade/output pade");
         t.nan.assign(nmo, std::numeric_limits<double>::quiet_NaN());

@@ -34,6 +34,16 @@ hipError_t launch_probe_f64(const double* a, const double* b, double* d, int nte
 // eri[P][R] = sum_K B[P][K] B[R][K] (P, R < N) from B padded to ((N+63)/64*64) x Kp,
 // Kp % 16 == 0, extra rows and columns zero; exactly symmetric in (P, R).
 hipError_t launch_eri_from_factor(const double* b, int N, int Kp, double* eri, hipStream_t s);
+// Rows P0 .. P0 + rows - 1 of the same product, all N columns: out[(P - P0) N + R],
+// bit for bit the values launch_eri_from_factor writes; B needs 64 zero rows
+// past P0 + rows.
+hipError_t launch_eri_rows_from_factor(const double* b, int N, int Kp, int P0, int rows,
+                                      double* out, hipStream_t s);
+// A slab of ERI rows p in [p0, p0 + rows) (nmo^3 values per p, C order) into
+// the held ph block eph[(p nmo + k) n + i nvir + a] = (pk|ia) (n = nocc nvir)
+// and the exchange diagonal sxd[p nocc + i] = (pi|pi).
+hipError_t launch_gather_ph(const double* slab, int p0, int rows, int nmo, int nocc, double* eph,
+                            double* sxd, hipStream_t s);
 
 // V[r][c] = eri[rowbase[r] + colidx[c]] for r, c < n; 0 in the padding (np x np).
 hipError_t launch_build_v(const double* eri, const int64_t* rowbase, const int* colidx,
@@ -125,15 +135,18 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* x
 
 // nrt = row tiles of the contraction that wrote qpart (np / 128 for FP64,
 // contract_split_row_tiles(np) for the split form).
-// pkrow[c] = ((p nmo + k0 + c) nmo^2) for c < width, 0 for width <= c < mpad
-hipError_t launch_pk_rows(int64_t* pkrow, int p, int k0, int width, int nmo, int mpad,
+// pkrow[c] = (p nmo + k0 + c) rs for c < width, 0 for width <= c < mpad (rs:
+// values per (p, k) row of the held ERI)
+hipError_t launch_pk_rows(int64_t* pkrow, int p, int k0, int width, int nmo, int mpad, int64_t rs,
                           hipStream_t s);
 hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, int nrt, int m,
                            int mpad, const int* f0p, int nb, hipStream_t s);
 
-// flag = 1 if (pk|ia) != (kp|ia) for some p, k, i < nocc <= a (flag must start at 0).
-hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int* flag,
-                                    hipStream_t s);
+// flag = 1 if (pk|ia) != (kp|ia) for some p, k, i < nocc <= a (flag must start
+// at 0), (pk|ia) at eri[(p nmo + k) rs + i cs + co + a - nocc] (a - nocc: the
+// virtual index).
+hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int64_t rs, int cs,
+                                    int co, int* flag, hipStream_t s);
 
 // Mixed precision: 16-bit hi/lo splits of the contraction operands.
 //   BF16: x = hi + lo with hi = bf16(x), lo = bf16(x - hi); no scaling.
@@ -188,8 +201,9 @@ hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_
                           int nstates, int nfreq, double2* gbuf, double* sig_re, double* sig_im,
                           hipStream_t s);
 
-// sx[p] = -sum_{i < nocc} eri[p, i, p, i]
-hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s);
+// sx[p] = -sum_{i < nocc} (pi|pi), with (pi|pi) at src[p sp + i si]
+hipError_t launch_sigma_x(const double* src, double* sx, int nmo, int nocc, int64_t sp, int64_t si,
+                          hipStream_t s);
 
 // Pade fit + QP fixed point per state (FP64, no FMA contraction).
 // Thiele Pade of one state's row evaluated at x_start + i * interval - fermi.

@@ -375,12 +375,13 @@ __global__ void k_pack_x(const double* __restrict__ src, const int64_t* __restri
     for (int k = lane; k < np; k += 64) o[k] = k < n ? src[base + colidx[k]] : 0.0;
 }
 
-// pkrow[c] = ERI row offset of column (p, k0 + c) for c < width, 0 in the padding
-__global__ void k_pk_rows(int64_t* __restrict__ pkrow, int p, int k0, int width, int nmo, int mpad) {
+// pkrow[c] = offset of row (p, k0 + c) of the held ERI (rows of rs values) for
+// c < width, 0 in the padding
+__global__ void k_pk_rows(int64_t* __restrict__ pkrow, int p, int k0, int width, int nmo, int mpad,
+                          int64_t rs) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= mpad) return;
-    const int64_t n2 = (int64_t)nmo * nmo;
-    pkrow[c] = c < width ? ((int64_t)p * nmo + k0 + c) * n2 : 0;
+    pkrow[c] = c < width ? ((int64_t)p * nmo + k0 + c) * rs : 0;
 }
 
 __global__ void k_reduce_q(const double* __restrict__ qpart, const int* __restrict__ colmap,
@@ -396,11 +397,11 @@ __global__ void k_reduce_q(const double* __restrict__ qpart, const int* __restri
 }
 
 // flag = 1 if (pk|ia) != (kp|ia) for any p, k and occupied-virtual (i, a).
-__global__ void k_check_pk_symmetry(const double* __restrict__ eri, int nmo, int nocc,
-                                    int* __restrict__ flag) {
+// (pk|ia) at eri[(p nmo + k) rs + i cs + co + a]
+__global__ void k_check_pk_symmetry(const double* __restrict__ eri, int nmo, int nocc, int64_t rs,
+                                    int cs, int co, int* __restrict__ flag) {
     const int nvir = nmo - nocc;
     const size_t total = (size_t)nmo * nmo * nocc * nvir;
-    const size_t n2 = (size_t)nmo * nmo;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (size_t)gridDim.x * blockDim.x) {
         const int a = (int)(e % nvir);
@@ -408,8 +409,30 @@ __global__ void k_check_pk_symmetry(const double* __restrict__ eri, int nmo, int
         const size_t pk = e / ((size_t)nocc * nvir);
         const int p = (int)(pk / nmo), k = (int)(pk % nmo);
         if (p >= k) continue;
-        const size_t col = (size_t)i * nmo + nocc + a;
-        if (eri[((size_t)p * nmo + k) * n2 + col] != eri[((size_t)k * nmo + p) * n2 + col]) *flag = 1;
+        const size_t col = (size_t)i * cs + co + a;
+        if (eri[((size_t)p * nmo + k) * rs + col] != eri[((size_t)k * nmo + p) * rs + col]) *flag = 1;
+    }
+}
+
+// One slab of ERI rows p in [p0, p0 + rows) (rows (p, k) of nmo^2 values, C
+// order) -> the held parts: eph[(p nmo + k) n + i nvir + a] = (pk|ia) and
+// sxd[p nocc + i] = (pi|pi), i < nocc <= a.
+__global__ void k_gather_ph(const double* __restrict__ slab, int p0, int rows, int nmo, int nocc,
+                            double* __restrict__ eph, double* __restrict__ sxd) {
+    const int nvir = nmo - nocc, n = nocc * nvir;
+    const size_t n2 = (size_t)nmo * nmo;
+    const size_t total = (size_t)rows * nmo * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int ia = (int)(e % n);
+        const size_t pk = e / n;                       // slab row (p - p0) nmo + k
+        const int i = ia / nvir, a = ia % nvir;
+        eph[(size_t)p0 * nmo * n + e] = slab[pk * n2 + (size_t)i * nmo + nocc + a];
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < rows * nocc) {
+        const int pr = t / nocc, i = t % nocc, p = p0 + pr;
+        sxd[(size_t)p * nocc + i] = slab[((size_t)pr * nmo + i) * n2 + (size_t)p * nmo + i];
     }
 }
 
@@ -601,13 +624,13 @@ __global__ void k_sigma_reduce(const double* __restrict__ part, int nsplit, int 
     sim[e] = -im / pi;
 }
 
-__global__ void k_sigma_x(const double* __restrict__ eri, double* __restrict__ sx, int nmo,
-                          int nocc) {
+// sx[p] = -sum_{i < nocc} (pi|pi), (pi|pi) at src[p sp + i si]
+__global__ void k_sigma_x(const double* __restrict__ src, double* __restrict__ sx, int nmo,
+                          int nocc, int64_t sp, int64_t si) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nmo) return;
-    const size_t n2 = (size_t)nmo * nmo;
     double v = 0.0;
-    for (int i = 0; i < nocc; ++i) v -= eri[((size_t)p * nmo + i) * n2 + (size_t)p * nmo + i];
+    for (int i = 0; i < nocc; ++i) v -= src[(size_t)p * sp + (size_t)i * si];
     sx[p] = v;
 }
 
@@ -737,6 +760,27 @@ __global__ __launch_bounds__(256) void k_eri_from_factor(const double* __restric
     });
 }
 
+// Rows P in [P0, P0 + rows) of the same product, every column R < N:
+// out[(P - P0) N + R].  Tile (I, J) takes rows P0 + 64 I.. against rows 64 J..
+// of B with the K order of k_eri_from_factor; since each term B[P][K] B[R][K]
+// is the same number either way round, every element equals that kernel's
+// bit for bit.  B needs 64 readable rows past P0 + rows (zero padding).
+__global__ __launch_bounds__(256) void k_eri_rows_from_factor(const double* __restrict__ b, int N,
+                                                              int Kp, int P0, int rows,
+                                                              double* __restrict__ out) {
+    __shared__ double As[kBK][64 + kLdsPad];
+    __shared__ double Bs[kBK][64 + kLdsPad];
+    const int I = blockIdx.y, J = blockIdx.x;
+    Tile64 tile;
+    tile.zero();
+    gemm64<false>(b + (size_t)(P0 + I * 64) * Kp, Kp, b + (size_t)J * 64 * Kp, Kp, 0, Kp, tile, As,
+                  Bs);
+    tile.for_each([&](int r, int c, double v) {
+        const int P = I * 64 + r, R = J * 64 + c;
+        if (P < rows && R < N) out[(size_t)P * N + R] = v;
+    });
+}
+
 // Diagnostic: C = A B^T for 64 x 64 row-major A, B through the Tile64 path.
 __global__ __launch_bounds__(256) void k_tile_selftest(const double* __restrict__ a,
                                                        const double* __restrict__ b,
@@ -786,6 +830,22 @@ hipError_t launch_probe_f64(const double* a, const double* b, double* d, int nte
 hipError_t launch_eri_from_factor(const double* b, int N, int Kp, double* eri, hipStream_t s) {
     const int nt = (N + 63) / 64;
     hipLaunchKernelGGL(k_eri_from_factor, dim3(nt * (nt + 1) / 2), dim3(256), 0, s, b, N, Kp, eri);
+    return hipGetLastError();
+}
+
+hipError_t launch_eri_rows_from_factor(const double* b, int N, int Kp, int P0, int rows,
+                                      double* out, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eri_rows_from_factor, dim3((N + 63) / 64, (rows + 63) / 64), dim3(256), 0,
+                       s, b, N, Kp, P0, rows, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_ph(const double* slab, int p0, int rows, int nmo, int nocc, double* eph,
+                            double* sxd, hipStream_t s) {
+    const size_t total = (size_t)rows * nmo * nocc * (nmo - nocc);
+    const int g = std::max(grid_for(total), (rows * nocc + 255) / 256);
+    hipLaunchKernelGGL(k_gather_ph, dim3(g), dim3(256), 0, s, slab, p0, rows, nmo, nocc, eph, sxd);
     return hipGetLastError();
 }
 
@@ -887,10 +947,10 @@ hipError_t launch_contract_f64(const double* M, const double* S, const double* x
     return hipGetLastError();
 }
 
-hipError_t launch_pk_rows(int64_t* pkrow, int p, int k0, int width, int nmo, int mpad,
+hipError_t launch_pk_rows(int64_t* pkrow, int p, int k0, int width, int nmo, int mpad, int64_t rs,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_pk_rows, dim3((mpad + 255) / 256), dim3(256), 0, s, pkrow, p, k0, width,
-                       nmo, mpad);
+                       nmo, mpad, rs);
     return hipGetLastError();
 }
 
@@ -901,11 +961,11 @@ hipError_t launch_reduce_q(const double* qpart, const int* colmap, double* q, in
     return hipGetLastError();
 }
 
-hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int* flag,
-                                    hipStream_t s) {
+hipError_t launch_check_pk_symmetry(const double* eri, int nmo, int nocc, int64_t rs, int cs,
+                                    int co, int* flag, hipStream_t s) {
     const size_t total = (size_t)nmo * nmo * nocc * (nmo - nocc);
     hipLaunchKernelGGL(k_check_pk_symmetry, dim3(grid_for(total, 256, 65536)), dim3(256), 0, s, eri,
-                       nmo, nocc, flag);
+                       nmo, nocc, rs, cs, co, flag);
     return hipGetLastError();
 }
 
@@ -946,8 +1006,9 @@ hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_
     return hipGetLastError();
 }
 
-hipError_t launch_sigma_x(const double* eri, double* sx, int nmo, int nocc, hipStream_t s) {
-    hipLaunchKernelGGL(k_sigma_x, dim3((nmo + 63) / 64), dim3(64), 0, s, eri, sx, nmo, nocc);
+hipError_t launch_sigma_x(const double* src, double* sx, int nmo, int nocc, int64_t sp, int64_t si,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_sigma_x, dim3((nmo + 63) / 64), dim3(64), 0, s, src, sx, nmo, nocc, sp, si);
     return hipGetLastError();
 }
 

@@ -12,7 +12,11 @@
 // forms) fit the server's budget; jobs start in arrival order.  A job whose
 // estimate exceeds the whole budget fails with MGW_ERR_MEMORY instead of
 // waiting forever.  After each job a worker frees its device buffers, so the
-// reservation is the job's footprint only.
+// reservation is the job's footprint only.  The ERI storage of a job
+// (mgw_hip_server_set_eri_storage) enters its estimate: a streamed job
+// reserves the held (pk|ia) block and one upload slab instead of the nmo^4
+// tensor, and under MGW_ERI_AUTO a job that does not fit resident starts
+// streamed when that fits (the degrade path).
 //
 // Mixed-precision jobs run under the context's error judge: a sample of their
 // frequencies is recomputed in FP64, and a failed judge or a factorization
@@ -49,7 +53,10 @@ struct ServeJob {
     const double* factor = nullptr;
     int32_t kdim = 0;
     mgw_g0w0_out out{};
-    size_t estimate = 0;
+    size_t estimate = 0;      // reserved while running (of the storage taken)
+    size_t est_resident = 0, est_streamed = 0;
+    int32_t storage_req = MGW_ERI_RESIDENT;
+    int32_t storage = MGW_ERI_AUTO;     // taken at start
     int status = -1;          // -1 pending, -2 running, else final mgw_status
     int32_t cancel = 0;       // polled by the running job between frequency batches
     double judge_err = -1.0;
@@ -81,6 +88,7 @@ struct mgw_hip_server {
     int32_t judge_fallback = 1;
     double budget_scale = 1.0;        // fault-injection hook: per-job device budget scale
     int32_t group_max = 16;           // jobs one worker runs together (1: no grouping)
+    int32_t eri_storage = MGW_ERI_RESIDENT;   // storage of jobs submitted from now on
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServeJob*> pending;
@@ -114,6 +122,30 @@ struct mgw_hip_server {
                padded_order(a) == padded_order(b);
     }
 
+    // The storage a job would start in now and the bytes it would reserve
+    // (lock held): its requested storage, or under MGW_ERI_AUTO resident when
+    // that fits the budget left, else streamed.
+    size_t start_bytes(const ServeJob* j, int32_t* storage) const {
+        int32_t st = j->storage_req;
+        if (st == MGW_ERI_AUTO)
+            st = reserved + j->est_resident <= budget ? MGW_ERI_RESIDENT : MGW_ERI_STREAMED;
+        if (storage) *storage = st;
+        return st == MGW_ERI_RESIDENT ? j->est_resident : j->est_streamed;
+    }
+    // the smallest reservation the job can ever start with
+    static size_t least_bytes(const ServeJob* j) {
+        return j->storage_req == MGW_ERI_RESIDENT ? j->est_resident : j->est_streamed;
+    }
+    bool can_start(const ServeJob* j) const {
+        return reserved + start_bytes(j, nullptr) <= budget;
+    }
+    void start(ServeJob* j) {
+        int32_t st = MGW_ERI_RESIDENT;
+        j->estimate = start_bytes(j, &st);
+        j->storage = st;
+        reserved += j->estimate;
+    }
+
     void work() {
         // pool[0] leads a group (its stream and batch buffers run the shared
         // batches); pool[k] holds the k-th grouped job's resident ERI
@@ -130,7 +162,7 @@ struct mgw_hip_server {
                     if (stop) return true;
                     if (pending.empty()) return false;
                     ServeJob* h = pending.front();
-                    return h->cancel || h->estimate > budget || reserved + h->estimate <= budget;
+                    return h->cancel || least_bytes(h) > budget || can_start(h);
                 });
                 if (stop) {
                     while (!pending.empty()) {
@@ -143,7 +175,7 @@ struct mgw_hip_server {
                 }
                 ServeJob* j = pending.front();
                 pending.pop_front();
-                if (j->cancel || j->estimate > budget || rc != MGW_OK) {
+                if (j->cancel || least_bytes(j) > budget || rc != MGW_OK) {
                     if (j->cancel)
                         finish_unstarted(j, MGW_ERR_STATE, "cancelled before the job started");
                     else if (rc != MGW_OK)
@@ -155,7 +187,7 @@ struct mgw_hip_server {
                     continue;
                 }
                 group.push_back(j);
-                reserved += j->estimate;
+                start(j);
                 // every other admitted job that shares the head job's batches
                 // joins it (continuous batching: later arrivals that fit the
                 // budget start with it)
@@ -163,8 +195,8 @@ struct mgw_hip_server {
                     for (auto it = pending.begin();
                          it != pending.end() && (int)group.size() < group_max;) {
                         ServeJob* c = *it;
-                        if (!c->cancel && same_group(j, c) && reserved + c->estimate <= budget) {
-                            reserved += c->estimate;
+                        if (!c->cancel && same_group(j, c) && can_start(c)) {
+                            start(c);
                             group.push_back(c);
                             it = pending.erase(it);
                         } else {
@@ -219,6 +251,7 @@ struct mgw_hip_server {
                 if (r == MGW_OK) r = mgw_hip_set_batch_limit(ctx, i == 0 ? sum_nb : batch_freqs);
                 if (r == MGW_OK) r = mgw_hip_set_judge(ctx, jn, jt, jf);
                 if (r == MGW_OK) r = mgw_hip_set_cancel_flag(ctx, &j->cancel);
+                if (r == MGW_OK) r = mgw_hip_set_eri_storage(ctx, j->storage);
                 if (r == MGW_OK) {
                     r = j->factor ? mgw_hip_upload_eri_factor(ctx, j->factor, j->job.nmo, j->job.nocc, j->kdim)
                                   : mgw_hip_upload_eri(ctx, j->eri, j->job.nmo, j->job.nocc);
@@ -328,6 +361,23 @@ int mgw_hip_server_set_budget_scale(mgw_hip_server* s, double scale) {
     return MGW_OK;
 }
 
+int mgw_hip_server_set_eri_storage(mgw_hip_server* s, int32_t storage) {
+    if (!s || (storage != MGW_ERI_AUTO && storage != MGW_ERI_RESIDENT && storage != MGW_ERI_STREAMED))
+        return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->eri_storage = storage;
+    return MGW_OK;
+}
+
+int mgw_hip_server_job_eri_storage(mgw_hip_server* s, int64_t ticket, int32_t* storage) {
+    if (!s || !storage) return MGW_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    ServeJob* j = s->find(ticket);
+    if (!j) return MGW_ERR_ARG;
+    *storage = j->storage;
+    return MGW_OK;
+}
+
 int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw_g0w0_out* out,
                           int64_t* ticket) {
     if (!s || !job || !out || !ticket || !job->job.mo_energy || !job->job.vxc_diag || !out->qp)
@@ -354,15 +404,21 @@ int mgw_hip_server_submit(mgw_hip_server* s, const mgw_serve_job* job, const mgw
     j->kdim = job->kdim;
     j->out = *out;
     const int nb = std::min(g.nfreq, s->batch_freqs);
-    j->estimate = mgw_hip_estimate_bytes(g.nmo, g.nocc, g.nfreq, g.nstates, nb, g.precision);
+    size_t est = mgw_hip_estimate_bytes(g.nmo, g.nocc, g.nfreq, g.nstates, nb, g.precision);
     if (job->eri_factor) {
+        // the factor on the device during the upload (64 more rows when streamed)
         const size_t n2 = (size_t)g.nmo * g.nmo;
-        j->estimate += round_up(n2, 64) * round_up((size_t)job->kdim, 16) * 8;
+        est += (round_up(n2, 64) + 64) * round_up((size_t)job->kdim, 16) * 8;
     }
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->judge_n > 0 && (g.precision == MGW_PREC_MIXED || g.precision == MGW_PREC_MIXED_INT8 ||
                            g.precision == MGW_PREC_MIXED_INT8_4))
-        j->estimate += judge_bytes(g);
+        est += judge_bytes(g);
+    // the decomposed form factors the whole ERI pair matrix: resident only
+    j->storage_req = g.precision == MGW_PREC_FP64_CD ? MGW_ERI_RESIDENT : s->eri_storage;
+    j->est_resident = est;
+    j->est_streamed = est - mgw_hip_eri_bytes(g.nmo, g.nocc, MGW_ERI_RESIDENT) +
+                      mgw_hip_eri_bytes(g.nmo, g.nocc, MGW_ERI_STREAMED);
     j->t_submit = Clock::now();
     j->ticket = s->next_ticket++;
     *ticket = j->ticket;

@@ -6,8 +6,10 @@ so tests read like the reference's own:
 * ``HipScreeningWorkspace(v_ph)`` -- DeviceScreeningWorkspace
   (include/workspace/device_screening_workspace.hpp:27-59): ``compute_w_c(pi0)``,
   ``quadratic_forms_panel(x_panel)``, ``nph``, ``estimated_device_bytes``.
-* ``HipPqPhPanelView(eri, nocc)`` -- DevicePqPhPanelView with a resident ERI
-  (include/workspace/device_pq_ph_panel.hpp:67-99), whose panels feed
+* ``HipPqPhPanelView(eri, nocc, ctx, mode)`` -- DevicePqPhPanelView
+  (include/workspace/device_pq_ph_panel.hpp:67-99) with its storage mode
+  (``ERI_AUTO`` / ``ERI_RESIDENT`` / ``ERI_STREAMED``, the reference's Auto /
+  FullEriResident / StreamingPanel), whose panels feed
   ``quadratic_forms(p, k0, width)``.
 * ``run_g0w0(inp, ...)`` -- run_g0w0 (src/gw/gw.cpp:125-191) with the hoisted,
   frequency-batched device path.
@@ -30,6 +32,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libm
 MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_MEMORY, \
     MGW_ERR_STATE, MGW_ERR_IO = range(8)
 PREC_FP64, PREC_MIXED, PREC_MIXED_INT8, PREC_FP64_CD, PREC_MIXED_INT8_4 = 0, 1, 2, 3, 4
+ERI_AUTO, ERI_RESIDENT, ERI_STREAMED = 0, 1, 2
+ERI_STORAGE_NAMES = {ERI_AUTO: "auto", ERI_RESIDENT: "full-eri-resident", ERI_STREAMED: "streamed"}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -78,6 +82,9 @@ def _load():
         "mgw_hip_upload_eri": (C.c_int, [vp, _dp, C.c_int32, C.c_int32]),
         "mgw_hip_upload_eri_factor": (C.c_int, [vp, _dp, C.c_int32, C.c_int32, C.c_int32]),
         "mgw_hip_download_eri": (C.c_int, [vp, _dp]),
+        "mgw_hip_set_eri_storage": (C.c_int, [vp, C.c_int32]),
+        "mgw_hip_eri_storage": (C.c_int, [vp, _ip, C.POINTER(C.c_size_t)]),
+        "mgw_hip_eri_bytes": (C.c_size_t, [C.c_int32] * 3),
         "mgw_hip_release_buffers": (C.c_int, [vp]),
         "mgw_hip_build_vph_from_eri": (C.c_int, [vp]),
         "mgw_hip_quadratic_forms": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp]),
@@ -114,6 +121,8 @@ def _load():
         "mgw_hip_server_set_judge": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32]),
         "mgw_hip_server_set_budget_scale": (C.c_int, [vp, C.c_double]),
         "mgw_hip_server_set_grouping": (C.c_int, [vp, C.c_int32]),
+        "mgw_hip_server_set_eri_storage": (C.c_int, [vp, C.c_int32]),
+        "mgw_hip_server_job_eri_storage": (C.c_int, [vp, C.c_int64, _ip]),
         "mgw_hip_server_cancel": (C.c_int, [vp, C.c_int64]),
         "mgw_hip_server_job_judge": (C.c_int, [vp, C.c_int64, _dp, _ip]),
         "mgw_hip_server_release": (C.c_int, [vp, C.c_int64]),
@@ -156,6 +165,12 @@ def transformed_legendre_grid(nfreq: int):
 def estimate_bytes(nmo: int, nocc: int, nfreq: int, nstates: int = 0, nbatch: int = 0,
                    precision: int = PREC_FP64) -> int:
     return int(lib.mgw_hip_estimate_bytes(nmo, nocc, nfreq, nstates, nbatch, precision))
+
+
+def eri_bytes(nmo: int, nocc: int, storage: int) -> int:
+    """Device bytes an ERI of this size holds in a storage (ERI_RESIDENT or
+    ERI_STREAMED, the latter with its upload slab)."""
+    return lib.mgw_hip_eri_bytes(nmo, nocc, storage)
 
 
 def device_count() -> int:
@@ -207,6 +222,16 @@ class HipContext:
 
     def set_memory_budget(self, nbytes: int):
         self.call("mgw_hip_set_memory_budget", nbytes)
+
+    def set_eri_storage(self, storage: int):
+        """Storage of the ERI for the uploads that follow (ERI_AUTO by default)."""
+        self.call("mgw_hip_set_eri_storage", int(storage))
+
+    def eri_storage(self) -> tuple:
+        """(storage the uploaded ERI took, device bytes it holds)."""
+        st, nb = C.c_int32(0), C.c_size_t(0)
+        self.call("mgw_hip_eri_storage", C.byref(st), C.byref(nb))
+        return st.value, nb.value
 
     def upload_eri(self, eri: np.ndarray, nocc: int):
         e = _f64(eri)
@@ -357,17 +382,29 @@ class HipDevicePanel:
 
 
 class HipPqPhPanelView:
-    """DevicePqPhPanelView (full-ERI-resident mode) sharing a workspace context."""
+    """DevicePqPhPanelView sharing a workspace context; `mode` as the
+    reference's DevicePqPhPanelStorageMode (ERI_AUTO, ERI_RESIDENT,
+    ERI_STREAMED)."""
 
-    def __init__(self, eri: np.ndarray, nocc: int, ctx: HipContext):
+    def __init__(self, eri: np.ndarray, nocc: int, ctx: HipContext, mode: int = ERI_AUTO):
         self.ctx = ctx
         self.nmo = eri.shape[0]
         self.nocc = nocc
+        ctx.set_eri_storage(mode)
         ctx.upload_eri(eri, nocc)
 
     @property
     def nph(self) -> int:
         return self.nocc * (self.nmo - self.nocc)
+
+    def storage_mode(self) -> int:
+        return self.ctx.eri_storage()[0]
+
+    def storage_mode_name(self) -> str:
+        return ERI_STORAGE_NAMES[self.storage_mode()]
+
+    def estimated_device_bytes(self) -> int:
+        return self.ctx.eri_storage()[1]
 
     def build_v_ph(self):
         self.ctx.call("mgw_hip_build_vph_from_eri")
@@ -563,6 +600,12 @@ class HipServer:
         """Most jobs a worker runs together through mgw_hip_g0w0_group (1: off)."""
         _check("mgw_hip_server_set_grouping", lib.mgw_hip_server_set_grouping(self.h, int(max_jobs)))
 
+    def set_eri_storage(self, storage: int):
+        """ERI storage of jobs submitted from now on: ERI_RESIDENT (default),
+        ERI_STREAMED, or ERI_AUTO (resident when it fits the budget left, else
+        streamed)."""
+        _check("mgw_hip_server_set_eri_storage", lib.mgw_hip_server_set_eri_storage(self.h, int(storage)))
+
     def set_budget_scale(self, scale: float):
         """Fault injection: per-job device budget = scale x estimate."""
         _check("mgw_hip_server_set_budget_scale", lib.mgw_hip_server_set_budget_scale(self.h, scale))
@@ -572,7 +615,8 @@ class HipServer:
 
     def wait(self, ticket: int, timeout: float = -1.0):
         """Returns (HipG0W0Result, latency_ms, run_ms); raises HipError on failure.
-        The result's timings hold the judge outcome ("judge_err", "fell_back").
+        The result's timings hold the judge outcome ("judge_err", "fell_back") and
+        the storage the job's ERI took ("eri_storage").
         The job's record is released once it has been reported."""
         st, lat, run = C.c_int32(0), C.c_double(0), C.c_double(0)
         _check("mgw_hip_server_wait", lib.mgw_hip_server_wait(self.h, ticket, timeout, C.byref(st),
@@ -580,8 +624,9 @@ class HipServer:
         if st.value == -1:
             raise TimeoutError(f"job {ticket} not finished")
         res, _, _, stt, nmo = self._keep.pop(ticket)
-        je, fb = C.c_double(-1.0), C.c_int32(0)
+        je, fb, es = C.c_double(-1.0), C.c_int32(0), C.c_int32(0)
         lib.mgw_hip_server_job_judge(self.h, ticket, C.byref(je), C.byref(fb))
+        lib.mgw_hip_server_job_eri_storage(self.h, ticket, C.byref(es))
         msg = lib.mgw_hip_server_job_error(self.h, ticket)
         lib.mgw_hip_server_release(self.h, ticket)
         if st.value != MGW_OK:
@@ -589,7 +634,8 @@ class HipServer:
         out = HipG0W0Result(res["omega"], res["weight"],
                             list(range(nmo)) if stt is None else stt.tolist(), res["sx"],
                             res["sre"] + 1j * res["sim"], res["qp"], res["it"],
-                            {"judge_err": je.value, "fell_back": fb.value})
+                            {"judge_err": je.value, "fell_back": fb.value,
+                             "eri_storage": es.value})
         return out, lat.value, run.value
 
     def stats(self) -> dict:

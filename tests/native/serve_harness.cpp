@@ -28,6 +28,7 @@ struct FakeCtx {
     int32_t batch_limit = 0, judge_n = 0, judge_fallback = 0;
     double judge_tol = 0.0;
     const int32_t* cancel = nullptr;
+    int32_t storage = MGW_ERI_AUTO;
     int nmo = 0, nocc = 0;
     std::string error;
     double judge_err = -1.0;
@@ -100,9 +101,27 @@ int mgw_hip_set_cancel_flag(mgw_hip_ctx* ctx, const int32_t* flag) {
     return MGW_OK;
 }
 
+int mgw_hip_set_eri_storage(mgw_hip_ctx* ctx, int32_t storage) {
+    if (storage != MGW_ERI_AUTO && storage != MGW_ERI_RESIDENT && storage != MGW_ERI_STREAMED)
+        return MGW_ERR_ARG;
+    reinterpret_cast<FakeCtx*>(ctx)->storage = storage;
+    return MGW_OK;
+}
+
+// the held ph block plus one slab row when streamed
+size_t mgw_hip_eri_bytes(int32_t nmo, int32_t nocc, int32_t storage) {
+    if (nmo <= 1 || nocc <= 0 || nocc >= nmo) return 0;
+    if (storage == MGW_ERI_RESIDENT) return eri_bytes(nmo);
+    if (storage == MGW_ERI_STREAMED)
+        return ((size_t)nmo * nmo * nocc * (nmo - nocc) + (size_t)nmo * nmo * nmo) * 8;
+    return 0;
+}
+
 static int upload(mgw_hip_ctx* ctx, int32_t nmo, int32_t nocc) {
     auto* c = reinterpret_cast<FakeCtx*>(ctx);
-    if (c->budget && eri_bytes(nmo) > c->budget)
+    if (c->storage != MGW_ERI_RESIDENT && c->storage != MGW_ERI_STREAMED)
+        return ctx_fail(c, MGW_ERR_ARG, "the server must set the job's ERI storage");
+    if (c->budget && mgw_hip_eri_bytes(nmo, nocc, c->storage) > c->budget)
         return ctx_fail(c, MGW_ERR_MEMORY, "ERI: allocation would exceed the device budget");
     c->nmo = nmo;
     c->nocc = nocc;
@@ -336,6 +355,46 @@ void scenario_admission_and_faults() {
     CHECK(mgw_hip_server_destroy(s) == MGW_OK);
 }
 
+// ERI storage: the default keeps every job resident; under MGW_ERI_AUTO a
+// job that does not fit resident beside a running one starts streamed; a job
+// too large even streamed fails; a streamed server streams every job.
+void scenario_eri_storage() {
+    mgw_hip_server* s = nullptr;
+    CHECK(mgw_hip_server_create(0, 3, 2000000, 8, &s) == MGW_OK);
+    CHECK(mgw_hip_server_set_eri_storage(s, 7) == MGW_ERR_ARG);
+    Job first = make_job(20, 24);
+    CHECK(submit(s, first) == MGW_OK);
+    CHECK(wait_status(s, first) == MGW_OK);
+    int32_t st = -1;
+    CHECK(mgw_hip_server_job_eri_storage(s, first.ticket, &st) == MGW_OK && st == MGW_ERI_RESIDENT);
+    CHECK(mgw_hip_server_set_eri_storage(s, MGW_ERI_AUTO) == MGW_OK);
+    g_peak = 0;
+    std::vector<Job> jobs;
+    for (int i = 0; i < 6; ++i) jobs.push_back(make_job(20, 400));     // 30 ms each
+    for (auto& j : jobs) CHECK(submit(s, j) == MGW_OK);
+    int resident = 0, streamed = 0;
+    for (auto& j : jobs) {
+        CHECK(wait_status(s, j) == MGW_OK);
+        check_outputs(j);
+        CHECK(mgw_hip_server_job_eri_storage(s, j.ticket, &st) == MGW_OK);
+        resident += st == MGW_ERI_RESIDENT;
+        streamed += st == MGW_ERI_STREAMED;
+    }
+    // one resident job (1.4 MB) leaves room for one streamed job (0.4 MB) only
+    CHECK(resident + streamed == 6 && resident >= 1 && streamed >= 1 && g_peak.load() == 2);
+    Job big = make_job(40, 24);                      // 3.8 MB even streamed
+    CHECK(submit(s, big) == MGW_OK);
+    CHECK(wait_status(s, big) == MGW_ERR_MEMORY);
+    CHECK(mgw_hip_server_set_eri_storage(s, MGW_ERI_STREAMED) == MGW_OK);
+    Job one = make_job(20, 24);
+    CHECK(submit(s, one) == MGW_OK);
+    CHECK(wait_status(s, one) == MGW_OK);
+    CHECK(mgw_hip_server_job_eri_storage(s, one.ticket, &st) == MGW_OK && st == MGW_ERI_STREAMED);
+    size_t reserved = 1;
+    CHECK(mgw_hip_server_stats(s, nullptr, nullptr, nullptr, &reserved) == MGW_OK && reserved == 0);
+    CHECK(mgw_hip_server_destroy(s) == MGW_OK);
+}
+
 void scenario_cancel_and_fallback() {
     mgw_hip_server* s = nullptr;
     CHECK(mgw_hip_server_create(0, 1, (size_t)1 << 40, 4, &s) == MGW_OK);
@@ -438,6 +497,7 @@ void scenario_arguments() {
 int main() {
     scenario_mix();
     scenario_admission_and_faults();
+    scenario_eri_storage();
     scenario_cancel_and_fallback();
     scenario_grouping();
     scenario_destroy_with_queue();

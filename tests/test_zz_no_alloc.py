@@ -1,7 +1,8 @@
 """mgw_hip_quadratic_forms_noalloc (needs an MI355X): the allocation-free
 variant of mgw_hip_quadratic_forms, through the C ABI, against the existing
-entry point on two calls in a row (the second with a smaller panel), its
-limits, and the device bytes of the context unchanged by it."""
+entry point on two calls in a row (the second with a smaller panel), with the
+ERI resident and streamed, its limits, and the device bytes of the context
+unchanged by it."""
 import ctypes as C
 
 import numpy as np
@@ -54,13 +55,14 @@ def _noalloc(hip, ctx, p, k0, width, rt):
     return host[:width]
 
 
-def test_noalloc_quadratic_forms_match_the_allocating_entry_point(hip, hiprt):
+@pytest.mark.parametrize("storage", ["ERI_RESIDENT", "ERI_STREAMED"])
+def test_noalloc_quadratic_forms_match_the_allocating_entry_point(hip, hiprt, storage):
     for nmo, nocc, seed in [(24, 5, 20240601), (40, 9, 3)]:
-        say(f"case: no-alloc quadratic forms, nmo={nmo} nocc={nocc}")
+        say(f"case: no-alloc quadratic forms, nmo={nmo} nocc={nocc}, {storage}")
         inp = synthetic_input(nmo, nocc, seed)
         ph = orc.particle_hole_basis(inp.mo_energy, inp.nocc)
         with hip.HipContext(0) as ctx:
-            view = hip.HipPqPhPanelView(inp.eri, inp.nocc, ctx)
+            view = hip.HipPqPhPanelView(inp.eri, inp.nocc, ctx, getattr(hip, storage))
             view.build_v_ph()
             ws = hip.HipScreeningWorkspace.__new__(hip.HipScreeningWorkspace)
             ws.ctx, ws._n = ctx, ph.size
