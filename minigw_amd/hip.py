@@ -33,7 +33,7 @@ MGW_OK, MGW_ERR_ARG, MGW_ERR_HIP, MGW_ERR_NUMERIC, MGW_ERR_UNSUPPORTED, MGW_ERR_
     MGW_ERR_STATE, MGW_ERR_IO = range(8)
 PREC_FP64, PREC_MIXED, PREC_MIXED_INT8, PREC_FP64_CD, PREC_MIXED_INT8_4 = 0, 1, 2, 3, 4
 ERI_AUTO, ERI_RESIDENT, ERI_STREAMED = 0, 1, 2
-ERI_STORAGE_NAMES = {ERI_AUTO: "auto", ERI_RESIDENT: "full-eri-resident", ERI_STREAMED: "streamed"}
+ERI_STORAGE_NAMES = {ERI_AUTO: "auto", ERI_RESIDENT: "full-eri-resident", ERI_STREAMED: "streaming-panel"}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
