@@ -183,6 +183,29 @@ def cpu_baseline(config: str, inp, nfreq: int, sample_freqs: int) -> dict:
             "port": port}
 
 
+def storage_check(hip, inp, nfreq: int, npar: int, prec: int, res, resident_upload_s: float) -> dict:
+    """The headline molecule with its ERI in the streamed storage (the ERI
+    crosses to the device once in slabs of rows p; only the (pk|ia) block and
+    (pi|pi) stay): bytes held, upload time, one molecule's time after a
+    warm-up, and its result against the resident storage's, bit for bit."""
+    with hip.HipContext(0) as c:
+        c.set_eri_storage(hip.ERI_STREAMED)
+        t0 = time.perf_counter()
+        c.upload_eri(inp.eri, inp.nocc)
+        upload_s = time.perf_counter() - t0
+        storage, held = c.eri_storage()
+        hip.run_g0w0(inp, nfreq, npar, precision=prec, ctx=c, upload=False)
+        t1 = time.perf_counter()
+        r = hip.run_g0w0(inp, nfreq, npar, precision=prec, ctx=c, upload=False)
+        ms = (time.perf_counter() - t1) * 1e3
+    same = bool(np.array_equal(r.sigma_c, res.sigma_c) and np.array_equal(r.qp, res.qp, equal_nan=True)
+                and np.array_equal(r.qp_iterations, res.qp_iterations))
+    return {"storage": hip.ERI_STORAGE_NAMES[storage], "resident_eri_bytes": int(inp.eri.nbytes),
+            "streamed_eri_bytes": int(held),
+            "upload_s": {"resident": round(resident_upload_s, 3), "streamed": round(upload_s, 3)},
+            "ms_per_molecule_streamed": round(ms, 2), "bitwise_equal_to_resident": same}
+
+
 def _factor_job(nmo: int, nocc: int, seed: int):
     from minigw_amd.synthetic import synthetic_factor
     mo, fermi, b, vxc = synthetic_factor(nmo, nocc, seed)
@@ -333,6 +356,8 @@ def main() -> int:
                     help="also run the batch and serving workloads in the decomposed-ERI form")
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-storage-check", action="store_true",
+                    help="skip the headline molecule in the streamed ERI storage")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -352,8 +377,10 @@ def main() -> int:
 
     with hip.HipContext(0) as ctx:
         t0 = time.perf_counter()
+        ctx.set_eri_storage(hip.ERI_RESIDENT)
         ctx.upload_eri(inp.eri, inp.nocc)
-        log(f"bench: ERI resident ({inp.eri.nbytes / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s")
+        resident_upload_s = time.perf_counter() - t0
+        log(f"bench: ERI resident ({inp.eri.nbytes / 1e9:.2f} GB) in {resident_upload_s:.1f} s")
 
         peaks = ctx.measure_peaks()
         log("bench: measured peaks " + ", ".join(f"{k} {v / 1e12:.1f}T" for k, v in peaks.items()))
@@ -451,6 +478,10 @@ def main() -> int:
                 "phases_ms": {k: round(v, 3) for k, v in cd_phase.items()},
                 "roofline": roofline("fp64_cd", cd_phase, n, m, args.nfreq, args.config, peaks)}
         out.update(report)
+    if not args.no_storage_check:
+        log("bench: streamed ERI storage ...")
+        out["eri_storage"] = storage_check(hip, inp, args.nfreq, args.npar, prec, res,
+                                           resident_upload_s)
     cd_too = prec != 3 and args.decomposed_workloads  # the same workloads in the decomposed form
     if args.batch_molecules > 0:
         log("bench: uneven-nmo batch ...")
