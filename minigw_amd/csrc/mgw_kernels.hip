@@ -476,8 +476,14 @@ __device__ inline double cabs_(cplx a) { return hypot(a.re, a.im); }
 
 // g[(n * nfreq + f) * nmo + k] = d / (d^2 + w_f^2), d = (fermi - e_k) + i w_n: the
 // Green's-function factor of the quadrature, which does not depend on the state.
+// Every operation rounds as the host's d / (d * d - (i w_f) * (i w_f)) does
+// (sigma_c_driver.cpp:154-160): Re(d^2) + w_f^2 cancels almost exactly at
+// w_f ~ w_n >> |e_k - fermi|, and an FMA there (one rounding instead of two)
+// moved Sigma_c by 7e-10 of its largest value on a molecule with a 1e-4 Ha
+// HOMO-LUMO gap.
 __global__ void k_sigma_g(const double* __restrict__ mo, const double* __restrict__ omega,
                           double fermi, int nmo, int nfreq, double2* __restrict__ g) {
+#pragma clang fp contract(off)
     const size_t total = (size_t)nfreq * nfreq * nmo;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (size_t)gridDim.x * blockDim.x) {
@@ -503,6 +509,7 @@ __global__ __launch_bounds__(128) void k_sigma_c(const double* __restrict__ q,
                                                  const double* __restrict__ weight, int nmo,
                                                  int nstates, int nfreq, double* __restrict__ sre,
                                                  double* __restrict__ sim) {
+#pragma clang fp contract(off)       // the host's complex products round each term
     __shared__ double gre[kSgN][kSgK + 1], gim[kSgN][kSgK + 1];
     __shared__ double qs[kSgS][kSgK + 1], qis[kSgS][kSgK + 1];
     const int t = threadIdx.x, tn = t & (kSgN - 1), ts = t / kSgN;

@@ -140,6 +140,17 @@ def synthetic_input(nmo: int, nocc: int, seed: int) -> GwInput:
                    nocc=nocc, fermi_energy=fermi)
 
 
+def with_gap(inp: GwInput, gap: float) -> GwInput:
+    """The same molecule with its HOMO-LUMO gap set to `gap` Ha: every
+    virtual level moves down by the same amount, the Fermi level to mid-gap.
+    A small gap makes the dielectric ill-conditioned at small frequencies
+    (S^2 = 4 D / (w^2 + D^2) reaches 4 / gap)."""
+    mo = np.array(inp.mo_energy, dtype=np.float64)
+    mo[inp.nocc:] -= (mo[inp.nocc] - mo[inp.nocc - 1]) - gap
+    return GwInput(mo_energy=mo, eri=inp.eri, vxc=inp.vxc, nocc=inp.nocc,
+                   fermi_energy=0.5 * (mo[inp.nocc - 1] + mo[inp.nocc]))
+
+
 # Named workloads (nmo, nocc, seed).  Seeds are fixed here.
 CONFIGS = {
     "nmo24": (24, 5, 20240601),

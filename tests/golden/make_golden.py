@@ -13,8 +13,9 @@ its HDF5 tools; the GPU runs and the test-suite read the .npz files it writes.
    plus E_c_before_Pade.out of the other committed backends (spread estimate).
 3. ``ref_*.npz``: outputs of the reference's own path (oracle/_ref/gw_ref,
    built by oracle/Makefile from the reference sources, ReferenceBackend) for
-   all states of H2O, of the seeded synthetic molecule nmo=24/nocc=5 and of
-   small synthetic cases at eta = 0 and eta != 0.
+   all states of H2O, of the seeded synthetic molecule nmo=24/nocc=5, of
+   small synthetic cases at eta = 0 and eta != 0, and of an ill-conditioned
+   one (the nmo=30 molecule with its HOMO-LUMO gap set to 1e-4 Ha).
 4. ``ref_*_blas.npz``: the same cases through the reference's BLAS/LAPACK
    backend (oracle/_ref/gw_ref_blas: BlasLapackBackend, zgetrf/zgetrs/zgemm
    from MKL), selected by the reference's own backend factory.  The spread
@@ -39,7 +40,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
-from minigw_amd.synthetic import GwInput, synthetic_input  # noqa: E402
+from minigw_amd.synthetic import GwInput, synthetic_input, with_gap  # noqa: E402
 
 REF = "/root/reference"
 H5DUMP = "/opt/conda/bin/h5dump"
@@ -169,11 +170,13 @@ CASES = {
     "syn_small": (lambda: synthetic_input(10, 3, 7), 48, 16, -1, 0.0, True),
     "syn_small_eta": (lambda: synthetic_input(10, 3, 7), 48, 12, -1, 0.05, True),
     "syn_odd": (lambda: synthetic_input(13, 4, 11), 32, 8, -1, 0.0, True),
+    # ill-conditioned: the nmo=30 molecule with its HOMO-LUMO gap set to 1e-4 Ha
+    "syn_gap": (lambda: with_gap(synthetic_input(30, 6, 47), 1e-4), 64, 16, -1, 0.0, False),
 }
 
 
 # cases also run through the BLAS/LAPACK backend (ref_<name>_blas.npz)
-BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta", "syn_small_eta")
+BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta", "syn_small_eta", "syn_gap")
 
 
 def main() -> None:

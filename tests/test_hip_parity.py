@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from minigw_amd.synthetic import GwInput, synthetic_input, CONFIGS
+from minigw_amd.synthetic import GwInput, synthetic_input, with_gap, CONFIGS
 from oracle import oracle as orc
 from qp_bounds import SIGMA_REL, fp64_qp_violations, mixed_qp_violations
 
@@ -43,6 +43,8 @@ def golden_input(name, r):
         return h2o_input()
     if name == "syn_nmo24":
         return synthetic_input(*CONFIGS["nmo24"])
+    if name == "syn_gap":
+        return with_gap(synthetic_input(30, 6, 47), 1e-4)
     return GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
                    fermi_energy=float(r["fermi_energy"]))
 
@@ -206,13 +208,16 @@ def test_h2o_state5_fixture(hip, ctx):
 
 
 @pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta",
-                                  "syn_small_eta"])
+                                  "syn_small_eta", "syn_gap"])
 def test_all_states_vs_reference_path(hip, ctx, name):
     """FP64 device path against the reference's own outputs: Sigma_c(i w) to
     2e-14 of max |Sigma_c| from both reference backends; QP energies within
     1e-8 Ha of both, or, on states the reference's own backends show to be
     ill-conditioned, within the bound their amplification gives at that
-    Sigma_c precision (tests/qp_bounds.py)."""
+    Sigma_c precision (tests/qp_bounds.py).  syn_gap is the ill-conditioned
+    case: the nmo=30 molecule with a 1e-4 Ha HOMO-LUMO gap, where the
+    quadrature's d^2 + w'^2 cancels to a few digits and only the host's
+    rounding of every step reproduces the reference."""
     r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
     rb = dict(np.load(os.path.join(GOLD, f"ref_{name}_blas.npz")))
     say(f"case: all states vs reference path, {name}")

@@ -326,7 +326,7 @@ def test_oracle_vs_reference_h2o_eta():
 
 
 @pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta",
-                                  "syn_small_eta"])
+                                  "syn_small_eta", "syn_gap"])
 def test_reference_backends_agree_on_sigma_c(name):
     """The reference's two backends (Gauss-Jordan vs MKL getrf/getrs) agree on
     Sigma_c(i w) to round-off: the QP spread between them comes from the Pade /
@@ -343,6 +343,9 @@ def test_reference_backends_agree_on_sigma_c(name):
     elif name.startswith("h2o"):
         h = h2o_input()
         mo, vxc = h.mo_energy, np.diag(h.vxc)
+    elif name == "syn_gap":
+        h = gap_input()
+        mo, vxc = h.mo_energy, np.diag(h.vxc)
     else:
         from minigw_amd.synthetic import CONFIGS
         h = synthetic_input(*CONFIGS["nmo24"])
@@ -358,3 +361,20 @@ def test_oracle_vs_reference_synthetic_nmo24():
     inp = synthetic_input(*CONFIGS["nmo24"])
     assert int(r["nmo"]) == inp.nmo and float(r["fermi_energy"]) == inp.fermi_energy
     _check_chain(inp, "syn_nmo24", 1e-11)
+
+
+def gap_input():
+    """The ill-conditioned fixture case: nmo=30 synthetic molecule, HOMO-LUMO gap 1e-4 Ha."""
+    from minigw_amd.synthetic import with_gap
+    return with_gap(synthetic_input(30, 6, 47), 1e-4)
+
+
+def test_oracle_vs_reference_small_gap():
+    """The ill-conditioned case (max |Sigma_c| 2472 Ha, the quadrature's
+    d^2 + w'^2 cancelling to a few digits): the oracle still follows the
+    reference to round-off."""
+    r = ref_case("syn_gap")
+    inp = gap_input()
+    assert int(r["nmo"]) == inp.nmo and float(r["fermi_energy"]) == inp.fermi_energy
+    res = _check_chain(inp, "syn_gap", 1e-11)
+    assert np.abs(res.sigma_c - r["sigma_c"]).max() <= 2e-15 * np.abs(r["sigma_c"]).max()
