@@ -1,5 +1,5 @@
 """Builds the reference's `gw` executable with the hip backend wired in the way
-the integration guide (INTEGRATION.md at the repository root) tells a miniGW
+the integration guide at the repository root (GUIDE below) tells a miniGW
 maintainer to: its ```cpp file=...``` blocks are written into a scratch tree,
 its ```diff``` blocks are applied to copies of the reference files they name,
 and the result is compiled with the reference's own sources (from REF, where

@@ -22,8 +22,7 @@ backend-to-backend spread:
     bound_j = max(tol, 10 * kappa_j * eps * max|Sigma_ref|,
                   10 * max_k spread_k  if spread_j > tol)
 
-FP64 parity rule (north star: QP energies within 1e-8 Ha of the BLAS/LAPACK
-backend): Sigma_c within SIGMA_REL["fp64"] of both reference backends, and
+FP64 parity rule (QP energies within 1e-8 Ha of the BLAS/LAPACK backend): Sigma_c within SIGMA_REL["fp64"] of both reference backends, and
 every QP energy within bound_j (tol = 1e-8 Ha) of both.  Where the reference's
 own backends agree closely, bound_j is 1e-8 Ha itself.  A non-finite QP
 energy of a computed state fails.
