@@ -15,7 +15,8 @@ its HDF5 tools; the GPU runs and the test-suite read the .npz files it writes.
    built by oracle/Makefile from the reference sources, ReferenceBackend) for
    all states of H2O, of the seeded synthetic molecule nmo=24/nocc=5, of
    small synthetic cases at eta = 0 and eta != 0, and of an ill-conditioned
-   one (the nmo=30 molecule with its HOMO-LUMO gap set to 1e-4 Ha).
+   one (the nmo=30 molecule with its HOMO-LUMO gap set to 1e-4 Ha, at eta = 0
+   and eta = 1e-3).
 4. ``ref_*_blas.npz``: the same cases through the reference's BLAS/LAPACK
    backend (oracle/_ref/gw_ref_blas: BlasLapackBackend, zgetrf/zgetrs/zgemm
    from MKL), selected by the reference's own backend factory.  The spread
@@ -172,11 +173,13 @@ CASES = {
     "syn_odd": (lambda: synthetic_input(13, 4, 11), 32, 8, -1, 0.0, True),
     # ill-conditioned: the nmo=30 molecule with its HOMO-LUMO gap set to 1e-4 Ha
     "syn_gap": (lambda: with_gap(synthetic_input(30, 6, 47), 1e-4), 64, 16, -1, 0.0, False),
+    "syn_gap_eta": (lambda: with_gap(synthetic_input(30, 6, 47), 1e-4), 48, 16, -1, 1e-3, False),
 }
 
 
 # cases also run through the BLAS/LAPACK backend (ref_<name>_blas.npz)
-BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta", "syn_small_eta", "syn_gap")
+BLAS_CASES = ("h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta", "syn_small_eta", "syn_gap",
+              "syn_gap_eta")
 
 
 def main() -> None:

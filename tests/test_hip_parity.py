@@ -43,7 +43,7 @@ def golden_input(name, r):
         return h2o_input()
     if name == "syn_nmo24":
         return synthetic_input(*CONFIGS["nmo24"])
-    if name == "syn_gap":
+    if name in ("syn_gap", "syn_gap_eta"):
         return with_gap(synthetic_input(30, 6, 47), 1e-4)
     return GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
                    fermi_energy=float(r["fermi_energy"]))
@@ -208,14 +208,15 @@ def test_h2o_state5_fixture(hip, ctx):
 
 
 @pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta",
-                                  "syn_small_eta", "syn_gap"])
+                                  "syn_small_eta", "syn_gap", "syn_gap_eta"])
 def test_all_states_vs_reference_path(hip, ctx, name):
     """FP64 device path against the reference's own outputs: Sigma_c(i w) to
     2e-14 of max |Sigma_c| from both reference backends; QP energies within
     1e-8 Ha of both, or, on states the reference's own backends show to be
     ill-conditioned, within the bound their amplification gives at that
-    Sigma_c precision (tests/qp_bounds.py).  syn_gap is the ill-conditioned
-    case: the nmo=30 molecule with a 1e-4 Ha HOMO-LUMO gap, where the
+    Sigma_c precision (tests/qp_bounds.py).  syn_gap(_eta) is the
+    ill-conditioned case (eta = 0 and 1e-3): the nmo=30 molecule with a
+    1e-4 Ha HOMO-LUMO gap, where the
     quadrature's d^2 + w'^2 cancels to a few digits and only the host's
     rounding of every step reproduces the reference."""
     r = dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))

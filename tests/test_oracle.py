@@ -326,7 +326,7 @@ def test_oracle_vs_reference_h2o_eta():
 
 
 @pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd", "h2o_eta",
-                                  "syn_small_eta", "syn_gap"])
+                                  "syn_small_eta", "syn_gap", "syn_gap_eta"])
 def test_reference_backends_agree_on_sigma_c(name):
     """The reference's two backends (Gauss-Jordan vs MKL getrf/getrs) agree on
     Sigma_c(i w) to round-off: the QP spread between them comes from the Pade /
@@ -343,7 +343,7 @@ def test_reference_backends_agree_on_sigma_c(name):
     elif name.startswith("h2o"):
         h = h2o_input()
         mo, vxc = h.mo_energy, np.diag(h.vxc)
-    elif name == "syn_gap":
+    elif name.startswith("syn_gap"):
         h = gap_input()
         mo, vxc = h.mo_energy, np.diag(h.vxc)
     else:
@@ -377,4 +377,10 @@ def test_oracle_vs_reference_small_gap():
     inp = gap_input()
     assert int(r["nmo"]) == inp.nmo and float(r["fermi_energy"]) == inp.fermi_energy
     res = _check_chain(inp, "syn_gap", 1e-11)
+    assert np.abs(res.sigma_c - r["sigma_c"]).max() <= 2e-15 * np.abs(r["sigma_c"]).max()
+
+
+def test_oracle_vs_reference_small_gap_eta():
+    r = ref_case("syn_gap_eta")
+    res = _check_chain(gap_input(), "syn_gap_eta", 1e-11)
     assert np.abs(res.sigma_c - r["sigma_c"]).max() <= 2e-15 * np.abs(r["sigma_c"]).max()
