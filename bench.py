@@ -56,9 +56,9 @@ HA_MEV = 27.211386245988 * 1e3
 # launches, from two rocprofv3 --pmc runs of this bench with that form alone;
 # the commands and per-dispatch figures are in profiles/kernels_nmo200_*.md
 # (int8x4, int8, fp64, fp16split, fp64_cd).
-TRAFFIC = {("mixed_int8x4", "nmo200", 200): (793.97e9 + 0.78e9),
+TRAFFIC = {("mixed_int8x4", "nmo200", 200): (794.78e9 + 0.78e9),
            ("mixed_int8", "nmo200", 200): (985.16e9 + 0.78e9),
-           ("fp64", "nmo200", 200): (2844.91e9 + 0.78e9),
+           ("fp64", "nmo200", 200): (2844.54e9 + 0.78e9),
            ("mixed", "nmo200", 200): (266.04e9 + 0.39e9),
            ("fp64_cd", "nmo200", 200): (64.87e9 + 0.16e9)}
 
