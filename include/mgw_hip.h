@@ -242,7 +242,7 @@ int mgw_hip_quadratic_forms_noalloc(mgw_hip_ctx* ctx, int32_t p, int32_t k0, int
 typedef struct mgw_g0w0_job {
     int32_t nmo, nocc;
     int32_t nfreq;              /* transformed Gauss-Legendre points          */
-    int32_t npar;               /* Pade parameters (<= 64)                    */
+    int32_t npar;               /* Pade parameters (>= 1; min(npar, nfreq) used) */
     int32_t nstates;            /* 0 = all states                             */
     const int32_t* states;      /* nstates 0-based indices (ascending), or NULL */
     const double* mo_energy;    /* nmo                                        */

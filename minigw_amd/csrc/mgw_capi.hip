@@ -255,6 +255,7 @@ struct mgw_hip_ctx {
     // factor columns, and B^T / pair vectors / their norms
     Buf cdidx, cdwork, cdl, cdb, cdx, cdn;
     Buf hsg;                          // Sigma_c quadrature factors g(n, f, k)
+    Buf hpade;                        // Pade tables beyond kMaxPade points
     int eri_pair_sym = -1;            // symmetric to 1e-10 for the decomposed form: -1 unchecked
     double eri_asym = 0.0, eri_absmax = 0.0;
     double cd_tol = 1e-12;            // stop when the residual diagonal <= cd_tol * first pivot
@@ -280,7 +281,7 @@ struct mgw_hip_ctx {
                &hq, &homega, &hweight, &hmo, &hsig_re, &hsig_im, &hsx, &hvxc, &hstates, &hqp,
                &hiters, &xhi, &xlo, &xscale, &mhi, &mlo, &mscale, &hqim, &hY, &wY, &wW, &hfr, &hfc,
                &cdidx, &cdwork, &cdl, &cdb, &cdx, &cdn, &hsg, &hqj, &luz, &ludl, &ludu,
-               &lupiv, &wz, &ppk, &panel, &hinfob, &hjom, &hjinfo, &exsd, &eslab};
+               &lupiv, &wz, &ppk, &panel, &hinfob, &hjom, &hjinfo, &exsd, &eslab, &hpade};
     }
     ~mgw_hip_ctx() {
         (void)hipSetDevice(device);
@@ -1373,7 +1374,7 @@ struct MolRun {
     double *cdBt = nullptr, *cdLx = nullptr, *cdnrm = nullptr;
     uint16_t *xh = nullptr, *xl = nullptr;
     double *xs = nullptr, *xp = nullptr, *xj = nullptr;
-    double *sre = nullptr, *sim = nullptr, *sx = nullptr, *qp = nullptr;
+    double *sre = nullptr, *sim = nullptr, *sx = nullptr, *qp = nullptr, *pade = nullptr;
     int* iters = nullptr;
     double2* gq = nullptr;
     int *info = nullptr, *f0p = nullptr;
@@ -1405,7 +1406,7 @@ void mol_prepare(MolRun& r, mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_
             fail(MGW_ERR_STATE, "mgw_hip_g0w0: resident ERI has nmo=%d nocc=%d, job nmo=%d nocc=%d",
                  ctx->nmo, ctx->nocc, nmo, nocc);
         need(nfreq > 0, "generate_transformed_legendre_grid: num_freq_points must be positive");
-        need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
+        need(npar > 0, "Pade: npar must be positive");
         need(job->mo_energy && job->vxc_diag, "mgw_hip_g0w0: mo_energy / vxc_diag are NULL");
         need(out->qp != nullptr, "mgw_hip_g0w0: out->qp is NULL");
         const bool cplx = job->eta != 0.0;
@@ -1585,6 +1586,8 @@ void mol_prepare(MolRun& r, mgw_hip_ctx* ctx, const mgw_g0w0_job* job, mgw_g0w0_
         r.sx = as<double>(ctx->get(ctx->hsx, (size_t)nmo * 8, "sigma_x"));
         r.qp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
         r.iters = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
+        if (const size_t pb = mgw::pade_work_bytes(nfreq, r.npar, ns))
+            r.pade = as<double>(ctx->get(ctx->hpade, pb, "pade tables"));
         r.gq = as<double2>(
             ctx->get(ctx->hsg, mgw::sigma_workspace_bytes(nmo, ns, nfreq), "quadrature factors"));
         r.info = as<int>(ctx->get(ctx->hinfo, (size_t)nfreq * 4, "info"));
@@ -2097,7 +2100,8 @@ void mol_tail_begin(MolRun& r, mgw_hip_ctx* ctx, mgw_g0w0_out* out, Tail& t) {
         ctx->h2d(r.qp, t.nan.data(), nmo);
         ctx->h2d(r.iters, t.zero.data(), nmo);
         hip_check(mgw::launch_pade_qp(r.sre, r.sim, r.dom, r.dmo, r.sx, r.dvxc, r.dstates,
-                                      job->fermi_energy, ns, nfreq, r.npar, r.qp, r.iters, st),
+                                      job->fermi_energy, ns, nfreq, r.npar, r.qp, r.iters, r.pade,
+                                      st),
                   "pade/qp");
         hip_check(hipEventRecord(t.ev[2], st), "hipEventRecord");
         if (out->sigma_c_re) ctx->d2h(out->sigma_c_re, r.sre, (size_t)ns * nfreq);
@@ -2324,7 +2328,7 @@ int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_
         need(ctx && omega && sigma_re && sigma_im && sigma_x_diag && mo_energy && vxc_diag && qp,
              "mgw_hip_pade_qp: NULL argument");
         need(nmo > 0 && nfreq > 0, "mgw_hip_pade_qp: nmo and nfreq must be positive");
-        need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
+        need(npar > 0, "Pade: npar must be positive");
         std::vector<int> st;
         if (nstates <= 0 || !states) {
             st.resize(nmo);
@@ -2347,6 +2351,8 @@ int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_
         int* dst = as<int>(ctx->get(ctx->hstates, (size_t)ns * 4, "states"));
         double* dqp = as<double>(ctx->get(ctx->hqp, (size_t)nmo * 8, "qp"));
         int* dit = as<int>(ctx->get(ctx->hiters, (size_t)nmo * 4, "qp iterations"));
+        const size_t pb = mgw::pade_work_bytes(nfreq, npar, ns);
+        double* dpw = pb ? as<double>(ctx->get(ctx->hpade, pb, "pade tables")) : nullptr;
         ctx->h2d(dom, omega, nfreq);
         ctx->h2d(sre, sigma_re, (size_t)ns * nfreq);
         ctx->h2d(sim, sigma_im, (size_t)ns * nfreq);
@@ -2359,7 +2365,7 @@ int mgw_hip_pade_qp(mgw_hip_ctx* ctx, int32_t nmo, int32_t nstates, const int32_
         ctx->h2d(dqp, nan.data(), nmo);
         ctx->h2d(dit, zero.data(), nmo);
         hip_check(mgw::launch_pade_qp(sre, sim, dom, dmo, sx, dvxc, dst, fermi_energy, ns, nfreq,
-                                      npar, dqp, dit, s),
+                                      npar, dqp, dit, dpw, s),
                   "pade/qp");
         ctx->d2h(qp, dqp, nmo);
         if (iterations) ctx->d2h(iterations, dit, nmo);
@@ -2375,16 +2381,19 @@ int mgw_hip_pade_real_axis(mgw_hip_ctx* ctx, const double* sigma_re, const doubl
         need(ctx && sigma_re && sigma_im && omega && out_re && out_im,
              "mgw_hip_pade_real_axis: NULL argument");
         need(nfreq > 0 && npoints >= 0, "mgw_hip_pade_real_axis: nfreq must be positive");
-        need(npar > 0 && npar <= mgw::kMaxPade, "Pade: npar must be in [1, %d]", mgw::kMaxPade);
+        need(npar > 0, "Pade: npar must be positive");
         if (npoints == 0) return;
-        double* d = as<double>(ctx->get(ctx->wsrc, (size_t)(3 * nfreq + 2 * npoints) * 8, "pade"));
+        const size_t pb = mgw::pade_work_bytes(nfreq, npar, 1);
+        const size_t wo = ((size_t)3 * nfreq + 2 * (size_t)npoints + 1) & ~(size_t)1;   // 16 B aligned
+        double* d = as<double>(ctx->get(ctx->wsrc, wo * 8 + pb, "pade"));
         ctx->h2d(d, sigma_re, nfreq);
         ctx->h2d(d + nfreq, sigma_im, nfreq);
         ctx->h2d(d + 2 * nfreq, omega, nfreq);
         double* ore = d + 3 * nfreq;
         double* oim = ore + npoints;
         hip_check(mgw::launch_pade_real_axis(d, d + nfreq, d + 2 * nfreq, nfreq, npar, fermi_energy,
-                                             x_start, interval, npoints, ore, oim, ctx->stream),
+                                             x_start, interval, npoints, ore, oim,
+                                             pb ? d + wo : nullptr, ctx->stream),
                   "pade real axis");
         ctx->d2h(out_re, ore, npoints);
         ctx->d2h(out_im, oim, npoints);

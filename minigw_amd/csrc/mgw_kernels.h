@@ -205,15 +205,20 @@ hipError_t launch_sigma_c(const double* q, const double* q_im, const double* mo_
 hipError_t launch_sigma_x(const double* src, double* sx, int nmo, int nocc, int64_t sp, int64_t si,
                           hipStream_t s);
 
-// Pade fit + QP fixed point per state (FP64, no FMA contraction).
+// Pade fit + QP fixed point per state (FP64, no FMA contraction).  With more
+// than kMaxPade points (min(npar, nfreq)) the fraction's tables go to `work`,
+// pade_work_bytes(nfreq, npar, nstates) bytes (1 state for the real axis);
+// otherwise work may be NULL.
+size_t pade_work_bytes(int nfreq, int npar, int nstates);
 // Thiele Pade of one state's row evaluated at x_start + i * interval - fermi.
 hipError_t launch_pade_real_axis(const double* sig_re, const double* sig_im, const double* omega,
                                  int nfreq, int npar, double fermi, double x_start, double interval,
-                                 int npoints, double* out_re, double* out_im, hipStream_t s);
+                                 int npoints, double* out_re, double* out_im, double* work,
+                                 hipStream_t s);
 hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const double* omega,
                           const double* mo_energy, const double* sx, const double* vxc_diag,
                           const int* states, double fermi, int nstates, int nfreq, int npar,
-                          double* qp, int* iters, hipStream_t s);
+                          double* qp, int* iters, double* work, hipStream_t s);
 
 // Peak calibration: `blocks` x 256 threads issuing 8 x iters independent MFMAs
 // (kind 0 f64 16x16x4, 1 f16 16x16x32, 2 bf16 16x16x32) on registers; and a

@@ -642,14 +642,14 @@ __global__ void k_sigma_x(const double* __restrict__ src, double* __restrict__ s
 }
 
 // Thiele continued-fraction Pade (src/gw/pade.cpp:10-70): coefficients of
-// one state's Sigma_c row from npar sampled points (0, s, 2s, ..., nfreq-1),
-// reciprocal differences with the 1e-20 guard.  Returns the number of points.
+// one state's Sigma_c row from min(npar, nfreq) sampled points (0, s, 2s, ...,
+// nfreq-1), reciprocal differences with the 1e-20 guard, in the table g (as
+// many entries as points).  Returns the number of points.
 __device__ int pade_fit(const double* __restrict__ row_re, const double* __restrict__ row_im,
                         const double* __restrict__ omega, int nfreq, int npar, cplx* xs,
-                        cplx* coef) {
+                        cplx* coef, cplx* g) {
 #pragma clang fp contract(off)
     const int npa = npar < nfreq ? npar : nfreq;
-    cplx g[kMaxPade];
     const int step = npa > 1 ? nfreq / (npa - 1) : 0;
     for (int i = 0; i < npa; ++i) {
         int idx;
@@ -689,18 +689,27 @@ __device__ cplx pade_eval(const cplx* coef, const cplx* xs, int npa, double z) {
 }
 
 // Pade + QP fixed point (src/gw/qp_solver.cpp:10-47), one thread per state.
+// Up to kMaxPade points the tables live in the thread's private arrays; with
+// more (kGlobal) in `work`, 3 min(npar, nfreq) complex entries per state.
+template <bool kGlobal>
 __global__ void k_pade_qp(const double* __restrict__ sre, const double* __restrict__ sim,
                           const double* __restrict__ omega, const double* __restrict__ mo,
                           const double* __restrict__ sx, const double* __restrict__ vxc,
                           const int* __restrict__ states, double fermi, int nstates, int nfreq,
-                          int npar, double* __restrict__ qp, int* __restrict__ iters) {
+                          int npar, double* __restrict__ qp, int* __restrict__ iters,
+                          double2* __restrict__ work) {
 #pragma clang fp contract(off)
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
     if (si >= nstates) return;
     const int state = states[si];
-    cplx xs[kMaxPade], coef[kMaxPade];
+    constexpr int kLocal = kGlobal ? 1 : kMaxPade;
+    cplx xs_l[kLocal], coef_l[kLocal], g_l[kLocal];
+    const int nt = npar < nfreq ? npar : nfreq;
+    cplx* xs = kGlobal ? reinterpret_cast<cplx*>(work) + (size_t)si * 3 * nt : xs_l;
+    cplx* coef = kGlobal ? xs + nt : coef_l;
+    cplx* g = kGlobal ? coef + nt : g_l;
     const int npa = pade_fit(sre + (size_t)si * nfreq, sim + (size_t)si * nfreq, omega, nfreq, npar,
-                             xs, coef);
+                             xs, coef, g);
     const double e0 = mo[state];
     double cur = e0;
     int used = 200;
@@ -727,10 +736,35 @@ __global__ void k_pade_real_axis(const double* __restrict__ sre, const double* _
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoints) return;
-    cplx xs[kMaxPade], coef[kMaxPade];
-    const int npa = pade_fit(sre, sim, omega, nfreq, npar, xs, coef);
+    cplx xs[kMaxPade], coef[kMaxPade], g[kMaxPade];
+    const int npa = pade_fit(sre, sim, omega, nfreq, npar, xs, coef, g);
     const double e = x_start + static_cast<double>(i) * interval - fermi;
     const cplx v = pade_eval(coef, xs, npa, e);
+    out_re[i] = v.re;
+    out_im[i] = v.im;
+}
+
+// The same continuation with more than kMaxPade points: one thread fits the
+// fraction into `work` (xs, coef, g: 3 min(npar, nfreq) entries) ...
+__global__ void k_pade_fit_one(const double* __restrict__ sre, const double* __restrict__ sim,
+                               const double* __restrict__ omega, int nfreq, int npar,
+                               double2* __restrict__ work) {
+    const int nt = npar < nfreq ? npar : nfreq;
+    cplx* xs = reinterpret_cast<cplx*>(work);
+    if (blockIdx.x == 0 && threadIdx.x == 0) pade_fit(sre, sim, omega, nfreq, npar, xs, xs + nt, xs + 2 * nt);
+}
+
+// ... and every point evaluates it from there.
+__global__ void k_pade_eval_axis(const double2* __restrict__ work, int nfreq, int npar, double fermi,
+                                 double x_start, double interval, int npoints,
+                                 double* __restrict__ out_re, double* __restrict__ out_im) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoints) return;
+    const int nt = npar < nfreq ? npar : nfreq;
+    const cplx* xs = reinterpret_cast<const cplx*>(work);
+    const double e = x_start + static_cast<double>(i) * interval - fermi;
+    const cplx v = pade_eval(xs + nt, xs, nt, e);
     out_re[i] = v.re;
     out_im[i] = v.im;
 }
@@ -1019,20 +1053,39 @@ hipError_t launch_sigma_x(const double* src, double* sx, int nmo, int nocc, int6
     return hipGetLastError();
 }
 
+size_t pade_work_bytes(int nfreq, int npar, int nstates) {
+    const int nt = npar < nfreq ? npar : nfreq;
+    return nt <= kMaxPade ? 0 : (size_t)nstates * 3 * nt * sizeof(double2);
+}
+
 hipError_t launch_pade_real_axis(const double* sig_re, const double* sig_im, const double* omega,
                                  int nfreq, int npar, double fermi, double x_start, double interval,
-                                 int npoints, double* out_re, double* out_im, hipStream_t s) {
-    hipLaunchKernelGGL(k_pade_real_axis, dim3((npoints + 127) / 128), dim3(128), 0, s, sig_re, sig_im,
-                       omega, nfreq, npar, fermi, x_start, interval, npoints, out_re, out_im);
+                                 int npoints, double* out_re, double* out_im, double* work,
+                                 hipStream_t s) {
+    if (pade_work_bytes(nfreq, npar, 1) == 0) {
+        hipLaunchKernelGGL(k_pade_real_axis, dim3((npoints + 127) / 128), dim3(128), 0, s, sig_re,
+                           sig_im, omega, nfreq, npar, fermi, x_start, interval, npoints, out_re,
+                           out_im);
+        return hipGetLastError();
+    }
+    if (!work) return hipErrorInvalidValue;
+    double2* w = reinterpret_cast<double2*>(work);
+    hipLaunchKernelGGL(k_pade_fit_one, dim3(1), dim3(64), 0, s, sig_re, sig_im, omega, nfreq, npar, w);
+    hipLaunchKernelGGL(k_pade_eval_axis, dim3((npoints + 127) / 128), dim3(128), 0, s,
+                       static_cast<const double2*>(w), nfreq, npar, fermi, x_start, interval, npoints,
+                       out_re, out_im);
     return hipGetLastError();
 }
 
 hipError_t launch_pade_qp(const double* sig_re, const double* sig_im, const double* omega,
                           const double* mo_energy, const double* sx, const double* vxc_diag,
                           const int* states, double fermi, int nstates, int nfreq, int npar,
-                          double* qp, int* iters, hipStream_t s) {
-    hipLaunchKernelGGL(k_pade_qp, dim3((nstates + 63) / 64), dim3(64), 0, s, sig_re, sig_im, omega,
-                       mo_energy, sx, vxc_diag, states, fermi, nstates, nfreq, npar, qp, iters);
+                          double* qp, int* iters, double* work, hipStream_t s) {
+    const bool global = pade_work_bytes(nfreq, npar, nstates) > 0;
+    if (global && !work) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(global ? k_pade_qp<true> : k_pade_qp<false>, dim3((nstates + 63) / 64),
+                       dim3(64), 0, s, sig_re, sig_im, omega, mo_energy, sx, vxc_diag, states, fermi,
+                       nstates, nfreq, npar, qp, iters, reinterpret_cast<double2*>(work));
     return hipGetLastError();
 }
 

@@ -39,6 +39,24 @@ def test_pade_real_axis_reproduces_e_c_out(tmp_path):
     assert p.read_text() == "".join(f"{a:.17g} {b:.17g} {c:.17g}\n" for a, b, c in ec)
 
 
+@pytest.mark.parametrize("npar", [16, 100])
+def test_pade_real_axis_any_order_matches_oracle(npar):
+    """The real-axis continuation at 16 and at 100 Pade points (beyond 64 the
+    fraction is fitted once into global memory and every point evaluates it
+    from there) equals the oracle's evaluation of the same fraction."""
+    say(f"case: device real-axis Pade, npar={npar}, 2000 points")
+    from minigw_amd import gwio, hip
+    from oracle import oracle as orc
+    fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
+    inp = h2o_input()
+    om, _ = hip.transformed_legendre_grid(200)
+    with hip.HipContext(0) as ctx:
+        v = gwio.pade_real_axis(fx["e_c_before_pade"], om, npar, inp.fermi_energy, npoints=2000,
+                                ctx=ctx)
+    want = orc.pade_real_axis(fx["e_c_before_pade"], om, npar, inp.fermi_energy, num_points=2000)
+    assert np.array_equal(v.real, want[:, 1]) and np.array_equal(v.imag, want[:, 2])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed_int8", "fp64_cd"])
 def test_runner_h2o_against_regression_outputs(tmp_path, precision):
     """python -m minigw_amd.run on H2O (raw input written from the fixture):

@@ -266,8 +266,28 @@ def test_pade_qp_entry_point_matches_oracle(hip, ctx):
     assert np.isnan(qp_d[1]) and it_d[1] == 0
     with pytest.raises(hip.HipError) as e:
         hip.pade_qp(sig, want.omegas, want.sigma_x_diag, inp.mo_energy, np.diag(inp.vxc),
-                    inp.fermi_energy, 65, states=states, ctx=ctx)
-    assert e.value.code == hip.MGW_ERR_ARG
+                    inp.fermi_energy, 0, states=states, ctx=ctx)
+    assert e.value.code == hip.MGW_ERR_ARG and "npar must be positive" in str(e.value)
+
+
+@pytest.mark.parametrize("nfreq,npar", [(96, 80), (96, 200), (40, 64), (40, 65), (24, 1)])
+def test_pade_qp_any_order_matches_oracle(hip, ctx, nfreq, npar):
+    """Any Pade order the reference accepts (it fits min(npar, nfreq) points,
+    src/gw/pade.cpp:25-48): beyond 64 points the device keeps the fraction's
+    tables in global memory instead of the thread's registers; QP energies
+    and iteration counts equal the oracle's bit for bit."""
+    say(f"case: mgw_hip_pade_qp, nfreq={nfreq}, npar={npar}, nmo=24")
+    inp = synthetic_input(*CONFIGS["nmo24"])
+    want = orc.run_g0w0(inp, nfreq, 16)
+    rng = np.random.default_rng(nfreq + npar)
+    sig = want.sigma_c * (1 + 1e-3 * rng.standard_normal(want.sigma_c.shape))
+    states = list(range(inp.nmo))
+    qp_o, it_o = orc.solve_qp(inp.mo_energy, inp.fermi_energy, want.sigma_x_diag,
+                              np.diag(inp.vxc), sig, want.omegas, states, npar)
+    qp_d, it_d = hip.pade_qp(sig, want.omegas, want.sigma_x_diag, inp.mo_energy, np.diag(inp.vxc),
+                             inp.fermi_energy, npar, ctx=ctx)
+    assert np.array_equal(qp_d, qp_o, equal_nan=True)
+    assert np.array_equal(it_d, it_o)
 
 
 @pytest.mark.parametrize("nmo,nocc,nfreq,eta", [(60, 12, 12, 0.0), (90, 18, 6, 0.0),
@@ -323,8 +343,17 @@ def test_subset_states_and_errors(hip, ctx):
     with pytest.raises(hip.HipError) as e:
         hip.run_g0w0(inp, 24, 8, states=[3, 3], ctx=ctx, upload=False)
     assert e.value.code == hip.MGW_ERR_ARG
-    with pytest.raises(hip.HipError):
-        hip.run_g0w0(inp, 24, 65, ctx=ctx, upload=False)
+    with pytest.raises(hip.HipError) as e:
+        hip.run_g0w0(inp, 24, 0, ctx=ctx, upload=False)
+    assert e.value.code == hip.MGW_ERR_ARG and "npar must be positive" in str(e.value)
+    # a Pade order beyond the points fits all of them, as the reference does
+    more = hip.run_g0w0(inp, 24, 65, ctx=ctx, upload=False)
+    same = hip.run_g0w0(inp, 24, 24, ctx=ctx, upload=False)
+    assert np.array_equal(more.qp, same.qp, equal_nan=True)
+    big = hip.run_g0w0(inp, 96, 80, ctx=ctx, upload=False)
+    qp_o, it_o = orc.solve_qp(inp.mo_energy, inp.fermi_energy, big.sigma_x_diag, np.diag(inp.vxc),
+                              big.sigma_c, big.omegas, list(range(inp.nmo)), 80)
+    assert np.array_equal(big.qp, qp_o, equal_nan=True) and np.array_equal(big.qp_iterations, it_o)
 
 
 # ------------------------------------------------------------ mixed precision
