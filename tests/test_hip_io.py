@@ -57,8 +57,19 @@ def test_pade_real_axis_any_order_matches_oracle(npar):
     assert np.array_equal(v.real, want[:, 1]) and np.array_equal(v.imag, want[:, 2])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed_int8", "fp64_cd"])
-def test_runner_h2o_against_regression_outputs(tmp_path, precision):
+REFERENCE_SCHEDULING = ["--frequency-parallel", "serial", "--kernel-parallel", "openmp",
+                        "--contraction-panel-size", "16", "--tasks-per-gpu", "1",
+                        "--print-memory-footprint"]
+
+
+@pytest.mark.parametrize("precision,argv", [
+    ("fp64", ["--precision", "fp64"]),
+    ("mixed_int8", ["--precision", "mixed_int8"]),
+    ("fp64_cd", ["--precision", "fp64_cd"]),
+    # the form chosen by backend name, with the reference CLI's scheduling options
+    ("mixed_int8", ["--linalg-backend", "hip-int8"] + REFERENCE_SCHEDULING),
+])
+def test_runner_h2o_against_regression_outputs(tmp_path, precision, argv):
     """python -m minigw_amd.run on H2O (raw input written from the fixture):
     E_c_before_Pade.out within the regression test's 1e-8 component check,
     E_c.out within the spread of the reference's own backends (2e-6 Ha) in
@@ -71,8 +82,8 @@ def test_runner_h2o_against_regression_outputs(tmp_path, precision):
     fx = np.load(os.path.join(GOLD, "h2o_serial_fixtures.npz"))
     gwio.write_gw_input_raw(str(tmp_path / "gw_input.raw"), h2o_input())
     outdir = tmp_path / "out"
-    assert run.main(["--input-dir", str(tmp_path), "--output-dir", str(outdir),
-                     "--precision", precision]) == 0
+    assert run.main(["--input-dir", str(tmp_path), "--output-dir", str(outdir)] + argv) == 0
+    assert f"({precision}, 200 frequencies)" in (outdir / "gw.out").read_text()
     got = []
     for line in (outdir / "E_c_before_Pade.out").read_text().splitlines():
         re_, im_ = line.strip("()").split(",")

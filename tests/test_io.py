@@ -137,3 +137,28 @@ def test_energy_table_format(gwio):
         parts = line.split()
         assert len(parts) == 6 and parts[0] == str(p + 1)
         assert all(math.isclose(float(a), b, rel_tol=1e-5) for a, b in zip(parts[1:], vals))
+
+
+@pytest.mark.parametrize("argv,message", [
+    (["--freq-points", "0"], "--freq-points must be positive"),
+    (["--pade-params", "0"], "--pade-params must be positive"),
+    (["--contraction-panel-size", "0"], "--contraction-panel-size must be positive"),
+    (["--tasks-per-gpu", "0"], "--tasks-per-gpu must be positive"),
+    (["--scalapack-ranks-per-group", "0"], "--scalapack-ranks-per-group must be positive"),
+    (["--frequency-parallel", "threads"], "--frequency-parallel must be one of"),
+    (["--kernel-parallel", "mpi"], "--kernel-parallel must be one of"),
+])
+def test_runner_checks_options_as_the_reference_cli(tmp_path, argv, message):
+    """The runner refuses the option values the reference CLI refuses, with
+    its messages (src/cli.cpp:75-100), before it touches a device."""
+    from minigw_amd import run
+    with pytest.raises(SystemExit) as e:
+        run.main(["--input-dir", str(tmp_path), "--output-dir", str(tmp_path / "o")] + argv)
+    assert message in str(e.value)
+
+
+def test_runner_backend_names():
+    """Backend names select the forms as the reference-side hip backend does."""
+    from minigw_amd import run
+    assert run.BACKENDS == {"hip": "fp64", "hip-int8": "mixed_int8", "hip-int8x4": "mixed_int8x4",
+                            "hip-fp16": "mixed", "hip-cd": "fp64_cd"}
