@@ -420,17 +420,23 @@ def _golden_case(name):
         inp = h2o_input()
     elif name == "syn_nmo24":
         inp = synthetic_input(*CONFIGS["nmo24"])
+    elif name == "syn_gap":
+        inp = with_gap(synthetic_input(30, 6, 47), 1e-4)
     else:
         inp = GwInput(mo_energy=r["mo_energy"], eri=r["eri"], vxc=r["vxc"], nocc=int(r["nocc"]),
                       fermi_energy=float(r["fermi_energy"]))
     return r, inp
 
 
-@pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd"])
+@pytest.mark.parametrize("name", ["h2o_all", "syn_nmo24", "syn_small", "syn_odd", "syn_gap"])
 def test_fp64_cd_vs_reference_path(hip, ctx, name):
     """Decomposed-ERI form (device pivoted Cholesky of the pair matrix, pair-space
     screening) against the reference path's all-state outputs: Sigma_c to
-    1e-10 absolute; the QP energies are those the oracle's Pade + QP step
+    1e-10 absolute; on the ill-conditioned syn_gap case (max |Sigma_c| 2472
+    Ha) to 5e-11 of max |Sigma_c| (measured 8.3e-12, 2e-8 Ha): its q is the
+    difference of two norms, ||L_C^-1 l||^2 - ||l||^2 (Woodbury in pair
+    space), which cancels more as S^2 grows toward 4 / gap, while the plain
+    form stays at 2e-15.  The QP energies are those the oracle's Pade + QP step
     gives on the same Sigma_c (the QP step of these molecules is too
     ill-conditioned for a tolerance on the QP energies alone: 1e-12 relative
     noise on Sigma_c moves some by 0.1 Ha).  The synthetic ERIs have rank
@@ -444,7 +450,7 @@ def test_fp64_cd_vs_reference_path(hip, ctx, name):
     say(f"   rank {rank}; max |dSigma_c| = {dsig:.3e}")
     npairs = inp.nmo * (inp.nmo + 1) // 2
     assert 0 < rank <= (npairs if name == "h2o_all" else min(npairs, 3 * inp.nmo))
-    assert dsig < 1e-10
+    assert dsig < (5e-11 * np.abs(r["sigma_c"]).max() if name == "syn_gap" else 1e-10)
     states = list(range(inp.nmo))
     qp_orc, _ = orc.solve_qp(inp.mo_energy, inp.fermi_energy, res.sigma_x_diag, np.diag(inp.vxc),
                              res.sigma_c, res.omegas, states, npar)
