@@ -92,9 +92,12 @@ int mgw_hip_device_for_local_rank(size_t local_rank, size_t tasks_per_gpu, int* 
  * MGW_SPLIT_UNIT_FREQS (unit shape of the mixed contractions),
  * MGW_FACTOR_STREAMS (1, 2 or 4 streams for the parts of a frequency batch
  * in the factorization; default 2), MGW_FACTOR_SPLIT_ORDER (smallest
- * matrix order that is split; default 4096) and MGW_ERI_SLAB_BYTES (largest
- * slab of a streamed ERI upload; default 512 MiB, at least one row p).  The
- * last four change scheduling only, never a result. */
+ * matrix order that is split; default 4096), MGW_ERI_SLAB_BYTES (largest
+ * slab of a streamed ERI upload; default 512 MiB, at least one row p) and,
+ * read once per process, MGW_ASYNC_ALLOC (1: stream-ordered allocation
+ * from the device's default memory pool instead of hipMalloc / hipFree;
+ * default 0).  All but the first change scheduling or memory handling only,
+ * never a result. */
 int mgw_hip_create(int device, mgw_hip_ctx** out);
 int mgw_hip_destroy(mgw_hip_ctx* ctx);
 
@@ -348,7 +351,8 @@ typedef struct mgw_serve_job {
 } mgw_serve_job;
 
 /* Start `workers` threads on `device`, each with its own context and stream
- * (graphs on).  Jobs are admitted in arrival order while the bytes reserved
+ * (graphs on; MGW_SERVE_GRAPHS=0 in the environment at creation runs the
+ * frequency batches eagerly, results unchanged).  Jobs are admitted in arrival order while the bytes reserved
  * by running jobs plus the next job's estimate fit hbm_budget; batch_freqs
  * caps the frequencies factored at once per job (mgw_hip_set_batch_limit).
  * Mixed-precision jobs run under the error judge with FP64 fallback

@@ -168,6 +168,20 @@ struct mgw_hip_ctx {
     };
     std::vector<Buf*> all;
 
+    // Stream-ordered allocation from the device's default memory pool
+    // (MGW_ASYNC_ALLOC=1; off by default): a free does not wait for the whole
+    // device as a plain hipFree does, which stalls the other workers of the
+    // request loop at each job's end.  Off by default because a context then
+    // keeps device memory past the release of its buffers until it is
+    // destroyed (2 GiB after an nph = 4096 job, outside the pool's own
+    // accounting), which the released-memory test catches.
+    static bool async_alloc() {
+        static const bool on = [] {
+            const char* env = std::getenv("MGW_ASYNC_ALLOC");
+            return env != nullptr && env[0] == '1';
+        }();
+        return on;
+    }
     void* get(Buf& b, size_t bytes, const char* name) {
         if (bytes <= b.bytes && b.p) return b.p;
         release(b);
@@ -176,7 +190,8 @@ struct mgw_hip_ctx {
             fail(MGW_ERR_MEMORY, "%s: %zu bytes would exceed the device budget (%zu held of %zu)",
                  name, bytes, held, budget);
         void* p = nullptr;
-        const hipError_t e = hipMalloc(&p, bytes);
+        const hipError_t e = async_alloc() && stream ? hipMallocAsync(&p, bytes, stream)
+                                                     : hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             fail(MGW_ERR_MEMORY, "hipMalloc(%s, %zu bytes) failed: %s", name, bytes,
@@ -190,7 +205,10 @@ struct mgw_hip_ctx {
     void release(Buf& b) {
         if (b.p && !graphs.empty()) drop_graphs();   // a freed address may be handed out again
         if (b.p) {
-            (void)hipFree(b.p);
+            if (async_alloc() && stream)
+                (void)hipFreeAsync(b.p, stream);
+            else
+                (void)hipFree(b.p);
             held -= b.bytes;
         }
         b.p = nullptr;
@@ -741,6 +759,15 @@ int mgw_hip_create(int device, mgw_hip_ctx** out) {
         ctx->slab_bytes = env_slab_bytes();
         if (const char* env = std::getenv("MGW_FREQ_BATCH_CAP"))
             ctx->batch_cap = std::min(1024, std::max(1, std::atoi(env)));
+        if (mgw_hip_ctx::async_alloc()) {
+            // keep freed memory in the pool (the default threshold returns it at every sync)
+            hipMemPool_t pool = nullptr;
+            if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+                uint64_t keep = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+            (void)hipGetLastError();
+        }
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
         // only the side streams the split uses: idle streams still take hardware queues,
@@ -1088,6 +1115,22 @@ size_t streamed_eri_bytes(int nmo, int nocc, size_t slab_bytes) {
     return held + (size_t)slab_rows(nmo, slab_bytes) * nmo * nmo * nmo * 8;
 }
 
+// Free device memory as the allocator sees it: the driver's free bytes plus
+// what the memory pool holds unused (freed buffers it keeps for reuse).
+hipError_t device_free_bytes(int device, size_t* free_b, size_t* total_b) {
+    hipError_t e = hipMemGetInfo(free_b, total_b);
+    if (e != hipSuccess || !mgw_hip_ctx::async_alloc()) return e;
+    hipMemPool_t pool = nullptr;
+    uint64_t reserved = 0, used = 0;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess &&
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess &&
+        reserved > used)
+        *free_b += (size_t)(reserved - used);
+    (void)hipGetLastError();
+    return hipSuccess;
+}
+
 // The storage an upload takes: the requested one, or for MGW_ERI_AUTO the
 // reference's rule (src/workspace/device_pq_ph_panel.cu:168-190): resident
 // only while the tensor takes less than a quarter of the free device memory
@@ -1100,7 +1143,7 @@ int choose_storage(mgw_hip_ctx* ctx, int nmo) {
     ctx->release(ctx->exsd);
     if (ctx->eri_req != MGW_ERI_AUTO) return ctx->eri_req;
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    if (device_free_bytes(ctx->device, &free_b, &total_b) != hipSuccess) {
         (void)hipGetLastError();
         return MGW_ERI_STREAMED;
     }
@@ -1228,6 +1271,7 @@ int mgw_hip_release_buffers(mgw_hip_ctx* ctx) {
         need(ctx != nullptr, "mgw_hip_release_buffers: ctx is NULL");
         ctx->sync();
         for (auto* b : ctx->all) ctx->release(*b);
+        ctx->sync();                 // stream-ordered frees have returned the memory to the pool
         ctx->nmo = ctx->nocc = 0;
         ctx->eri_storage = 0;
         ctx->n = ctx->np = 0;
@@ -1629,7 +1673,7 @@ void run_batches(mgw_hip_ctx* lead, const std::vector<MolRun*>& mols, bool allow
 
         // frequency batch size from the device budget
         size_t free_b = 0, total_b = 0;
-        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        hip_check(device_free_bytes(lead->device, &free_b, &total_b), "hipMemGetInfo");
         // the batch buffers this context already holds are reused or released
         // before a larger allocation: they count as room
         size_t reusable = 0;

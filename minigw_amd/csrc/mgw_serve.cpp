@@ -12,7 +12,8 @@
 // forms) fit the server's budget; jobs start in arrival order.  A job whose
 // estimate exceeds the whole budget fails with MGW_ERR_MEMORY instead of
 // waiting forever.  After each job a worker frees its device buffers, so the
-// reservation is the job's footprint only.  The ERI storage of a job
+// reservation is the job's footprint only (a plain hipFree waits for the
+// whole device; MGW_ASYNC_ALLOC=1 makes the frees stream-ordered instead).  The ERI storage of a job
 // (mgw_hip_server_set_eri_storage) enters its estimate: a streamed job
 // reserves the held (pk|ia) block and one upload slab instead of the nmo^4
 // tensor, and under MGW_ERI_AUTO a job that does not fit resident starts
@@ -31,6 +32,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -89,6 +91,7 @@ struct mgw_hip_server {
     double budget_scale = 1.0;        // fault-injection hook: per-job device budget scale
     int32_t group_max = 16;           // jobs one worker runs together (1: no grouping)
     int32_t eri_storage = MGW_ERI_RESIDENT;   // storage of jobs submitted from now on
+    int32_t graphs = 1;               // workers' contexts replay frequency batches as hipGraphs
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServeJob*> pending;
@@ -151,7 +154,7 @@ struct mgw_hip_server {
         // batches); pool[k] holds the k-th grouped job's resident ERI
         std::vector<mgw_hip_ctx*> pool(1, nullptr);
         const int rc = mgw_hip_create(device, &pool[0]);
-        if (rc == MGW_OK) mgw_hip_set_graphs(pool[0], 1);   // batches of a job replay one graph
+        if (rc == MGW_OK) mgw_hip_set_graphs(pool[0], graphs);   // batches of a job replay one graph
         for (;;) {
             std::vector<ServeJob*> group;
             int32_t jn = 0, jf = 0;
@@ -219,7 +222,7 @@ struct mgw_hip_server {
             while ((int)pool.size() < k) {
                 mgw_hip_ctx* c = nullptr;
                 if (mgw_hip_create(device, &c) != MGW_OK) break;
-                mgw_hip_set_graphs(c, 1);
+                mgw_hip_set_graphs(c, graphs);
                 pool.push_back(c);
             }
             std::vector<int> st(k, MGW_OK);
@@ -321,6 +324,7 @@ int mgw_hip_server_create(int device, int32_t workers, size_t hbm_budget, int32_
     s->device = device;
     s->budget = hbm_budget;
     s->batch_freqs = batch_freqs;
+    if (const char* env = std::getenv("MGW_SERVE_GRAPHS")) s->graphs = env[0] != '0';
     for (int i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->work(); });
     *out = s;
     return MGW_OK;
