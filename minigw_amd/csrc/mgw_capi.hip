@@ -567,6 +567,7 @@ void lu_screen(mgw_hip_ctx* ctx, const double* V, const double* S, double* A, do
     }
     ctx->h2d(perm, hperm.data(), hperm.size());
     hip_check(mgw::launch_lu_inverse(A, dl, du, perm, W, np, count, st), "LU inverse");
+    ctx->sync();       // the copy reads hperm, which ends with this function
 }
 
 struct CdResult {
