@@ -1433,6 +1433,9 @@ struct MolRun {
     MolRun(const MolRun&) = delete;
     MolRun& operator=(const MolRun&) = delete;
     ~MolRun() {
+        // the host vectors above feed asynchronous copies on the context's
+        // stream: on an error path the run can end while they are pending
+        if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
     }
