@@ -217,28 +217,29 @@ def run_batch(hip, count: int, workers: int, nfreq: int, npar: int, prec: int,
     """Uneven-nmo batch: `count` molecules (nmo log-uniform in [24, 120],
     nocc = round(nmo / 5)), all submitted at once to the request loop, whose
     workers group molecules of one padded order (up to `grouping` per group;
-    1: every molecule alone on its worker).  The batch runs `reps` times on
-    one server; the figures are those of the repetition with the median wall
-    time (single batches have shown rare multi-second outliers), and every
-    repetition's wall time is listed."""
+    1: every molecule alone on its worker).  The batch runs `reps` times, each
+    on a new server after one warm-up job (batches run back to back on one
+    server have taken longer each time on some boxes); the figures are those
+    of the repetition with the median wall time, and every repetition's wall
+    time is listed."""
     from minigw_amd.synthetic import batch_molecules
     mols = [_factor_job(*m) for m in batch_molecules(count)]
     runs = []
-    with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=64) as srv:
-        srv.set_grouping(grouping)
-        # warm-up: compile-free, but first-touch allocations and graph capture
-        t = srv.submit(factor=mols[0]["b"], mo_energy=mols[0]["mo"], vxc=mols[0]["vxc"],
-                       nocc=mols[0]["nocc"], fermi=mols[0]["fermi"], nfreq=nfreq, npar=npar,
-                       precision=prec)
-        srv.wait(t)
-        for _ in range(reps):
+    for _ in range(reps):
+        with hip.HipServer(0, workers=workers, hbm_budget=200 << 30, batch_freqs=64) as srv:
+            srv.set_grouping(grouping)
+            # warm-up: compile-free, but first-touch allocations and graph capture
+            t = srv.submit(factor=mols[0]["b"], mo_energy=mols[0]["mo"], vxc=mols[0]["vxc"],
+                           nocc=mols[0]["nocc"], fermi=mols[0]["fermi"], nfreq=nfreq, npar=npar,
+                           precision=prec)
+            srv.wait(t)
             t0 = time.perf_counter()
             tickets = [srv.submit(factor=m["b"], mo_energy=m["mo"], vxc=m["vxc"], nocc=m["nocc"],
                                   fermi=m["fermi"], nfreq=nfreq, npar=npar, precision=prec)
                        for m in mols]
             lat = [srv.wait(t)[1] for t in tickets]
             runs.append((time.perf_counter() - t0, lat))
-        st = srv.stats()
+            st = srv.stats()
     wall, lat = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
     solves = sum(m["nmo"] for m in mols)
     return {"molecules": count, "nmo_range": [min(m["nmo"] for m in mols), max(m["nmo"] for m in mols)],
