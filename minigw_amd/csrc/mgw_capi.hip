@@ -1135,14 +1135,19 @@ hipError_t device_free_bytes(int device, size_t* free_b, size_t* total_b) {
 // The storage an upload takes: the requested one, or for MGW_ERI_AUTO the
 // reference's rule (src/workspace/device_pq_ph_panel.cu:168-190): resident
 // only while the tensor takes less than a quarter of the free device memory
-// (of the context's budget left, when that is smaller).  The context's
-// previous ERI is released first so that it does not count against the new one.
+// (of the context's budget left, when that is smaller).  Under MGW_ERI_AUTO
+// the context's previous ERI is released first so that it does not count
+// against the new one; a requested storage keeps the held ERI buffer, which
+// the upload reuses when it is large enough.
 int choose_storage(mgw_hip_ctx* ctx, int nmo) {
     ctx->nmo = 0;
     ctx->eri_storage = 0;
+    if (ctx->eri_req != MGW_ERI_AUTO) {
+        if (ctx->eri_req == MGW_ERI_RESIDENT) ctx->release(ctx->exsd);   // streamed-only
+        return ctx->eri_req;
+    }
     ctx->release(ctx->eri);
     ctx->release(ctx->exsd);
-    if (ctx->eri_req != MGW_ERI_AUTO) return ctx->eri_req;
     size_t free_b = 0, total_b = 0;
     if (device_free_bytes(ctx->device, &free_b, &total_b) != hipSuccess) {
         (void)hipGetLastError();

@@ -11,9 +11,11 @@
 // mgw_hip_estimate_bytes, plus the error judge's FP64 panel for the mixed
 // forms) fit the server's budget; jobs start in arrival order.  A job whose
 // estimate exceeds the whole budget fails with MGW_ERR_MEMORY instead of
-// waiting forever.  After each job a worker frees its device buffers, so the
-// reservation is the job's footprint only (a plain hipFree waits for the
-// whole device; MGW_ASYNC_ALLOC=1 makes the frees stream-ordered instead).  The ERI storage of a job
+// waiting forever.  After each job a worker frees the device buffers of its
+// grouped members; its lead context keeps its own (reused by the next job
+// when large enough) while they hold under a quarter of the budget's
+// per-worker share, so the reservation exceeds the running jobs' estimates
+// by at most that much per worker.  The ERI storage of a job
 // (mgw_hip_server_set_eri_storage) enters its estimate: a streamed job
 // reserves the held (pk|ia) block and one upload slab instead of the nmo^4
 // tensor, and under MGW_ERI_AUTO a job that does not fit resident starts
@@ -92,6 +94,12 @@ struct mgw_hip_server {
     int32_t group_max = 16;           // jobs one worker runs together (1: no grouping)
     int32_t eri_storage = MGW_ERI_RESIDENT;   // storage of jobs submitted from now on
     int32_t graphs = 1;               // workers' contexts replay frequency batches as hipGraphs
+    // A worker's lead context keeps its device buffers from one job to the
+    // next (reused when large enough) while they stay under a quarter of the
+    // budget's per-worker share: allocating and freeing every job's ERI and
+    // batch buffers slowed the loop down with the age of the process
+    // (MGW_SERVE_KEEP_BUFFERS=0: free them after every job).
+    int32_t keep_buffers = 1;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServeJob*> pending;
@@ -250,7 +258,12 @@ struct mgw_hip_server {
                 ServeJob* j = group[i];
                 mgw_hip_ctx* ctx = pool[i];
                 const size_t est = i == 0 ? sum_est : j->estimate;
-                int r = mgw_hip_set_memory_budget(ctx, std::max<size_t>(1, (size_t)((double)est * scale)));
+                // a lead that keeps buffers holds some of an earlier job's: no per-job
+                // cap then (admission still reserves the estimate), unless the
+                // fault-injection hook scales the budget
+                const bool uncapped = keep_buffers && i == 0 && scale == 1.0;
+                int r = mgw_hip_set_memory_budget(
+                    ctx, uncapped ? 0 : std::max<size_t>(1, (size_t)((double)est * scale)));
                 if (r == MGW_OK) r = mgw_hip_set_batch_limit(ctx, i == 0 ? sum_nb : batch_freqs);
                 if (r == MGW_OK) r = mgw_hip_set_judge(ctx, jn, jt, jf);
                 if (r == MGW_OK) r = mgw_hip_set_cancel_flag(ctx, &j->cancel);
@@ -286,7 +299,11 @@ struct mgw_hip_server {
             for (int i = 0; i < k && i < (int)pool.size(); ++i) {
                 if (st[i] == MGW_OK) mgw_hip_last_judge(pool[i], &jerr[i], nullptr, &fell[i]);
                 mgw_hip_set_cancel_flag(pool[i], nullptr);
-                mgw_hip_release_buffers(pool[i]);
+                size_t held = 0;
+                const bool keep = keep_buffers && i == 0 && st[i] == MGW_OK &&
+                                  mgw_hip_device_bytes(pool[i], &held) == MGW_OK &&
+                                  held <= budget / (4 * workers.size());
+                if (!keep) mgw_hip_release_buffers(pool[i]);
             }
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -325,6 +342,7 @@ int mgw_hip_server_create(int device, int32_t workers, size_t hbm_budget, int32_
     s->budget = hbm_budget;
     s->batch_freqs = batch_freqs;
     if (const char* env = std::getenv("MGW_SERVE_GRAPHS")) s->graphs = env[0] != '0';
+    if (const char* env = std::getenv("MGW_SERVE_KEEP_BUFFERS")) s->keep_buffers = env[0] != '0';
     for (int i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->work(); });
     *out = s;
     return MGW_OK;

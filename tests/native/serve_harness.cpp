@@ -144,6 +144,14 @@ int mgw_hip_release_buffers(mgw_hip_ctx* ctx) {
     return MGW_OK;
 }
 
+// bytes held: the resident ERI of the last upload, until released
+int mgw_hip_device_bytes(const mgw_hip_ctx* ctx, size_t* bytes) {
+    if (!ctx || !bytes) return MGW_ERR_ARG;
+    const FakeCtx* c = reinterpret_cast<const FakeCtx*>(ctx);
+    *bytes = c->nmo ? eri_bytes(c->nmo) : 0;
+    return MGW_OK;
+}
+
 // Outputs: qp[p] = 1000 * nmo + p (+ 0.5 when rerun in FP64), sigma = nfreq.
 // The judge and outputs of one finished job.
 static void finish_job(FakeCtx* c, const mgw_g0w0_job* job, mgw_g0w0_out* out) {
