@@ -352,7 +352,10 @@ typedef struct mgw_serve_job {
 
 /* Start `workers` threads on `device`, each with its own context and stream
  * (graphs on; MGW_SERVE_GRAPHS=0 in the environment at creation runs the
- * frequency batches eagerly, results unchanged).  Jobs are admitted in arrival order while the bytes reserved
+ * frequency batches eagerly, results unchanged).  A worker's lead context
+ * keeps its device buffers from one job to the next while they hold under a
+ * quarter of hbm_budget / workers (MGW_SERVE_KEEP_BUFFERS=0: freed after
+ * every job; results unchanged).  Jobs are admitted in arrival order while the bytes reserved
  * by running jobs plus the next job's estimate fit hbm_budget; batch_freqs
  * caps the frequencies factored at once per job (mgw_hip_set_batch_limit).
  * Mixed-precision jobs run under the error judge with FP64 fallback
