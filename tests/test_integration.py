@@ -16,9 +16,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = "/root/reference"
 GOLD = os.path.join(REPO, "tests", "golden")
 LIB = os.path.join(REPO, "minigw_amd", "lib", "libmgw_hip.so")
-sys.path.insert(0, os.path.join(REPO, "oracle"))
-
-import build_integration as bi  # noqa: E402
+from oracle import build_integration as bi  # noqa: E402
 
 needs_tree = pytest.mark.skipif(not (os.path.isdir(os.path.join(REF, "src")) and os.path.exists(LIB)),
                                 reason="reference tree or built library absent")
